@@ -1,0 +1,256 @@
+// internal.hpp -- shared declarations of the gfx950 SPH library (not part of
+// the public C-ABI; see include/fdps_sph_gpu.h for that).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace sphgpu {
+
+// ------------------------------------------------------------------ errors
+struct Error : public std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define SPH_HIP(call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            throw ::sphgpu::Error(-4, std::string(#call) + ": " + hipGetErrorString(e_) + " at " + \
+                                          __FILE__ + ":" + std::to_string(__LINE__));              \
+    } while (0)
+
+#define SPH_LAUNCH_CHECK() SPH_HIP(hipGetLastError())
+
+// ------------------------------------------------------------------ device buffer (grow-only)
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void) hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    T *ensure(size_t count) {
+        if (count <= n && p) return p;
+        release();
+        size_t c = count ? count : 1;
+        SPH_HIP(hipMalloc(&p, c * sizeof(T)));
+        n = c;
+        return p;
+    }
+    T *get() const { return p; }
+};
+
+template <class T>
+struct PinnedBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf &) = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+    ~PinnedBuf() {
+        if (p) (void) hipHostFree(p);
+    }
+    T *ensure(size_t count) {
+        if (count <= n && p) return p;
+        if (p) (void) hipHostFree(p);
+        p = nullptr;
+        size_t c = count ? count : 1;
+        SPH_HIP(hipHostMalloc(&p, c * sizeof(T), hipHostMallocDefault));
+        n = c;
+        return p;
+    }
+};
+
+// ------------------------------------------------------------------ constants
+// src/param.h:9-17, src/kernel.h:36-38, src/force.h:171-172, src/mathfunc.h:4
+constexpr double kSMTH = 1.2;
+constexpr double kCFL = 0.3;
+constexpr double kAV = 1.0;
+constexpr double kSupport = 2.5;
+constexpr double kG = 6.67e-11;
+constexpr double kPi = 3.14159265358979323846;
+// relative widening of the squared support radius in the exact skip test: a
+// pair with r^2 above (2.5h)^2 (1 + 1e-10) has W = gradW = 0 exactly
+constexpr double kCutWiden = 1.0 + 1.0e-10;
+
+constexpr int kWave = 64;
+constexpr int kMaxLevels = 24;
+
+// ------------------------------------------------------------------ interaction-list reference
+// Group g's j-list is idx[begin(g) .. begin(g)+cnt[g]).  off == nullptr means
+// fixed capacity lists: begin(g) = g * cap.
+struct ListRef {
+    const int32_t *idx;
+    const int64_t *off;
+    const int32_t *cnt;
+    int64_t cap;
+};
+
+// Particle SoA arrays of the device-resident simulation.
+struct PArrays {
+    double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az, *gx, *gy, *gz, *pot, *m, *h, *rho, *u, *p, *cs, *divv,
+        *rotx, *roty, *rotz, *gs, *bal, *du, *dti, *vhx, *vhy, *vhz, *uh;
+    int32_t *mat, *sys, *ngb;
+    int64_t *id;
+};
+
+// ------------------------------------------------------------------ interaction kernel arguments
+struct DensArgs {
+    int ngroups;
+    const int32_t *gstart;
+    const double *xi, *yi, *zi, *mi, *hi;
+    const double *mj, *xj, *yj, *zj;
+    ListRef L;
+    double *rho_out, *h_out;
+    int32_t *ngb_out;
+};
+
+struct DrvtArgs {
+    int ngroups;
+    const int32_t *gstart;
+    const double *xi, *yi, *zi, *vxi, *vyi, *vzi, *hi, *rhoi;
+    const double *csi;  // non-null: also write the Balsara switch (bal)
+    const double *mj, *xj, *yj, *zj, *vxj, *vyj, *vzj;
+    ListRef L;
+    double *divv, *rotx, *roty, *rotz, *gs, *bal;
+};
+
+struct HydroArgs {
+    int ngroups;
+    const int32_t *gstart;
+    const double *xi, *yi, *zi, *vxi, *vyi, *vzi, *hi, *rhoi, *pi, *csi, *gsi, *bali;
+    const double *xj, *yj, *zj, *vxj, *vyj, *vzj, *hj, *rhoj, *pj, *csj, *gsj, *balj, *mj;
+    ListRef L;
+    double *ax, *ay, *az, *du, *dt;
+};
+
+struct GravArgs {
+    int ngroups;
+    const int32_t *gstart;
+    const double *xi, *yi, *zi;
+    const double *eps2i;  // if null, eps2 = h*h from hi
+    const double *hi;
+    const double *xj, *yj, *zj, *mj;  // particles (EPJ)
+    ListRef Lj;
+    const double *xs, *ys, *zs, *ms;  // monopoles (SPJ); may be null
+    ListRef Ls;
+    double *ax, *ay, *az, *pot;
+};
+
+// FP64 (plain) interaction kernels -- kernels_fp64.hip
+void launch_dens_fp64(const DensArgs &a, hipStream_t s);
+void launch_drvt_fp64(const DrvtArgs &a, hipStream_t s);
+void launch_hydro_fp64(const HydroArgs &a, hipStream_t s);
+void launch_grav_fp64(const GravArgs &a, hipStream_t s);
+
+// Mixed-precision interaction kernels -- kernels_mixed.hip
+void launch_dens_mixed(const DensArgs &a, hipStream_t s);
+void launch_drvt_mixed(const DrvtArgs &a, hipStream_t s);
+void launch_hydro_mixed(const HydroArgs &a, hipStream_t s);
+void launch_grav_mixed(const GravArgs &a, hipStream_t s);
+
+// ------------------------------------------------------------------ EoS -- eos.hip
+struct MatDev {
+    int nr, nu;
+    const double *rho;    // nr
+    const double *u;      // nu
+    const double *plane;  // [6][nr][nu]
+    const double *full_rho, *full_S, *full_u;  // row-major cell fields (nr*nu)
+    int sorted_rho;       // full_rho non-decreasing (binary search allowed)
+};
+void launch_eos_pressure(int64_t n, const int32_t *mat, const double *rho, const double *u, const MatDev *mats,
+                         int nmat, double *p, double *cs, hipStream_t s);
+void launch_eos_property(int64_t n, const int32_t *mat, const double *rho, const double *u, const MatDev *mats,
+                         int nmat, int prop, double *out, hipStream_t s);
+void launch_eos_internal_energy(int64_t n, const int32_t *mat, const double *rho, const double *S,
+                                const MatDev *mats, int nmat, int abs_int, double *out, hipStream_t s);
+
+// ------------------------------------------------------------------ tree -- tree.hip
+struct TreeParams {
+    int n_leaf, n_group, nsys, bits, sysbits, nlevels;
+    double theta;
+};
+
+struct Tree {
+    // per-system
+    DevBuf<int32_t> sys_start;  // nsys+1 (host-known, fixed)
+    DevBuf<double> sys_lo, sys_scale;  // 3*nsys, nsys
+    DevBuf<double> part;           // bbox partials
+    DevBuf<int32_t> sys_root;      // nsys
+    // keys and sort
+    DevBuf<uint64_t> key, key_alt;
+    DevBuf<int32_t> perm, iota;
+    DevBuf<uint8_t> sort_tmp, sel_tmp;
+    size_t sort_tmp_bytes = 0, sel_tmp_bytes = 0;
+    DevBuf<int64_t> cnt;
+    DevBuf<uint8_t> flags;
+    DevBuf<int32_t> sel_out, counts;
+    // nodes (levels flattened, level 0 = leaves)
+    int nlevels = 0;
+    int64_t level_off[kMaxLevels + 1] = {0};
+    int64_t nnodes = 0;
+    DevBuf<int32_t> nstart, nend, c0, c1;
+    DevBuf<double> mass, mx, my, mz, comx, comy, comz, lox, loy, loz, hix, hiy, hiz, maxH;
+    // i-groups
+    int64_t ngroups = 0;
+    DevBuf<int32_t> gstart;
+};
+
+void tree_sort_and_build(Tree &t, const TreeParams &tp, int64_t n, PArrays &P, PArrays &Palt, hipStream_t s);
+void tree_build_only(Tree &t, const TreeParams &tp, int64_t n, const PArrays &P, hipStream_t s);
+void tree_update_maxH(Tree &t, const double *h, hipStream_t s);
+
+struct Lists {
+    DevBuf<int32_t> idx, cnt;
+    DevBuf<int32_t> sidx, scnt;
+    int64_t cap = 0, scap = 0;
+    DevBuf<int32_t> frontier;
+    DevBuf<int32_t> err;
+};
+// kind 0 gather, 1 symmetric.  Returns false on capacity overflow.
+bool build_lists_sph(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, hipStream_t s);
+bool build_lists_grav(const Tree &t, const TreeParams &tp, const PArrays &P, Lists &L, hipStream_t s);
+int64_t lists_max_count(const Lists &L, int64_t ngroups, bool spj, hipStream_t s);
+
+// ------------------------------------------------------------------ integrator -- integrate.hip
+void launch_init_h(int64_t n, PArrays P, hipStream_t s);
+void launch_kick_drift(int64_t n, PArrays P, const double *dt_sys, hipStream_t s);
+void launch_time_advance(int nsys, double *time_sys, const double *dt_sys, hipStream_t s);
+void launch_finalize(int64_t n, PArrays P, const double *dt_sys, const double *time_sys, double until,
+                     double damping, hipStream_t s);
+void launch_ext_force(int64_t n, PArrays P, const double *time_sys, double until, hipStream_t s);
+void launch_dt_reduce(int nsys, const int32_t *sys_start_host, const int32_t *sys_start, int64_t n,
+                      const double *dti, double *part, double *dt_sys, hipStream_t s);
+
+// ------------------------------------------------------------------ device helpers
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+    // contiguous range of groups so that neighbouring groups share its L2.
+    const int q = n >> 3, r = n & 7;
+    const int x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
+__device__ __forceinline__ int64_t list_begin(const ListRef &L, int g) {
+    return L.off ? L.off[g] : (int64_t) g * L.cap;
+}
+
+template <class T>
+__device__ __forceinline__ T dmax(T a, T b) { return (a < b) ? b : a; }  // std::max
+template <class T>
+__device__ __forceinline__ T dmin(T a, T b) { return (b < a) ? b : a; }  // std::min
+
+}  // namespace sphgpu

@@ -1,0 +1,345 @@
+// kernels_fp64.hip -- FP64 interaction kernels (the "plain" path).
+//
+// One 64-lane wavefront per i-group: lane l owns i-particle gstart[g]+l, and
+// the group's j-list is streamed through LDS in tiles of 64 (each lane
+// gathers one j, then every lane reads the tile back as an LDS broadcast).
+// The j-order is the list order, and every expression keeps the operand
+// order of src/force.h / src/kernel.h; this file is compiled with
+// -ffp-contract=off so no multiply-add is fused.  Pairs outside the support
+// radius (r^2 > (2.5h)^2 (1+1e-10)) are skipped: there W and gradW are
+// exactly zero, so the skip leaves every sum bit-identical.
+#include "internal.hpp"
+
+namespace sphgpu {
+namespace {
+
+__device__ __forceinline__ double plus_(double a) { return (a > 0) ? a : 0; }
+__device__ __forceinline__ double pow7_(double a) { const double a3 = a * a * a; return a3 * a3 * a; }
+__device__ __forceinline__ double pow8_(double a) { const double a2 = a * a; const double a4 = a2 * a2; return a4 * a4; }
+
+// WendlandC6 normalisation (1365/64)/(H^3 pi) (src/kernel.h:16)
+__device__ __forceinline__ double wnorm_(double H) { return (1365. / 64.) / (H * H * H * kPi); }
+
+// WendlandC6::W(r) with s = r/H (src/kernel.h:8-19)
+__device__ __forceinline__ double W_(double s, double norm) {
+    double r = (1.0 + s * (8.0 + s * (25.0 + s * (32.0)))) * pow8_(plus_(1.0 - s));
+    r *= norm;
+    return r;
+}
+
+// WendlandC6::gradW scalar factor: gradW = dr * f (src/kernel.h:22-34); the
+// vector division of FDPS is a multiplication by the reciprocal.
+__device__ __forceinline__ double gradW_factor_(double s, double r, double H, double h, double norm, double &inv) {
+    double rv = pow7_(plus_(1.0 - s)) *
+                (plus_(1.0 - s) * (8.0 + s * (50.0 + s * (96.0))) - 8.0 * (1.0 + s * (8.0 + s * (25.0 + s * (32.0)))));
+    rv *= norm;
+    inv = 1.0 / (r * H + 1.0e-6 * h);
+    return rv;
+}
+
+// ---------------------------------------------------------------- density
+// CalcDensity::operator() (src/force.h:9-23)
+__global__ __launch_bounds__(kWave) void k_dens(DensArgs a) {
+    const int g = xcd_remap(blockIdx.x, a.ngroups);
+    const int lane = threadIdx.x;
+    const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
+    const bool act = lane < ni;
+    const int ip = i0 + (act ? lane : 0);
+    const double xi = a.xi[ip], yi = a.yi[ip], zi = a.zi[ip], hi = a.hi[ip], mi = a.mi[ip];
+    const double H = kSupport * hi;
+    const double norm = wnorm_(H);
+    const double rcut2 = act ? (H * H) * kCutWiden : -1.0;
+    double dens = 0.0;
+    int ngb = 0;
+    const int64_t lb = list_begin(a.L, g);
+    const int nj = a.L.cnt[g];
+    __shared__ double sm[kWave], sx[kWave], sy[kWave], sz[kWave];
+    for (int t = 0; t < nj; t += kWave) {
+        const int k = t + lane;
+        if (k < nj) {
+            const int q = a.L.idx[lb + k];
+            sm[lane] = a.mj[q];
+            sx[lane] = a.xj[q];
+            sy[lane] = a.yj[q];
+            sz[lane] = a.zj[q];
+        }
+        __syncthreads();
+        const int nt = min(kWave, nj - t);
+        for (int jj = 0; jj < nt; ++jj) {
+            const double dx = sx[jj] - xi, dy = sy[jj] - yi, dz = sz[jj] - zi;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 <= rcut2) {
+                const double s = sqrt(r2) / H;
+                dens += sm[jj] * W_(s, norm);
+                ngb += (plus_(1.0 - s) > 0) ? 1 : 0;
+            }
+        }
+        __syncthreads();
+    }
+    if (act) {
+        dens = dmax(5.0, dens);  // FLAG_GI floor (src/force.h:18-20)
+        a.rho_out[ip] = dens;
+        a.h_out[ip] = kSMTH * pow(mi / dens, 1.0 / 3.0);
+        if (a.ngb_out) a.ngb_out[ip] = ngb;
+    }
+}
+
+// ---------------------------------------------------------------- derivative
+// CalcDerivative::operator() (src/force.h:89-106) + Balsara switch of
+// RealPtcl::copyFromForce (src/class.h:132-142) when csi is given.
+__global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
+    const int g = xcd_remap(blockIdx.x, a.ngroups);
+    const int lane = threadIdx.x;
+    const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
+    const bool act = lane < ni;
+    const int ip = i0 + (act ? lane : 0);
+    const double xi = a.xi[ip], yi = a.yi[ip], zi = a.zi[ip];
+    const double vxi = a.vxi[ip], vyi = a.vyi[ip], vzi = a.vzi[ip];
+    const double hi = a.hi[ip], rhoi = a.rhoi[ip];
+    const double H = kSupport * hi;
+    const double norm = wnorm_(H);
+    const double rcut2 = act ? (H * H) * kCutWiden : -1.0;
+    double divv = 0.0, rx = 0.0, ry = 0.0, rz = 0.0, gs = 0.0;
+    const int64_t lb = list_begin(a.L, g);
+    const int nj = a.L.cnt[g];
+    __shared__ double sm[kWave], sx[kWave], sy[kWave], sz[kWave], svx[kWave], svy[kWave], svz[kWave];
+    for (int t = 0; t < nj; t += kWave) {
+        const int k = t + lane;
+        if (k < nj) {
+            const int q = a.L.idx[lb + k];
+            sm[lane] = a.mj[q];
+            sx[lane] = a.xj[q];
+            sy[lane] = a.yj[q];
+            sz[lane] = a.zj[q];
+            svx[lane] = a.vxj[q];
+            svy[lane] = a.vyj[q];
+            svz[lane] = a.vzj[q];
+        }
+        __syncthreads();
+        const int nt = min(kWave, nj - t);
+        for (int jj = 0; jj < nt; ++jj) {
+            const double drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
+            const double r2 = drx * drx + dry * dry + drz * drz;
+            if (r2 <= rcut2) {
+                const double dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
+                const double r = sqrt(r2);
+                const double s = r / H;
+                double inv;
+                const double f = gradW_factor_(s, r, H, hi, norm, inv);
+                const double gwx = (drx * f) * inv, gwy = (dry * f) * inv, gwz = (drz * f) * inv;
+                const double w = W_(s, norm);
+                const double mj = sm[jj];
+                const double mneg = -mj;
+                const double ax = mneg * dvx, ay = mneg * dvy, az = mneg * dvz;
+                divv += ax * gwx + ay * gwy + az * gwz;
+                rx += ay * gwz - az * gwy;
+                ry += az * gwx - ax * gwz;
+                rz += ax * gwy - ay * gwx;
+                gs -= mj / hi * (3.0 * w + (drx * gwx + dry * gwy + drz * gwz));
+            }
+        }
+        __syncthreads();
+    }
+    if (act) {
+        gs = 1.0 / (1.0 + hi * gs / (3.0 * rhoi));
+        divv /= rhoi;
+        const double inv = 1.0 / rhoi;
+        rx *= inv;
+        ry *= inv;
+        rz *= inv;
+        a.divv[ip] = divv;
+        a.rotx[ip] = rx;
+        a.roty[ip] = ry;
+        a.rotz[ip] = rz;
+        a.gs[ip] = gs;
+        if (a.csi)
+            a.bal[ip] = fabs(divv) / (fabs(divv) + sqrt(rx * rx + ry * ry + rz * rz) + 1.0e-4 * a.csi[ip] / hi);
+    }
+}
+
+// ---------------------------------------------------------------- hydro force
+// CalcHydroForce::operator() (src/force.h:112-147), `#if 1` branch.  v_sig_max
+// runs over every j of the list; the force terms only inside either support.
+__global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
+    const int g = xcd_remap(blockIdx.x, a.ngroups);
+    const int lane = threadIdx.x;
+    const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
+    const bool act = lane < ni;
+    const int ip = i0 + (act ? lane : 0);
+    const double xi = a.xi[ip], yi = a.yi[ip], zi = a.zi[ip];
+    const double vxi = a.vxi[ip], vyi = a.vyi[ip], vzi = a.vzi[ip];
+    const double hi = a.hi[ip], rhoi = a.rhoi[ip], pi = a.pi[ip], csi = a.csi[ip], gsi = a.gsi[ip],
+                 bali = a.bali[ip];
+    const double Hi = kSupport * hi;
+    const double normi = wnorm_(Hi);
+    const double rcut2i = act ? (Hi * Hi) * kCutWiden : -1.0;
+    const double Ai = gsi * pi / (rhoi * rhoi);
+    double ax = 0.0, ay = 0.0, az = 0.0, du = 0.0, vmax = 0.0;
+    const int64_t lb = list_begin(a.L, g);
+    const int nj = a.L.cnt[g];
+    __shared__ double sx[kWave], sy[kWave], sz[kWave], svx[kWave], svy[kWave], svz[kWave], sm[kWave];
+    __shared__ double srho[kWave], scs[kWave], sbal[kWave], sgs[kWave], sA[kWave], sh[kWave], sH[kWave],
+        snorm[kWave], scut[kWave];
+    for (int t = 0; t < nj; t += kWave) {
+        const int k = t + lane;
+        if (k < nj) {
+            const int q = a.L.idx[lb + k];
+            sx[lane] = a.xj[q];
+            sy[lane] = a.yj[q];
+            sz[lane] = a.zj[q];
+            svx[lane] = a.vxj[q];
+            svy[lane] = a.vyj[q];
+            svz[lane] = a.vzj[q];
+            sm[lane] = a.mj[q];
+            const double rj = a.rhoj[q], gj = a.gsj[q], hj = a.hj[q];
+            srho[lane] = rj;
+            scs[lane] = a.csj[q];
+            sbal[lane] = a.balj[q];
+            sgs[lane] = gj;
+            sA[lane] = gj * a.pj[q] / (rj * rj);
+            sh[lane] = hj;
+            const double Hj = kSupport * hj;
+            sH[lane] = Hj;
+            snorm[lane] = wnorm_(Hj);
+            scut[lane] = (Hj * Hj) * kCutWiden;
+        }
+        __syncthreads();
+        const int nt = min(kWave, nj - t);
+        for (int jj = 0; jj < nt; ++jj) {
+            const double drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
+            const double dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
+            const double dvdr = dvx * drx + dvy * dry + dvz * drz;
+            const double r2 = drx * drx + dry * dry + drz * drz;
+            const double w = (dvdr < 0) ? dvdr / sqrt(r2) : 0;
+            const double v_sig = csi + scs[jj] - 3.0 * w;
+            vmax = dmax(vmax, v_sig);
+            if (r2 <= rcut2i || (act && r2 <= scut[jj])) {
+                const double rhoj = srho[jj];
+                const double AV = -kAV * 0.5 * v_sig * w / (0.5 * (rhoi + rhoj)) * 0.5 * (bali + sbal[jj]);
+                const double r = sqrt(r2);
+                double invi, invj;
+                const double fi = gradW_factor_(r / Hi, r, Hi, hi, normi, invi);
+                const double Hj = sH[jj];
+                const double fj = gradW_factor_(r / Hj, r, Hj, sh[jj], snorm[jj], invj);
+                const double gix = (drx * fi) * invi, giy = (dry * fi) * invi, giz = (drz * fi) * invi;
+                const double gjx = (drx * fj) * invj, gjy = (dry * fj) * invj, gjz = (drz * fj) * invj;
+                const double gsj = sgs[jj];
+                const double gx = 0.5 * (gix * gsi + gjx * gsj);
+                const double gy = 0.5 * (giy * gsi + gjy * gsj);
+                const double gz = 0.5 * (giz * gsi + gjz * gsj);
+                const double mj = sm[jj], Aj = sA[jj];
+                ax -= mj * (Ai * gix + Aj * gjx + AV * gx);
+                ay -= mj * (Ai * giy + Aj * gjy + AV * gy);
+                az -= mj * (Ai * giz + Aj * gjz + AV * gz);
+                const double sc = mj * (Ai + 0.5 * AV);
+                du += (sc * dvx) * gx + (sc * dvy) * gy + (sc * dvz) * gz;
+            }
+        }
+        __syncthreads();
+    }
+    if (act) {
+        a.ax[ip] = ax;
+        a.ay[ip] = ay;
+        a.az[ip] = az;
+        a.du[ip] = du;
+        a.dt[ip] = kCFL * 2.0 * hi / vmax;
+    }
+}
+
+// ---------------------------------------------------------------- gravity
+// CalcGravityForce<EPJ::Grav> then CalcGravityForce<SPJMonopole>
+// (src/force.h:154-168): the particle list first, then the monopoles.
+__device__ __forceinline__ void grav_tile_(const double *sx, const double *sy, const double *sz, const double *sm,
+                                           int nt, double xi, double yi, double zi, double eps2, double &ax,
+                                           double &ay, double &az, double &pot) {
+    for (int jj = 0; jj < nt; ++jj) {
+        const double drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
+        const double dr2 = drx * drx + dry * dry + drz * drz;
+        const double dr_inv = 1.0 / sqrt(dr2 + eps2);
+        const double m = sm[jj];
+        const double m_dr3_inv = m * (dr_inv * dr_inv * dr_inv);
+        ax -= kG * m_dr3_inv * drx;
+        ay -= kG * m_dr3_inv * dry;
+        az -= kG * m_dr3_inv * drz;
+        pot -= kG * m * dr_inv;
+    }
+}
+
+__global__ __launch_bounds__(kWave) void k_grav(GravArgs a) {
+    const int g = xcd_remap(blockIdx.x, a.ngroups);
+    const int lane = threadIdx.x;
+    const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
+    const bool act = lane < ni;
+    const int ip = i0 + (act ? lane : 0);
+    const double xi = a.xi[ip], yi = a.yi[ip], zi = a.zi[ip];
+    double eps2;
+    if (a.eps2i) eps2 = a.eps2i[ip];
+    else { const double h = a.hi[ip]; eps2 = h * h; }
+    double ax = 0.0, ay = 0.0, az = 0.0, pot = 0.0;
+    __shared__ double sx[kWave], sy[kWave], sz[kWave], sm[kWave];
+    {
+        const int64_t lb = list_begin(a.Lj, g);
+        const int nj = a.Lj.cnt[g];
+        for (int t = 0; t < nj; t += kWave) {
+            const int k = t + lane;
+            if (k < nj) {
+                const int q = a.Lj.idx[lb + k];
+                sx[lane] = a.xj[q];
+                sy[lane] = a.yj[q];
+                sz[lane] = a.zj[q];
+                sm[lane] = a.mj[q];
+            }
+            __syncthreads();
+            grav_tile_(sx, sy, sz, sm, min(kWave, nj - t), xi, yi, zi, eps2, ax, ay, az, pot);
+            __syncthreads();
+        }
+    }
+    if (a.ms) {
+        const int64_t lb = list_begin(a.Ls, g);
+        const int ns = a.Ls.cnt[g];
+        for (int t = 0; t < ns; t += kWave) {
+            const int k = t + lane;
+            if (k < ns) {
+                const int q = a.Ls.idx[lb + k];
+                sx[lane] = a.xs[q];
+                sy[lane] = a.ys[q];
+                sz[lane] = a.zs[q];
+                sm[lane] = a.ms[q];
+            }
+            __syncthreads();
+            grav_tile_(sx, sy, sz, sm, min(kWave, ns - t), xi, yi, zi, eps2, ax, ay, az, pot);
+            __syncthreads();
+        }
+    }
+    if (act) {
+        a.ax[ip] = ax;
+        a.ay[ip] = ay;
+        a.az[ip] = az;
+        a.pot[ip] = pot;
+    }
+}
+
+}  // namespace
+
+void launch_dens_fp64(const DensArgs &a, hipStream_t s) {
+    if (a.ngroups <= 0) return;
+    hipLaunchKernelGGL(k_dens, dim3(a.ngroups), dim3(kWave), 0, s, a);
+    SPH_LAUNCH_CHECK();
+}
+void launch_drvt_fp64(const DrvtArgs &a, hipStream_t s) {
+    if (a.ngroups <= 0) return;
+    hipLaunchKernelGGL(k_drvt, dim3(a.ngroups), dim3(kWave), 0, s, a);
+    SPH_LAUNCH_CHECK();
+}
+void launch_hydro_fp64(const HydroArgs &a, hipStream_t s) {
+    if (a.ngroups <= 0) return;
+    hipLaunchKernelGGL(k_hydro, dim3(a.ngroups), dim3(kWave), 0, s, a);
+    SPH_LAUNCH_CHECK();
+}
+void launch_grav_fp64(const GravArgs &a, hipStream_t s) {
+    if (a.ngroups <= 0) return;
+    hipLaunchKernelGGL(k_grav, dim3(a.ngroups), dim3(kWave), 0, s, a);
+    SPH_LAUNCH_CHECK();
+}
+
+}  // namespace sphgpu

@@ -1,0 +1,784 @@
+// tree.hip -- on-device replacement of FDPS's tree / i-group / interaction-list
+// construction (FDPS v5.0c, used at src/main.cpp:79-91,139-146; not vendored).
+//
+// Particles are sorted by (system, 63-bit Morton key).  For each adjacent
+// pair (k-1, k) the particle count of the smallest octree cell holding both
+// is computed by two binary searches on the sorted keys; a node boundary at
+// threshold T sits exactly where that count exceeds T.  Leaves use
+// T = n_leaf, tree level l uses n_leaf * 8^l, the top level holds one node
+// per system, and i-groups use T = n_group.  Every such node is an octree cell
+// (or a run of equal keys), nodes nest level by level, and node moments are
+// summed sequentially in index order so that the CPU oracle reproduces them
+// bit for bit.  This file is compiled with -ffp-contract=off.
+#include <hipcub/hipcub.hpp>
+
+#include "internal.hpp"
+
+namespace sphgpu {
+namespace {
+
+constexpr int kChunk = 4096;       // bbox partial-reduction chunk
+constexpr int kListSlots = 2048;   // persistent list-builder workgroups
+constexpr int64_t kFrontier = 16384;  // per-workgroup BFS frontier capacity
+
+__device__ __forceinline__ uint64_t expand_bits(uint64_t v) {
+    v &= 0x1fffffULL;
+    v = (v | (v << 32)) & 0x1f00000000ffffULL;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffULL;
+    v = (v | (v << 8)) & 0x100f00f00f00f00fULL;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ULL;
+    v = (v | (v << 2)) & 0x1249249249249249ULL;
+    return v;
+}
+
+__device__ __forceinline__ uint64_t quantise(double x, double lo, double scale, uint64_t qmax) {
+    const double f = (x - lo) * scale;
+    uint64_t q = (f > 0) ? (uint64_t) f : 0;
+    return q > qmax ? qmax : q;
+}
+
+// ---- per-system bounding cube
+__global__ void k_bbox_partial(const int32_t *sys_start, int nchunk, const double *x, const double *y,
+                               const double *z, double *part) {
+    const int s = blockIdx.y, c = blockIdx.x;
+    const int b = sys_start[s] + c * kChunk;
+    const int e = min(sys_start[s + 1], b + kChunk);
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int p = b + threadIdx.x; p < e; p += blockDim.x) {
+        const double v[3] = {x[p], y[p], z[p]};
+        for (int k = 0; k < 3; ++k) { lo[k] = dmin(lo[k], v[k]); hi[k] = dmax(hi[k], v[k]); }
+    }
+    __shared__ double sh[6][256];
+    for (int k = 0; k < 3; ++k) { sh[k][threadIdx.x] = lo[k]; sh[3 + k][threadIdx.x] = hi[k]; }
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int) threadIdx.x < o) {
+            for (int k = 0; k < 3; ++k) {
+                sh[k][threadIdx.x] = dmin(sh[k][threadIdx.x], sh[k][threadIdx.x + o]);
+                sh[3 + k][threadIdx.x] = dmax(sh[3 + k][threadIdx.x], sh[3 + k][threadIdx.x + o]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; ++k) part[((int64_t) s * nchunk + c) * 6 + k] = sh[k][0];
+}
+
+__global__ void k_bbox_final(int nchunk, int bits, const double *part, double *lo_out, double *scale_out) {
+    const int s = blockIdx.x;
+    __shared__ double sh[6][256];
+    double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int c = threadIdx.x; c < nchunk; c += blockDim.x) {
+        const double *q = part + ((int64_t) s * nchunk + c) * 6;
+        for (int k = 0; k < 3; ++k) { v[k] = dmin(v[k], q[k]); v[3 + k] = dmax(v[3 + k], q[3 + k]); }
+    }
+    for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int) threadIdx.x < o) {
+            for (int k = 0; k < 3; ++k) {
+                sh[k][threadIdx.x] = dmin(sh[k][threadIdx.x], sh[k][threadIdx.x + o]);
+                sh[3 + k][threadIdx.x] = dmax(sh[3 + k][threadIdx.x], sh[3 + k][threadIdx.x + o]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double sd = 0;
+        for (int k = 0; k < 3; ++k) sd = dmax(sd, sh[3 + k][0] - sh[k][0]);
+        const double side = (sd > 0) ? sd : 1.0;
+        for (int k = 0; k < 3; ++k) lo_out[3 * s + k] = sh[k][0];
+        scale_out[s] = (double) (1ULL << bits) / side;
+    }
+}
+
+__global__ void k_keys(int64_t n, const double *x, const double *y, const double *z, const int32_t *sys, int bits,
+                       const double *lo, const double *scale, uint64_t *key, int32_t *val) {
+    const int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int s = sys[p];
+    const uint64_t qmax = (1ULL << bits) - 1;
+    const double sc = scale[s];
+    const uint64_t ix = quantise(x[p], lo[3 * s + 0], sc, qmax);
+    const uint64_t iy = quantise(y[p], lo[3 * s + 1], sc, qmax);
+    const uint64_t iz = quantise(z[p], lo[3 * s + 2], sc, qmax);
+    key[p] = ((uint64_t) s << (3 * bits)) | (expand_bits(ix) << 2) | (expand_bits(iy) << 1) | expand_bits(iz);
+    val[p] = (int32_t) p;
+}
+
+struct PermArgs {
+    const int32_t *perm;
+    const double *src[13];
+    double *dst[13];
+    const int32_t *srci[2];
+    int32_t *dsti[2];
+    const int64_t *srcl;
+    int64_t *dstl;
+};
+
+__global__ void k_permute(int64_t n, PermArgs a) {
+    const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int32_t q = a.perm[k];
+#pragma unroll
+    for (int f = 0; f < 13; ++f) a.dst[f][k] = a.src[f][q];
+    a.dsti[0][k] = a.srci[0][q];
+    a.dsti[1][k] = a.srci[1][q];
+    a.dstl[k] = a.srcl[q];
+}
+
+__device__ __forceinline__ int64_t lb_u64(const uint64_t *a, int64_t n, uint64_t v) {
+    int64_t first = 0, len = n;
+    while (len > 0) {
+        const int64_t half = len >> 1;
+        if (a[first + half] < v) { first += half + 1; len = len - half - 1; }
+        else len = half;
+    }
+    return first;
+}
+__device__ __forceinline__ int64_t ub_u64(const uint64_t *a, int64_t n, uint64_t v) {
+    int64_t first = 0, len = n;
+    while (len > 0) {
+        const int64_t half = len >> 1;
+        if (!(v < a[first + half])) { first += half + 1; len = len - half - 1; }
+        else len = half;
+    }
+    return first;
+}
+__device__ __forceinline__ int lb_i32(const int32_t *a, int n, int32_t v) {
+    int first = 0, len = n;
+    while (len > 0) {
+        const int half = len >> 1;
+        if (a[first + half] < v) { first += half + 1; len = len - half - 1; }
+        else len = half;
+    }
+    return first;
+}
+
+// count of the smallest octree cell holding particles k-1 and k
+__global__ void k_lca(int64_t n, const uint64_t *key, int bits, int sysbits, int64_t *cnt) {
+    const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    if (k == 0) { cnt[0] = INT64_MAX; return; }
+    const uint64_t a = key[k - 1], b = key[k];
+    if ((a >> (3 * bits)) != (b >> (3 * bits))) { cnt[k] = INT64_MAX; return; }
+    const uint64_t xo = a ^ b;
+    const int p = (xo == 0) ? 63 : (__clzll((long long) xo) - 1);
+    const int unused = 63 - sysbits - 3 * bits;
+    int lev = (p - unused - sysbits) / 3;
+    if (lev > bits) lev = bits;
+    const int shift = 3 * (bits - lev);
+    const uint64_t mask = (shift >= 64) ? ~0ULL : ((1ULL << shift) - 1);
+    const uint64_t clo = a & ~mask, chi = clo | mask;
+    cnt[k] = ub_u64(key, n, chi) - lb_u64(key, n, clo);
+}
+
+__global__ void k_flags(int64_t n, const int64_t *cnt, int64_t T, uint8_t *flags) {
+    const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    flags[k] = (cnt[k] > T) ? 1 : 0;
+}
+
+// node ranges and children of one level
+__global__ void k_node_ranges(int64_t cnt_l, int64_t off_l, int64_t n, int32_t *nstart, int32_t *nend,
+                              int32_t *c0, int32_t *c1, int64_t cnt_below, int64_t off_below) {
+    const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= cnt_l) return;
+    const int64_t id = off_l + q;
+    const int32_t st = nstart[id];
+    const int32_t en = (q + 1 < cnt_l) ? nstart[id + 1] : (int32_t) n;
+    nend[id] = en;
+    if (off_below < 0) { c0[id] = -1; c1[id] = -1; return; }
+    const int32_t *below = nstart + off_below;
+    c0[id] = (int32_t) (off_below + lb_i32(below, (int) cnt_below, st));
+    c1[id] = (int32_t) (off_below + lb_i32(below, (int) cnt_below, en));
+}
+
+struct MomArgs {
+    const int32_t *nstart, *nend, *c0, *c1;
+    double *mass, *mx, *my, *mz, *comx, *comy, *comz, *lox, *loy, *loz, *hix, *hiy, *hiz, *maxH;
+};
+
+__global__ void k_moments_leaf(int64_t cnt_l, MomArgs a, const double *x, const double *y, const double *z,
+                               const double *m, const double *h) {
+    const int64_t id = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= cnt_l) return;
+    double M = 0, sx = 0, sy = 0, sz = 0, mh = 0;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int p = a.nstart[id]; p < a.nend[id]; ++p) {
+        const double mp = m[p];
+        const double c[3] = {x[p], y[p], z[p]};
+        M += mp;
+        sx += mp * c[0];
+        sy += mp * c[1];
+        sz += mp * c[2];
+        for (int k = 0; k < 3; ++k) { lo[k] = dmin(lo[k], c[k]); hi[k] = dmax(hi[k], c[k]); }
+        mh = dmax(mh, kSupport * h[p]);
+    }
+    a.mass[id] = M;
+    a.mx[id] = sx; a.my[id] = sy; a.mz[id] = sz;
+    a.comx[id] = sx / M; a.comy[id] = sy / M; a.comz[id] = sz / M;
+    a.lox[id] = lo[0]; a.loy[id] = lo[1]; a.loz[id] = lo[2];
+    a.hix[id] = hi[0]; a.hiy[id] = hi[1]; a.hiz[id] = hi[2];
+    a.maxH[id] = mh;
+}
+
+__global__ void k_moments_level(int64_t cnt_l, int64_t off_l, MomArgs a) {
+    const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= cnt_l) return;
+    const int64_t id = off_l + q;
+    double M = 0, sx = 0, sy = 0, sz = 0, mh = 0;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int c = a.c0[id]; c < a.c1[id]; ++c) {
+        M += a.mass[c];
+        sx += a.mx[c];
+        sy += a.my[c];
+        sz += a.mz[c];
+        lo[0] = dmin(lo[0], a.lox[c]); lo[1] = dmin(lo[1], a.loy[c]); lo[2] = dmin(lo[2], a.loz[c]);
+        hi[0] = dmax(hi[0], a.hix[c]); hi[1] = dmax(hi[1], a.hiy[c]); hi[2] = dmax(hi[2], a.hiz[c]);
+        mh = dmax(mh, a.maxH[c]);
+    }
+    a.mass[id] = M;
+    a.mx[id] = sx; a.my[id] = sy; a.mz[id] = sz;
+    a.comx[id] = sx / M; a.comy[id] = sy / M; a.comz[id] = sz / M;
+    a.lox[id] = lo[0]; a.loy[id] = lo[1]; a.loz[id] = lo[2];
+    a.hix[id] = hi[0]; a.hiy[id] = hi[1]; a.hiz[id] = hi[2];
+    a.maxH[id] = mh;
+}
+
+__global__ void k_maxH_leaf(int64_t cnt_l, const int32_t *nstart, const int32_t *nend, const double *h, double *maxH) {
+    const int64_t id = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= cnt_l) return;
+    double mh = 0;
+    for (int p = nstart[id]; p < nend[id]; ++p) mh = dmax(mh, kSupport * h[p]);
+    maxH[id] = mh;
+}
+
+__global__ void k_maxH_level(int64_t cnt_l, int64_t off_l, const int32_t *c0, const int32_t *c1, double *maxH) {
+    const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= cnt_l) return;
+    const int64_t id = off_l + q;
+    double mh = 0;
+    for (int c = c0[id]; c < c1[id]; ++c) mh = dmax(mh, maxH[c]);
+    maxH[id] = mh;
+}
+
+__global__ void k_sys_root(int64_t cnt_top, int64_t off_top, const int32_t *nstart, const int32_t *sys,
+                           int32_t *sys_root) {
+    const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= cnt_top) return;
+    sys_root[sys[nstart[off_top + q]]] = (int32_t) (off_top + q);
+}
+
+__global__ void k_fill_i32(int64_t n, int32_t *a, int32_t v) {
+    const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] = v;
+}
+__global__ void k_iota(int64_t n, int32_t *a) {
+    const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] = (int32_t) k;
+}
+
+// ------------------------------------------------------------------ lists
+struct ListArgs {
+    int ngroups;
+    const int32_t *gstart, *sys;
+    const double *x, *y, *z, *h;
+    int nlevels;
+    const int32_t *sys_root, *nstart, *nend, *c0, *c1;
+    const double *lox, *loy, *loz, *hix, *hiy, *hiz, *comx, *comy, *comz, *maxH;
+    int32_t *out, *cnt;
+    int64_t cap;
+    int32_t *sout, *scnt;
+    int64_t scap;
+    int32_t *frontier;
+    int64_t fcap;
+    double theta2;
+    int32_t *err;
+};
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int t = __shfl_up(v, o, kWave);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = dmin(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = dmax(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+struct GBox {
+    double slo[3], shi[3], glo[3], ghi[3];
+};
+
+__device__ __forceinline__ void group_box(const ListArgs &a, int i0, int ni, GBox &b) {
+    const int lane = threadIdx.x;
+    const bool act = lane < ni;
+    const int p = i0 + (act ? lane : 0);
+    const double c[3] = {a.x[p], a.y[p], a.z[p]};
+    const double H = kSupport * a.h[p];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        b.glo[k] = wave_min(act ? c[k] : INFINITY);
+        b.ghi[k] = wave_max(act ? c[k] : -INFINITY);
+        b.slo[k] = wave_min(act ? c[k] - H : INFINITY);
+        b.shi[k] = wave_max(act ? c[k] + H : -INFINITY);
+    }
+}
+
+template <int KIND>
+__device__ __forceinline__ bool node_test(const ListArgs &a, int id, const GBox &b) {
+    const double lo[3] = {a.lox[id], a.loy[id], a.loz[id]}, hi[3] = {a.hix[id], a.hiy[id], a.hiz[id]};
+    bool ov = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ov = ov & (lo[k] <= b.shi[k]) & (hi[k] >= b.slo[k]);
+    if (ov || KIND == 0) return ov;
+    const double mh = a.maxH[id];
+    bool ov2 = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ov2 = ov2 & (lo[k] - mh <= b.ghi[k]) & (hi[k] + mh >= b.glo[k]);
+    return ov2;
+}
+
+template <int KIND>
+__device__ __forceinline__ bool part_test(const ListArgs &a, int p, const GBox &b) {
+    const double c[3] = {a.x[p], a.y[p], a.z[p]};
+    bool in = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) in = in & (b.slo[k] <= c[k]) & (c[k] <= b.shi[k]);
+    if (in || KIND == 0) return in;
+    const double H = kSupport * a.h[p];
+    bool ov = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ov = ov & (c[k] - H <= b.ghi[k]) & (c[k] + H >= b.glo[k]);
+    return ov;
+}
+
+// monopole acceptance (see the CPU oracle for the same rule)
+__device__ __forceinline__ bool accept_node(const ListArgs &a, int id, const GBox &b) {
+    const double lo[3] = {a.lox[id], a.loy[id], a.loz[id]}, hi[3] = {a.hix[id], a.hiy[id], a.hiz[id]};
+    bool ov = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ov = ov & (lo[k] <= b.ghi[k]) & (hi[k] >= b.glo[k]);
+    if (ov) return false;
+    const double c[3] = {a.comx[id], a.comy[id], a.comz[id]};
+    double d[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) d[k] = (c[k] < b.glo[k]) ? (b.glo[k] - c[k]) : ((c[k] > b.ghi[k]) ? (c[k] - b.ghi[k]) : 0.0);
+    const double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    double sz = hi[0] - lo[0];
+    sz = dmax(sz, hi[1] - lo[1]);
+    sz = dmax(sz, hi[2] - lo[2]);
+    return (sz * sz) < (a.theta2 * d2);
+}
+
+// groups of the XCD this workgroup runs on, walked by the workgroups of that XCD
+struct GroupRange {
+    int beg, end, step, first;
+};
+__device__ __forceinline__ GroupRange xcd_groups(int ngroups) {
+    const int x = blockIdx.x & 7, kb = blockIdx.x >> 3;
+    const int nbx = ((int) gridDim.x - x + 7) >> 3;
+    const int q = ngroups >> 3, r = ngroups & 7;
+    GroupRange R;
+    R.beg = x * q + (x < r ? x : r);
+    R.end = R.beg + q + (x < r ? 1 : 0);
+    R.step = nbx;
+    R.first = R.beg + kb;
+    return R;
+}
+
+// expand the frontier of one level: children of the nodes that pass `test`
+template <class Test>
+__device__ __forceinline__ int expand_level(const ListArgs &a, int32_t *fa, int nf, int32_t *fb, Test test) {
+    const int lane = threadIdx.x;
+    int nn = 0;
+    for (int base = 0; base < nf; base += kWave) {
+        const int idx = base + lane;
+        int nc = 0, cfirst = 0;
+        if (idx < nf) {
+            const int node = fa[idx];
+            if (test(node)) { cfirst = a.c0[node]; nc = a.c1[node] - cfirst; }
+        }
+        const int incl = wave_incl_scan(nc);
+        const int tot = __shfl(incl, kWave - 1, kWave);
+        const int pos0 = nn + incl - nc;
+        for (int c = 0; c < nc; ++c)
+            if (pos0 + c < a.fcap) fb[pos0 + c] = cfirst + c;
+        nn += tot;
+    }
+    if (nn > a.fcap) {
+        if (lane == 0) atomicOr(a.err, 4);
+        nn = (int) a.fcap;
+    }
+    __syncthreads();
+    return nn;
+}
+
+// emit the particles of the leaves fa[0..nf) that pass leaf_ok, filtered by
+// part_ok, in leaf order then index order
+template <class LeafOk, class PartOk>
+__device__ __forceinline__ int emit_particles(const ListArgs &a, const int32_t *fa, int nf, int32_t *out, int64_t cap,
+                                              int count, LeafOk leaf_ok, PartOk part_ok, int *s_pref, int *s_leaf) {
+    const int lane = threadIdx.x;
+    const uint64_t lt = (lane == 0) ? 0ULL : ((~0ULL) >> (64 - lane));
+    for (int base = 0; base < nf; base += kWave) {
+        const int idx = base + lane;
+        int sz = 0, leaf = 0;
+        if (idx < nf) {
+            leaf = fa[idx];
+            if (leaf_ok(leaf)) sz = a.nend[leaf] - a.nstart[leaf];
+        }
+        const int incl = wave_incl_scan(sz);
+        const int tot = __shfl(incl, kWave - 1, kWave);
+        s_pref[lane] = incl - sz;
+        s_leaf[lane] = leaf;
+        if (lane == kWave - 1) s_pref[kWave] = incl;
+        __syncthreads();
+        for (int s0 = 0; s0 < tot; s0 += kWave) {
+            const int slot = s0 + lane;
+            bool pass = false;
+            int p = 0;
+            if (slot < tot) {
+                int lo = 0, hi = kWave;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_pref[mid] <= slot) lo = mid;
+                    else hi = mid;
+                }
+                p = a.nstart[s_leaf[lo]] + (slot - s_pref[lo]);
+                pass = part_ok(p);
+            }
+            const uint64_t m = __ballot(pass);
+            if (pass) {
+                const int pos = count + __popcll(m & lt);
+                if (pos < cap) out[pos] = p;
+            }
+            count += __popcll(m);
+        }
+        __syncthreads();
+    }
+    return count;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kWave) void k_list_sph(ListArgs a) {
+    const int lane = threadIdx.x;
+    int32_t *fa0 = a.frontier + (int64_t) blockIdx.x * 2 * a.fcap;
+    int32_t *fb0 = fa0 + a.fcap;
+    __shared__ int s_pref[kWave + 1];
+    __shared__ int s_leaf[kWave];
+    const GroupRange R = xcd_groups(a.ngroups);
+    for (int g = R.first; g < R.end; g += R.step) {
+        const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
+        GBox b;
+        group_box(a, i0, ni, b);
+        int32_t *fa = fa0, *fb = fb0;
+        if (lane == 0) fa[0] = a.sys_root[a.sys[i0]];
+        __syncthreads();
+        int nf = 1;
+        auto test = [&](int node) { return node_test<KIND>(a, node, b); };
+        for (int l = a.nlevels - 1; l >= 1; --l) {
+            nf = expand_level(a, fa, nf, fb, test);
+            int32_t *t = fa; fa = fb; fb = t;
+        }
+        int32_t *out = a.out + (int64_t) g * a.cap;
+        auto ptest = [&](int p) { return part_test<KIND>(a, p, b); };
+        const int count = emit_particles(a, fa, nf, out, a.cap, 0, test, ptest, s_pref, s_leaf);
+        if (lane == 0) {
+            a.cnt[g] = count;
+            if (count > a.cap) atomicOr(a.err, 1);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kWave) void k_list_grav(ListArgs a) {
+    const int lane = threadIdx.x;
+    const uint64_t lt = (lane == 0) ? 0ULL : ((~0ULL) >> (64 - lane));
+    int32_t *fa0 = a.frontier + (int64_t) blockIdx.x * 2 * a.fcap;
+    int32_t *fb0 = fa0 + a.fcap;
+    __shared__ int s_pref[kWave + 1];
+    __shared__ int s_leaf[kWave];
+    const GroupRange R = xcd_groups(a.ngroups);
+    for (int g = R.first; g < R.end; g += R.step) {
+        const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
+        GBox b;
+        group_box(a, i0, ni, b);
+        int32_t *fa = fa0, *fb = fb0;
+        if (lane == 0) fa[0] = a.sys_root[a.sys[i0]];
+        __syncthreads();
+        int nf = 1, cj = 0, cs = 0;
+        int32_t *jout = a.out + (int64_t) g * a.cap;
+        int32_t *sout = a.sout + (int64_t) g * a.scap;
+        for (int l = a.nlevels - 1; l >= 0; --l) {
+            // monopoles accepted at this level, in frontier order
+            for (int base = 0; base < nf; base += kWave) {
+                const int idx = base + lane;
+                const bool acc = (idx < nf) && accept_node(a, fa[idx], b);
+                const uint64_t m = __ballot(acc);
+                if (acc) {
+                    const int pos = cs + __popcll(m & lt);
+                    if (pos < a.scap) sout[pos] = fa[idx];
+                }
+                cs += __popcll(m);
+            }
+            auto open = [&](int node) { return !accept_node(a, node, b); };
+            if (l >= 1) {
+                nf = expand_level(a, fa, nf, fb, open);
+                int32_t *t = fa; fa = fb; fb = t;
+            } else {
+                auto all = [&](int) { return true; };
+                cj = emit_particles(a, fa, nf, jout, a.cap, cj, open, all, s_pref, s_leaf);
+            }
+        }
+        if (lane == 0) {
+            a.cnt[g] = cj;
+            a.scnt[g] = cs;
+            if (cj > a.cap) atomicOr(a.err, 1);
+            if (cs > a.scap) atomicOr(a.err, 2);
+        }
+        __syncthreads();
+    }
+}
+
+inline int nblk(int64_t n, int b) { return (int) ((n + b - 1) / b); }
+
+}  // namespace
+
+// ------------------------------------------------------------------ host side
+static void compute_sys_boxes(Tree &t, const TreeParams &tp, int64_t n, const PArrays &P, hipStream_t s) {
+    (void) n;
+    std::vector<int32_t> ss(tp.nsys + 1);
+    SPH_HIP(hipMemcpy(ss.data(), t.sys_start.get(), sizeof(int32_t) * (tp.nsys + 1), hipMemcpyDeviceToHost));
+    int maxn = 0;
+    for (int q = 0; q < tp.nsys; ++q) maxn = std::max(maxn, ss[q + 1] - ss[q]);
+    const int nchunk = std::max(1, (maxn + kChunk - 1) / kChunk);
+    t.part.ensure((size_t) tp.nsys * nchunk * 6);
+    t.sys_lo.ensure(3 * tp.nsys);
+    t.sys_scale.ensure(tp.nsys);
+    hipLaunchKernelGGL(k_bbox_partial, dim3(nchunk, tp.nsys), dim3(256), 0, s, t.sys_start.get(), nchunk, P.x, P.y,
+                       P.z, t.part.get());
+    SPH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_bbox_final, dim3(tp.nsys), dim3(256), 0, s, nchunk, tp.bits, t.part.get(), t.sys_lo.get(),
+                       t.sys_scale.get());
+    SPH_LAUNCH_CHECK();
+}
+
+static void build_levels(Tree &t, const TreeParams &tp, int64_t n, const PArrays &P, hipStream_t s) {
+    t.cnt.ensure(n);
+    hipLaunchKernelGGL(k_lca, dim3(nblk(n, 256)), dim3(256), 0, s, n, t.key.get(), tp.bits, tp.sysbits, t.cnt.get());
+    SPH_LAUNCH_CHECK();
+    const int L = tp.nlevels;
+    t.flags.ensure(n);
+    t.sel_out.ensure((size_t) (L + 1) * n);
+    t.counts.ensure(L + 1);
+    hipcub::CountingInputIterator<int32_t> it(0);
+    size_t need = 0;
+    SPH_HIP(hipcub::DeviceSelect::Flagged(nullptr, need, it, t.flags.get(), t.sel_out.get(), t.counts.get(), (int) n, s));
+    if (need > t.sel_tmp_bytes || !t.sel_tmp.get()) {
+        t.sel_tmp.ensure(need);
+        t.sel_tmp_bytes = need;
+    }
+    int64_t T = tp.n_leaf;
+    for (int l = 0; l <= L; ++l) {
+        int64_t thr;
+        if (l == L) thr = tp.n_group;
+        else if (l == L - 1) thr = INT64_MAX - 1;
+        else thr = T;
+        hipLaunchKernelGGL(k_flags, dim3(nblk(n, 256)), dim3(256), 0, s, n, t.cnt.get(), thr, t.flags.get());
+        SPH_LAUNCH_CHECK();
+        size_t bytes = t.sel_tmp_bytes;
+        SPH_HIP(hipcub::DeviceSelect::Flagged(t.sel_tmp.get(), bytes, it, t.flags.get(), t.sel_out.get() + (size_t) l * n,
+                                              t.counts.get() + l, (int) n, s));
+        if (l < L - 1) T *= 8;
+    }
+    std::vector<int32_t> cnts(L + 1);
+    SPH_HIP(hipMemcpyAsync(cnts.data(), t.counts.get(), sizeof(int32_t) * (L + 1), hipMemcpyDeviceToHost, s));
+    SPH_HIP(hipStreamSynchronize(s));
+    t.nlevels = L;
+    t.level_off[0] = 0;
+    for (int l = 0; l < L; ++l) t.level_off[l + 1] = t.level_off[l] + cnts[l];
+    t.nnodes = t.level_off[L];
+    const size_t nn = (size_t) t.nnodes;
+    t.nstart.ensure(nn); t.nend.ensure(nn); t.c0.ensure(nn); t.c1.ensure(nn);
+    t.mass.ensure(nn); t.mx.ensure(nn); t.my.ensure(nn); t.mz.ensure(nn);
+    t.comx.ensure(nn); t.comy.ensure(nn); t.comz.ensure(nn);
+    t.lox.ensure(nn); t.loy.ensure(nn); t.loz.ensure(nn);
+    t.hix.ensure(nn); t.hiy.ensure(nn); t.hiz.ensure(nn); t.maxH.ensure(nn);
+    for (int l = 0; l < L; ++l)
+        SPH_HIP(hipMemcpyAsync(t.nstart.get() + t.level_off[l], t.sel_out.get() + (size_t) l * n,
+                               sizeof(int32_t) * cnts[l], hipMemcpyDeviceToDevice, s));
+    t.ngroups = cnts[L];
+    t.gstart.ensure(t.ngroups + 1);
+    SPH_HIP(hipMemcpyAsync(t.gstart.get(), t.sel_out.get() + (size_t) L * n, sizeof(int32_t) * t.ngroups,
+                           hipMemcpyDeviceToDevice, s));
+    const int32_t nn32 = (int32_t) n;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s, (int64_t) 1, t.gstart.get() + t.ngroups, nn32);
+    SPH_LAUNCH_CHECK();
+    for (int l = 0; l < L; ++l) {
+        const int64_t cl = cnts[l];
+        const int64_t offb = (l == 0) ? -1 : t.level_off[l - 1];
+        const int64_t cb = (l == 0) ? 0 : cnts[l - 1];
+        hipLaunchKernelGGL(k_node_ranges, dim3(nblk(cl, 256)), dim3(256), 0, s, cl, t.level_off[l], n, t.nstart.get(),
+                           t.nend.get(), t.c0.get(), t.c1.get(), cb, offb);
+        SPH_LAUNCH_CHECK();
+    }
+    MomArgs ma{t.nstart.get(), t.nend.get(), t.c0.get(), t.c1.get(), t.mass.get(), t.mx.get(), t.my.get(), t.mz.get(),
+               t.comx.get(), t.comy.get(), t.comz.get(), t.lox.get(), t.loy.get(), t.loz.get(), t.hix.get(),
+               t.hiy.get(), t.hiz.get(), t.maxH.get()};
+    hipLaunchKernelGGL(k_moments_leaf, dim3(nblk(cnts[0], 128)), dim3(128), 0, s, (int64_t) cnts[0], ma, P.x, P.y, P.z,
+                       P.m, P.h);
+    SPH_LAUNCH_CHECK();
+    for (int l = 1; l < L; ++l) {
+        hipLaunchKernelGGL(k_moments_level, dim3(nblk(cnts[l], 128)), dim3(128), 0, s, (int64_t) cnts[l],
+                           t.level_off[l], ma);
+        SPH_LAUNCH_CHECK();
+    }
+    t.sys_root.ensure(tp.nsys);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(tp.nsys, 64)), dim3(64), 0, s, (int64_t) tp.nsys, t.sys_root.get(), -1);
+    SPH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sys_root, dim3(nblk(cnts[L - 1], 64)), dim3(64), 0, s, (int64_t) cnts[L - 1],
+                       t.level_off[L - 1], t.nstart.get(), P.sys, t.sys_root.get());
+    SPH_LAUNCH_CHECK();
+}
+
+static void compute_keys(Tree &t, const TreeParams &tp, int64_t n, const PArrays &P, hipStream_t s) {
+    compute_sys_boxes(t, tp, n, P, s);
+    t.key.ensure(n);
+    t.key_alt.ensure(n);
+    t.perm.ensure(n);
+    t.iota.ensure(n);
+    hipLaunchKernelGGL(k_keys, dim3(nblk(n, 256)), dim3(256), 0, s, n, P.x, P.y, P.z, P.sys, tp.bits, t.sys_lo.get(),
+                       t.sys_scale.get(), t.key.get(), t.iota.get());
+    SPH_LAUNCH_CHECK();
+}
+
+void tree_sort_and_build(Tree &t, const TreeParams &tp, int64_t n, PArrays &P, PArrays &Palt, hipStream_t s) {
+    compute_keys(t, tp, n, P, s);
+    size_t need = 0;
+    SPH_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need, t.key.get(), t.key_alt.get(), t.iota.get(), t.perm.get(),
+                                                (int) n, 0, 63, s));
+    if (need > t.sort_tmp_bytes || !t.sort_tmp.get()) {
+        t.sort_tmp.ensure(need);
+        t.sort_tmp_bytes = need;
+    }
+    size_t bytes = t.sort_tmp_bytes;
+    SPH_HIP(hipcub::DeviceRadixSort::SortPairs(t.sort_tmp.get(), bytes, t.key.get(), t.key_alt.get(), t.iota.get(),
+                                                t.perm.get(), (int) n, 0, 63, s));
+    std::swap(t.key.p, t.key_alt.p);
+    std::swap(t.key.n, t.key_alt.n);
+    // carry the fields that survive into the next pass of the step
+    PermArgs pa;
+    pa.perm = t.perm.get();
+    const double *src[13] = {P.x, P.y, P.z, P.vx, P.vy, P.vz, P.vhx, P.vhy, P.vhz, P.uh, P.u, P.m, P.h};
+    double *dst[13] = {Palt.x, Palt.y, Palt.z, Palt.vx, Palt.vy, Palt.vz, Palt.vhx, Palt.vhy, Palt.vhz, Palt.uh, Palt.u,
+                       Palt.m, Palt.h};
+    for (int f = 0; f < 13; ++f) { pa.src[f] = src[f]; pa.dst[f] = dst[f]; }
+    pa.srci[0] = P.mat; pa.dsti[0] = Palt.mat;
+    pa.srci[1] = P.sys; pa.dsti[1] = Palt.sys;
+    pa.srcl = P.id; pa.dstl = Palt.id;
+    hipLaunchKernelGGL(k_permute, dim3(nblk(n, 256)), dim3(256), 0, s, n, pa);
+    SPH_LAUNCH_CHECK();
+    std::swap(P, Palt);
+    build_levels(t, tp, n, P, s);
+}
+
+void tree_build_only(Tree &t, const TreeParams &tp, int64_t n, const PArrays &P, hipStream_t s) {
+    compute_keys(t, tp, n, P, s);
+    // keys of an already sorted state are sorted; no permutation
+    build_levels(t, tp, n, P, s);
+}
+
+void tree_update_maxH(Tree &t, const double *h, hipStream_t s) {
+    const int L = t.nlevels;
+    const int64_t c0n = t.level_off[1] - t.level_off[0];
+    hipLaunchKernelGGL(k_maxH_leaf, dim3(nblk(c0n, 128)), dim3(128), 0, s, c0n, t.nstart.get(), t.nend.get(), h,
+                       t.maxH.get());
+    SPH_LAUNCH_CHECK();
+    for (int l = 1; l < L; ++l) {
+        const int64_t cl = t.level_off[l + 1] - t.level_off[l];
+        hipLaunchKernelGGL(k_maxH_level, dim3(nblk(cl, 128)), dim3(128), 0, s, cl, t.level_off[l], t.c0.get(),
+                           t.c1.get(), t.maxH.get());
+        SPH_LAUNCH_CHECK();
+    }
+}
+
+static ListArgs list_args(const Tree &t, const TreeParams &tp, const PArrays &P, Lists &L) {
+    ListArgs a;
+    a.ngroups = (int) t.ngroups;
+    a.gstart = t.gstart.get();
+    a.sys = P.sys;
+    a.x = P.x; a.y = P.y; a.z = P.z; a.h = P.h;
+    a.nlevels = t.nlevels;
+    a.sys_root = t.sys_root.get();
+    a.nstart = t.nstart.get(); a.nend = t.nend.get(); a.c0 = t.c0.get(); a.c1 = t.c1.get();
+    a.lox = t.lox.get(); a.loy = t.loy.get(); a.loz = t.loz.get();
+    a.hix = t.hix.get(); a.hiy = t.hiy.get(); a.hiz = t.hiz.get();
+    a.comx = t.comx.get(); a.comy = t.comy.get(); a.comz = t.comz.get();
+    a.maxH = t.maxH.get();
+    a.out = L.idx.get(); a.cnt = L.cnt.get(); a.cap = L.cap;
+    a.sout = L.sidx.get(); a.scnt = L.scnt.get(); a.scap = L.scap;
+    a.frontier = L.frontier.get();
+    a.fcap = kFrontier;
+    a.theta2 = tp.theta * tp.theta;
+    a.err = L.err.get();
+    return a;
+}
+
+static int list_grid(int64_t ngroups) {
+    int64_t b = std::min<int64_t>(ngroups, kListSlots);
+    b = (b + 7) & ~7LL;
+    return (int) std::max<int64_t>(8, b);
+}
+
+static bool run_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, hipStream_t s) {
+    const int64_t G = t.ngroups;
+    L.idx.ensure((size_t) G * L.cap);
+    L.cnt.ensure(G);
+    if (kind == 2) {
+        L.sidx.ensure((size_t) G * L.scap);
+        L.scnt.ensure(G);
+    }
+    const int grid = list_grid(G);
+    L.frontier.ensure((size_t) grid * 2 * kFrontier);
+    L.err.ensure(1);
+    SPH_HIP(hipMemsetAsync(L.err.get(), 0, sizeof(int32_t), s));
+    ListArgs a = list_args(t, tp, P, L);
+    if (kind == 0) hipLaunchKernelGGL(k_list_sph<0>, dim3(grid), dim3(kWave), 0, s, a);
+    else if (kind == 1) hipLaunchKernelGGL(k_list_sph<1>, dim3(grid), dim3(kWave), 0, s, a);
+    else hipLaunchKernelGGL(k_list_grav, dim3(grid), dim3(kWave), 0, s, a);
+    SPH_LAUNCH_CHECK();
+    int32_t err = 0;
+    SPH_HIP(hipMemcpyAsync(&err, L.err.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SPH_HIP(hipStreamSynchronize(s));
+    if (err & 4) throw Error(-2, "tree-walk frontier exceeds its capacity (16384 nodes per level)");
+    if (err & 1) { L.cap *= 2; return false; }
+    if (err & 2) { L.scap *= 2; return false; }
+    return true;
+}
+
+bool build_lists_sph(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, hipStream_t s) {
+    for (int attempt = 0; attempt < 8; ++attempt)
+        if (run_lists(t, tp, P, kind, L, s)) return true;
+    throw Error(-2, "interaction list capacity could not be grown enough");
+}
+
+bool build_lists_grav(const Tree &t, const TreeParams &tp, const PArrays &P, Lists &L, hipStream_t s) {
+    for (int attempt = 0; attempt < 8; ++attempt)
+        if (run_lists(t, tp, P, 2, L, s)) return true;
+    throw Error(-2, "gravity list capacity could not be grown enough");
+}
+
+}  // namespace sphgpu
