@@ -39,6 +39,11 @@ class SimParams(C.Structure):
                 ("list_cap_epj", C.c_int32), ("list_cap_spj", C.c_int32)]
 
 
+class KernelStat(C.Structure):
+    _fields_ = [("ms", C.c_double), ("launches", C.c_int64), ("n_i", C.c_int64), ("n_j", C.c_int64),
+                ("n_s", C.c_int64), ("pairs", C.c_int64)]
+
+
 class SphGpuError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"sphgpu error {code}: {msg}")
@@ -85,8 +90,8 @@ def lib() -> C.CDLL:
     L.sphgpu_sim_stage_ms.argtypes = [P, P, i32]
     L.sphgpu_sim_lists.argtypes = [P, i32, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64), P, P, P, P, P]
     L.sphgpu_sim_nodes.argtypes = [P, C.POINTER(i64), P]
-    for name in [n for n in dir(L) if n.startswith("sphgpu_")]:
-        pass
+    L.sphgpu_sim_set_stats.argtypes = [P, i32]
+    L.sphgpu_sim_kernel_stats.argtypes = [P, C.POINTER(KernelStat), i32]
     _LIB = L
     return L
 

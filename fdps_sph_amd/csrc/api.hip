@@ -63,6 +63,32 @@ __global__ void k_iota(int64_t n, int32_t *a) {
     if (k < n) a[k] = (int32_t) k;
 }
 
+// sums of n_i, n_j, n_s and n_i*(n_j+n_s) over the groups of one launch
+__global__ void k_list_counts(int ngroups, const int32_t *gstart, const int32_t *cnt, const int32_t *scnt,
+                              long long *acc) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    long long v[4] = {0, 0, 0, 0};
+    if (g < ngroups) {
+        const long long ni = gstart[g + 1] - gstart[g];
+        const long long nj = cnt[g];
+        const long long ns = scnt ? scnt[g] : 0;
+        v[0] = ni;
+        v[1] = nj;
+        v[2] = ns;
+        v[3] = ni * (nj + ns);
+    }
+    __shared__ long long sh[4][256];
+    for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int) threadIdx.x < o)
+            for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) atomicAdd((unsigned long long *) &acc[k], (unsigned long long) sh[k][0]);
+}
+
 // ------------------------------------------------------------------ batches
 enum Kind { KIND_DENS = 0, KIND_DRVT, KIND_HYDRO, KIND_GRAV };
 
@@ -116,9 +142,19 @@ struct SimState {
     hipEvent_t ev[kStages + 1] = {};
     double stage_ms[kStages] = {0};
     bool ready = false;
+    // per-kernel statistics (density x4, derivative, hydro, gravity per step)
+    bool stats = false;
+    hipEvent_t kev[7][2] = {};
+    int kkind[7] = {0, 0, 0, 0, 1, 2, 3};
+    double kms[4] = {0};
+    int64_t klaunch[4] = {0};
+    DevBuf<long long> kcount;  // [4][4]: n_i, n_j, n_s, pairs
     ~SimState() {
         for (auto &e : ev)
             if (e) (void) hipEventDestroy(e);
+        for (auto &p : kev)
+            for (auto &e : p)
+                if (e) (void) hipEventDestroy(e);
     }
 };
 
@@ -461,13 +497,29 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
     PArrays &P = S.P;
     const int G = (int) S.tree.ngroups;
     const int32_t *gs = S.tree.gstart.get();
+    int kslot = 0;
+    auto kbegin = [&](const Lists &L, bool spj) {
+        if (!S.stats) return;
+        const int kind = S.kkind[kslot];
+        hipLaunchKernelGGL(k_list_counts, dim3(nblk(G, 256)), dim3(256), 0, s, G, gs, L.cnt.get(),
+                           spj ? L.scnt.get() : nullptr, S.kcount.get() + 4 * kind);
+        SPH_LAUNCH_CHECK();
+        SPH_HIP(hipEventRecord(S.kev[kslot][0], s));
+    };
+    auto kend = [&]() {
+        if (!S.stats) return;
+        SPH_HIP(hipEventRecord(S.kev[kslot][1], s));
+        ++kslot;
+    };
     // 4 density passes, each with lists from the current h (src/main.cpp:138-140)
     for (int loop = 0; loop <= 3; ++loop) {
         build_lists_sph(S.tree, S.tp, P, 0, S.lsph, s);
         DensArgs a{G, gs, P.x, P.y, P.z, P.m, P.h, P.m, P.x, P.y, P.z,
                    ListRef{S.lsph.idx.get(), nullptr, S.lsph.cnt.get(), S.lsph.cap}, P.rho, P.h, P.ngb};
+        kbegin(S.lsph, false);
         if (mixed) launch_dens_mixed(a, s);
         else launch_dens_fp64(a, s);
+        kend();
     }
     rec(3);
     launch_eos_pressure(n, P.mat, P.rho, P.u, c->d_mats.get(), SPHGPU_MAX_MATERIALS, P.p, P.cs, s);
@@ -477,8 +529,10 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
         DrvtArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.cs, P.m, P.x, P.y, P.z, P.vx, P.vy, P.vz,
                    ListRef{S.lsph.idx.get(), nullptr, S.lsph.cnt.get(), S.lsph.cap},
                    P.divv, P.rotx, P.roty, P.rotz, P.gs, P.bal};
+        kbegin(S.lsph, false);
         if (mixed) launch_drvt_mixed(a, s);
         else launch_drvt_fp64(a, s);
+        kend();
     }
     rec(5);
     tree_update_maxH(S.tree, P.h, s);
@@ -488,8 +542,10 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
                     P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal, P.m,
                     ListRef{S.lsph.idx.get(), nullptr, S.lsph.cnt.get(), S.lsph.cap},
                     P.ax, P.ay, P.az, P.du, P.dti};
+        kbegin(S.lsph, false);
         if (mixed) launch_hydro_mixed(a, s);
         else launch_hydro_fp64(a, s);
+        kend();
     }
     rec(6);
     build_lists_grav(S.tree, S.tp, P, S.lgrav, s);
@@ -500,10 +556,21 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
                    t.comx.get(), t.comy.get(), t.comz.get(), t.mass.get(),
                    ListRef{S.lgrav.sidx.get(), nullptr, S.lgrav.scnt.get(), S.lgrav.scap},
                    P.gx, P.gy, P.gz, P.pot};
+        kbegin(S.lgrav, true);
         if (mixed) launch_grav_mixed(a, s);
         else launch_grav_fp64(a, s);
+        kend();
     }
     rec(7);
+    if (S.stats) {
+        SPH_HIP(hipStreamSynchronize(s));
+        for (int k = 0; k < kslot; ++k) {
+            float ms = 0;
+            SPH_HIP(hipEventElapsedTime(&ms, S.kev[k][0], S.kev[k][1]));
+            S.kms[S.kkind[k]] += ms;
+            S.klaunch[S.kkind[k]] += 1;
+        }
+    }
 }
 
 void sim_dt(sphgpu_ctx *c, SimState &S) {
@@ -977,6 +1044,42 @@ int32_t sphgpu_sim_stage_ms(sphgpu_ctx *c, double *ms, int32_t n) {
         SimState &S = sim_of(c);
         if (!ms || n < 0) throw Error(SPHGPU_ERR_ARG, "bad output");
         for (int k = 0; k < std::min(n, kStages); ++k) ms[k] = S.stage_ms[k];
+    });
+}
+
+int32_t sphgpu_sim_set_stats(sphgpu_ctx *c, int32_t on) {
+    return guard([&] {
+        check_ctx(c);
+        SimState &S = sim_of(c);
+        SPH_HIP(hipStreamSynchronize(c->stream));
+        S.stats = on != 0;
+        for (auto &p : S.kev)
+            for (auto &e : p)
+                if (!e) SPH_HIP(hipEventCreate(&e));
+        for (int k = 0; k < 4; ++k) {
+            S.kms[k] = 0;
+            S.klaunch[k] = 0;
+        }
+        SPH_HIP(hipMemset(S.kcount.ensure(16), 0, sizeof(long long) * 16));
+    });
+}
+
+int32_t sphgpu_sim_kernel_stats(sphgpu_ctx *c, sphgpu_kernel_stat *out, int32_t n) {
+    return guard([&] {
+        check_ctx(c);
+        SimState &S = sim_of(c);
+        if (!out || n < 0) throw Error(SPHGPU_ERR_ARG, "bad output");
+        SPH_HIP(hipStreamSynchronize(c->stream));
+        long long cnt[16] = {0};
+        if (S.kcount.get()) SPH_HIP(hipMemcpy(cnt, S.kcount.get(), sizeof(cnt), hipMemcpyDeviceToHost));
+        for (int k = 0; k < std::min(n, 4); ++k) {
+            out[k].ms = S.kms[k];
+            out[k].launches = S.klaunch[k];
+            out[k].n_i = cnt[4 * k + 0];
+            out[k].n_j = cnt[4 * k + 1];
+            out[k].n_s = cnt[4 * k + 2];
+            out[k].pairs = cnt[4 * k + 3];
+        }
     });
 }
 

@@ -1,11 +1,384 @@
-// kernels_mixed.hip -- mixed-precision interaction kernels (placeholder until
-// the FP32 group-relative kernels land).
+// kernels_mixed.hip -- mixed-precision interaction kernels (the precision form).
+//
+// Positions and velocities stay FP64 in HBM.  For each i-group the wave takes
+// the centre of the group's position box (and of its velocity box) in FP64;
+// every i and j position / velocity enters the pair loop as an FP32 offset
+// from that centre, formed in FP64 and rounded once, so dr and dv carry
+// relative errors of order 2^-24 of the group's extent rather than of the
+// absolute coordinates.  Pair arithmetic is FP32 with the hardware rsqrt and
+// reciprocal; the kernel polynomials use the algebraically equal forms
+//   W'(s) = -22 s (1 + 7 s + 16 s^2) (1 - s)^7 * norm
+// that avoid the cancellation of the reference's (src/kernel.h:28-29) form.
+// Sums: each 64-j tile is accumulated in FP32 and the tile sums are combined
+// with Kahan compensation.  Finalisation (density floor, smoothing length,
+// grad-h factor, Balsara switch, dt) is FP64.
 #include "internal.hpp"
 
 namespace sphgpu {
-static void not_ready() { throw Error(-6, "mixed-precision kernels are not built yet"); }
-void launch_dens_mixed(const DensArgs &, hipStream_t) { not_ready(); }
-void launch_drvt_mixed(const DrvtArgs &, hipStream_t) { not_ready(); }
-void launch_hydro_mixed(const HydroArgs &, hipStream_t) { not_ready(); }
-void launch_grav_mixed(const GravArgs &, hipStream_t) { not_ready(); }
+namespace {
+
+struct Kahan {
+    float s = 0.f, c = 0.f;
+    __device__ __forceinline__ void add(float x) {
+        const float y = x - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ double value() const { return (double) s - (double) c; }
+};
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = dmin(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = dmax(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+// centre of the box spanned by the active lanes' values
+__device__ __forceinline__ double wave_centre(double v, bool act) {
+    const double lo = wave_min(act ? v : INFINITY);
+    const double hi = wave_max(act ? v : -INFINITY);
+    return 0.5 * (lo + hi);
+}
+
+constexpr float kNorm = (float) (1365.0 / 64.0 / kPi);
+
+// ---------------------------------------------------------------- density
+__global__ __launch_bounds__(kWave) void k_dens_mixed(DensArgs a) {
+    const int g = xcd_remap(blockIdx.x, a.ngroups);
+    const int lane = threadIdx.x;
+    const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
+    const bool act = lane < ni;
+    const int ip = i0 + (act ? lane : 0);
+    const double xd = a.xi[ip], yd = a.yi[ip], zd = a.zi[ip];
+    const double cx = wave_centre(xd, act), cy = wave_centre(yd, act), cz = wave_centre(zd, act);
+    const float xi = (float) (xd - cx), yi = (float) (yd - cy), zi = (float) (zd - cz);
+    const double hid = a.hi[ip];
+    const float H = (float) (kSupport * hid);
+    const float invH = rcp(H);
+    const float norm = kNorm * invH * invH * invH;
+    const float rcut2 = act ? H * H * 1.0001f : -1.f;
+    Kahan dens;
+    int ngb = 0;
+    const int64_t lb = list_begin(a.L, g);
+    const int nj = a.L.cnt[g];
+    __shared__ float sm[kWave], sx[kWave], sy[kWave], sz[kWave];
+    for (int t = 0; t < nj; t += kWave) {
+        const int k = t + lane;
+        if (k < nj) {
+            const int q = a.L.idx[lb + k];
+            sm[lane] = (float) a.mj[q];
+            sx[lane] = (float) (a.xj[q] - cx);
+            sy[lane] = (float) (a.yj[q] - cy);
+            sz[lane] = (float) (a.zj[q] - cz);
+        }
+        __syncthreads();
+        const int nt = min(kWave, nj - t);
+        float part = 0.f;
+        for (int jj = 0; jj < nt; ++jj) {
+            const float dx = sx[jj] - xi, dy = sy[jj] - yi, dz = sz[jj] - zi;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < rcut2) {
+                const float r = r2 * rsq(r2 + 1e-30f);
+                const float s = r * invH;
+                const float om = fmaxf(1.f - s, 0.f);
+                const float om2 = om * om, om4 = om2 * om2;
+                const float w = (1.f + s * (8.f + s * (25.f + s * 32.f))) * (om4 * om4);
+                part += sm[jj] * w;
+                ngb += (om > 0.f) ? 1 : 0;
+            }
+        }
+        dens.add(part * norm);
+        __syncthreads();
+    }
+    if (act) {
+        const double d = dmax(5.0, dens.value());
+        a.rho_out[ip] = d;
+        a.h_out[ip] = kSMTH * pow(a.mi[ip] / d, 1.0 / 3.0);
+        if (a.ngb_out) a.ngb_out[ip] = ngb;
+    }
+}
+
+// ---------------------------------------------------------------- derivative
+__global__ __launch_bounds__(kWave) void k_drvt_mixed(DrvtArgs a) {
+    const int g = xcd_remap(blockIdx.x, a.ngroups);
+    const int lane = threadIdx.x;
+    const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
+    const bool act = lane < ni;
+    const int ip = i0 + (act ? lane : 0);
+    const double xd = a.xi[ip], yd = a.yi[ip], zd = a.zi[ip];
+    const double vxd = a.vxi[ip], vyd = a.vyi[ip], vzd = a.vzi[ip];
+    const double cx = wave_centre(xd, act), cy = wave_centre(yd, act), cz = wave_centre(zd, act);
+    const double cvx = wave_centre(vxd, act), cvy = wave_centre(vyd, act), cvz = wave_centre(vzd, act);
+    const float xi = (float) (xd - cx), yi = (float) (yd - cy), zi = (float) (zd - cz);
+    const float vxi = (float) (vxd - cvx), vyi = (float) (vyd - cvy), vzi = (float) (vzd - cvz);
+    const double hid = a.hi[ip], rhoi = a.rhoi[ip];
+    const float hi = (float) hid;
+    const float H = (float) (kSupport * hid);
+    const float invH = rcp(H);
+    const float norm = kNorm * invH * invH * invH;
+    const float reg = 1.0e-6f * hi;
+    const float rcut2 = act ? H * H * 1.0001f : -1.f;
+    Kahan divv, rx, ry, rz, gsum;
+    const int64_t lb = list_begin(a.L, g);
+    const int nj = a.L.cnt[g];
+    __shared__ float sm[kWave], sx[kWave], sy[kWave], sz[kWave], svx[kWave], svy[kWave], svz[kWave];
+    for (int t = 0; t < nj; t += kWave) {
+        const int k = t + lane;
+        if (k < nj) {
+            const int q = a.L.idx[lb + k];
+            sm[lane] = (float) a.mj[q];
+            sx[lane] = (float) (a.xj[q] - cx);
+            sy[lane] = (float) (a.yj[q] - cy);
+            sz[lane] = (float) (a.zj[q] - cz);
+            svx[lane] = (float) (a.vxj[q] - cvx);
+            svy[lane] = (float) (a.vyj[q] - cvy);
+            svz[lane] = (float) (a.vzj[q] - cvz);
+        }
+        __syncthreads();
+        const int nt = min(kWave, nj - t);
+        float pd = 0.f, px = 0.f, py = 0.f, pz = 0.f, pg = 0.f;
+        for (int jj = 0; jj < nt; ++jj) {
+            const float drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
+            const float r2 = drx * drx + dry * dry + drz * drz;
+            if (r2 < rcut2) {
+                const float dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
+                const float r = r2 * rsq(r2 + 1e-30f);
+                const float s = r * invH;
+                const float om = fmaxf(1.f - s, 0.f);
+                const float om2 = om * om, om4 = om2 * om2, om7 = om4 * om2 * om;
+                const float w = (1.f + s * (8.f + s * (25.f + s * 32.f))) * (om4 * om4);
+                // gradW = dr * W'(s) / (r H + 1e-6 h), W'(s) = -22 s (1+7s+16s^2) (1-s)^7 (norm folded in later)
+                const float f = (-22.f * s * (1.f + s * (7.f + 16.f * s)) * om7) * rcp(r * H + reg);
+                const float m = sm[jj];
+                const float dvdr = dvx * drx + dvy * dry + dvz * drz;
+                pd -= m * f * dvdr;
+                // (-m dv) x (dr f)
+                px -= m * f * (dvy * drz - dvz * dry);
+                py -= m * f * (dvz * drx - dvx * drz);
+                pz -= m * f * (dvx * dry - dvy * drx);
+                pg -= m * (3.f * w + f * r2);
+            }
+        }
+        divv.add(pd * norm);
+        rx.add(px * norm);
+        ry.add(py * norm);
+        rz.add(pz * norm);
+        gsum.add(pg * norm);
+        __syncthreads();
+    }
+    if (act) {
+        const double gs = 1.0 / (1.0 + hid * (gsum.value() / hid) / (3.0 * rhoi));
+        const double dv = divv.value() / rhoi;
+        const double inv = 1.0 / rhoi;
+        const double ox = rx.value() * inv, oy = ry.value() * inv, oz = rz.value() * inv;
+        a.divv[ip] = dv;
+        a.rotx[ip] = ox;
+        a.roty[ip] = oy;
+        a.rotz[ip] = oz;
+        a.gs[ip] = gs;
+        if (a.csi) a.bal[ip] = fabs(dv) / (fabs(dv) + sqrt(ox * ox + oy * oy + oz * oz) + 1.0e-4 * a.csi[ip] / hid);
+    }
+}
+
+// ---------------------------------------------------------------- hydro force
+__global__ __launch_bounds__(kWave) void k_hydro_mixed(HydroArgs a) {
+    const int g = xcd_remap(blockIdx.x, a.ngroups);
+    const int lane = threadIdx.x;
+    const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
+    const bool act = lane < ni;
+    const int ip = i0 + (act ? lane : 0);
+    const double xd = a.xi[ip], yd = a.yi[ip], zd = a.zi[ip];
+    const double vxd = a.vxi[ip], vyd = a.vyi[ip], vzd = a.vzi[ip];
+    const double cx = wave_centre(xd, act), cy = wave_centre(yd, act), cz = wave_centre(zd, act);
+    const double cvx = wave_centre(vxd, act), cvy = wave_centre(vyd, act), cvz = wave_centre(vzd, act);
+    const float xi = (float) (xd - cx), yi = (float) (yd - cy), zi = (float) (zd - cz);
+    const float vxi = (float) (vxd - cvx), vyi = (float) (vyd - cvy), vzi = (float) (vzd - cvz);
+    const double hid = a.hi[ip], rhoid = a.rhoi[ip];
+    const float Hi = (float) (kSupport * hid);
+    const float invHi = rcp(Hi);
+    const float normi = kNorm * invHi * invHi * invHi;
+    const float regi = (float) (1.0e-6 * hid);
+    const float rcut2i = act ? Hi * Hi * 1.0001f : -1.f;
+    const float rhoi = (float) rhoid, csi = (float) a.csi[ip], bali = (float) a.bali[ip];
+    const float fAi = (float) (a.gsi[ip] * a.pi[ip] / (rhoid * rhoid));  // f_i P_i / rho_i^2
+    const float gsi = (float) a.gsi[ip];
+    Kahan ax, ay, az, du;
+    float vmax = 0.f;
+    const int64_t lb = list_begin(a.L, g);
+    const int nj = a.L.cnt[g];
+    __shared__ float sx[kWave], sy[kWave], sz[kWave], svx[kWave], svy[kWave], svz[kWave], sm[kWave];
+    __shared__ float srho[kWave], scs[kWave], sbal[kWave], sgs[kWave], sA[kWave], sinvH[kWave], sH[kWave],
+        sreg[kWave], snorm[kWave], scut[kWave];
+    for (int t = 0; t < nj; t += kWave) {
+        const int k = t + lane;
+        if (k < nj) {
+            const int q = a.L.idx[lb + k];
+            sx[lane] = (float) (a.xj[q] - cx);
+            sy[lane] = (float) (a.yj[q] - cy);
+            sz[lane] = (float) (a.zj[q] - cz);
+            svx[lane] = (float) (a.vxj[q] - cvx);
+            svy[lane] = (float) (a.vyj[q] - cvy);
+            svz[lane] = (float) (a.vzj[q] - cvz);
+            sm[lane] = (float) a.mj[q];
+            const double rj = a.rhoj[q], gj = a.gsj[q], hj = a.hj[q];
+            srho[lane] = (float) rj;
+            scs[lane] = (float) a.csj[q];
+            sbal[lane] = (float) a.balj[q];
+            sgs[lane] = (float) gj;
+            sA[lane] = (float) (gj * a.pj[q] / (rj * rj));
+            const double Hj = kSupport * hj;
+            sH[lane] = (float) Hj;
+            sinvH[lane] = (float) (1.0 / Hj);
+            sreg[lane] = (float) (1.0e-6 * hj);
+            snorm[lane] = (float) (1365.0 / 64.0 / kPi / (Hj * Hj * Hj));
+            scut[lane] = (float) (Hj * Hj) * 1.0001f;
+        }
+        __syncthreads();
+        const int nt = min(kWave, nj - t);
+        float px = 0.f, py = 0.f, pz = 0.f, pu = 0.f;
+        for (int jj = 0; jj < nt; ++jj) {
+            const float drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
+            const float dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
+            const float dvdr = dvx * drx + dvy * dry + dvz * drz;
+            const float r2 = drx * drx + dry * dry + drz * drz;
+            const float rinv = rsq(r2 + 1e-30f);
+            const float w = (dvdr < 0.f) ? dvdr * rinv : 0.f;
+            const float v_sig = csi + scs[jj] - 3.f * w;
+            vmax = fmaxf(vmax, v_sig);
+            if (r2 < rcut2i || (act && r2 < scut[jj])) {
+                const float r = r2 * rinv;
+                const float AV = (-0.5f * v_sig * w) * rcp(rhoi + srho[jj]) * (bali + sbal[jj]);
+                // W'(s) factors for h_i and h_j
+                const float si = r * invHi;
+                const float oi = fmaxf(1.f - si, 0.f);
+                const float oi2 = oi * oi, oi7 = oi2 * oi2 * oi2 * oi;
+                const float fi = normi * (-22.f * si * (1.f + si * (7.f + 16.f * si)) * oi7) * rcp(r * Hi + regi);
+                const float Hj = sH[jj];
+                const float sj = r * sinvH[jj];
+                const float oj = fmaxf(1.f - sj, 0.f);
+                const float oj2 = oj * oj, oj7 = oj2 * oj2 * oj2 * oj;
+                const float fj = snorm[jj] * (-22.f * sj * (1.f + sj * (7.f + 16.f * sj)) * oj7) * rcp(r * Hj + sreg[jj]);
+                // gradW_i = dr fi, gradW_j = dr fj, mean = 0.5 (fi gs_i + fj gs_j) dr
+                const float fm = 0.5f * (fi * gsi + fj * sgs[jj]);
+                const float m = sm[jj];
+                const float coef = m * (fAi * fi + sA[jj] * fj + AV * fm);
+                px -= coef * drx;
+                py -= coef * dry;
+                pz -= coef * drz;
+                pu += m * (fAi + 0.5f * AV) * fm * dvdr;
+            }
+        }
+        ax.add(px);
+        ay.add(py);
+        az.add(pz);
+        du.add(pu);
+        __syncthreads();
+    }
+    if (act) {
+        a.ax[ip] = ax.value();
+        a.ay[ip] = ay.value();
+        a.az[ip] = az.value();
+        a.du[ip] = du.value();
+        a.dt[ip] = kCFL * 2.0 * hid / (double) vmax;
+    }
+}
+
+// ---------------------------------------------------------------- gravity
+__device__ __forceinline__ void grav_tile_mixed(const float *sx, const float *sy, const float *sz, const float *sgm,
+                                                int nt, float xi, float yi, float zi, float eps2, float &px, float &py,
+                                                float &pz, float &pp) {
+#pragma unroll 4
+    for (int jj = 0; jj < nt; ++jj) {
+        const float drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
+        const float r2 = drx * drx + dry * dry + drz * drz + eps2;
+        const float inv = rsq(r2);
+        const float gm_inv = sgm[jj] * inv;
+        const float gm_inv3 = gm_inv * inv * inv;
+        px -= gm_inv3 * drx;
+        py -= gm_inv3 * dry;
+        pz -= gm_inv3 * drz;
+        pp -= gm_inv;
+    }
+}
+
+__global__ __launch_bounds__(kWave) void k_grav_mixed(GravArgs a) {
+    const int g = xcd_remap(blockIdx.x, a.ngroups);
+    const int lane = threadIdx.x;
+    const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
+    const bool act = lane < ni;
+    const int ip = i0 + (act ? lane : 0);
+    const double xd = a.xi[ip], yd = a.yi[ip], zd = a.zi[ip];
+    const double cx = wave_centre(xd, act), cy = wave_centre(yd, act), cz = wave_centre(zd, act);
+    const float xi = (float) (xd - cx), yi = (float) (yd - cy), zi = (float) (zd - cz);
+    double e2;
+    if (a.eps2i) e2 = a.eps2i[ip];
+    else { const double h = a.hi[ip]; e2 = h * h; }
+    const float eps2 = (float) e2;
+    Kahan ax, ay, az, pot;
+    __shared__ float sx[kWave], sy[kWave], sz[kWave], sgm[kWave];
+    for (int pass = 0; pass < 2; ++pass) {
+        const ListRef &L = pass ? a.Ls : a.Lj;
+        if (pass && !a.ms) break;
+        const double *X = pass ? a.xs : a.xj, *Y = pass ? a.ys : a.yj, *Z = pass ? a.zs : a.zj,
+                     *M = pass ? a.ms : a.mj;
+        const int64_t lb = list_begin(L, g);
+        const int nj = L.cnt[g];
+        for (int t = 0; t < nj; t += kWave) {
+            const int k = t + lane;
+            if (k < nj) {
+                const int q = L.idx[lb + k];
+                sx[lane] = (float) (X[q] - cx);
+                sy[lane] = (float) (Y[q] - cy);
+                sz[lane] = (float) (Z[q] - cz);
+                sgm[lane] = (float) (kG * M[q]);
+            }
+            __syncthreads();
+            float px = 0.f, py = 0.f, pz = 0.f, pp = 0.f;
+            grav_tile_mixed(sx, sy, sz, sgm, min(kWave, nj - t), xi, yi, zi, eps2, px, py, pz, pp);
+            ax.add(px);
+            ay.add(py);
+            az.add(pz);
+            pot.add(pp);
+            __syncthreads();
+        }
+    }
+    if (act) {
+        a.ax[ip] = ax.value();
+        a.ay[ip] = ay.value();
+        a.az[ip] = az.value();
+        a.pot[ip] = pot.value();
+    }
+}
+
+}  // namespace
+
+void launch_dens_mixed(const DensArgs &a, hipStream_t s) {
+    if (a.ngroups <= 0) return;
+    hipLaunchKernelGGL(k_dens_mixed, dim3(a.ngroups), dim3(kWave), 0, s, a);
+    SPH_LAUNCH_CHECK();
+}
+void launch_drvt_mixed(const DrvtArgs &a, hipStream_t s) {
+    if (a.ngroups <= 0) return;
+    hipLaunchKernelGGL(k_drvt_mixed, dim3(a.ngroups), dim3(kWave), 0, s, a);
+    SPH_LAUNCH_CHECK();
+}
+void launch_hydro_mixed(const HydroArgs &a, hipStream_t s) {
+    if (a.ngroups <= 0) return;
+    hipLaunchKernelGGL(k_hydro_mixed, dim3(a.ngroups), dim3(kWave), 0, s, a);
+    SPH_LAUNCH_CHECK();
+}
+void launch_grav_mixed(const GravArgs &a, hipStream_t s) {
+    if (a.ngroups <= 0) return;
+    hipLaunchKernelGGL(k_grav_mixed, dim3(a.ngroups), dim3(kWave), 0, s, a);
+    SPH_LAUNCH_CHECK();
+}
+
 }  // namespace sphgpu

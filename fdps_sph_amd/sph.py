@@ -16,7 +16,8 @@ from typing import Sequence
 import numpy as np
 
 from . import structs as S
-from ._lib import (FIELD, I32_FIELDS, I64_FIELDS, PREC_FP64, PREC_MIXED, VEC_FIELDS, SimParams, check, lib, ptr)
+from ._lib import (FIELD, I32_FIELDS, I64_FIELDS, PREC_FP64, PREC_MIXED, VEC_FIELDS, KernelStat, SimParams, check,
+                   lib, ptr)
 
 
 def _walk_arrays(walks: Sequence[np.ndarray], dtype):
@@ -179,6 +180,15 @@ class Simulation:
         ms = np.zeros(8)
         check(self._L.sphgpu_sim_stage_ms(self.ctx.handle, ptr(ms), 8))
         return ms
+
+    def set_stats(self, on: bool = True):
+        check(self._L.sphgpu_sim_set_stats(self.ctx.handle, int(bool(on))))
+
+    def kernel_stats(self) -> dict:
+        arr = (KernelStat * 4)()
+        check(self._L.sphgpu_sim_kernel_stats(self.ctx.handle, arr, 4))
+        names = ["dens", "drvt", "hydro", "grav"]
+        return {names[k]: {f: getattr(arr[k], f) for f, _ in KernelStat._fields_} for k in range(4)}
 
     def lists(self, kind: int):
         ng, nj, ns = C.c_int64(), C.c_int64(), C.c_int64()

@@ -181,6 +181,19 @@ int32_t sphgpu_sim_get_time(sphgpu_ctx *ctx, double *dt_sys, double *time_sys);
  * [4] derivative, [5] hydro, [6] gravity, [7] final kick + dt; n <= 8. */
 int32_t sphgpu_sim_stage_ms(sphgpu_ctx *ctx, double *ms, int32_t n);
 
+/* Per-kernel statistics of the interaction kernels over the steps run since
+ * statistics were switched on: summed kernel time (hipEvents on the
+ * context's stream, one pair per launch), launches, and the interaction-list
+ * sizes those launches processed: n_i = i-particles, n_j = j-list entries
+ * (particles), n_s = monopole entries, pairs = sum over groups of
+ * n_i * (n_j + n_s).  Index 0 density, 1 derivative, 2 hydro, 3 gravity. */
+typedef struct {
+    double ms;
+    int64_t launches, n_i, n_j, n_s, pairs;
+} sphgpu_kernel_stat;
+int32_t sphgpu_sim_set_stats(sphgpu_ctx *ctx, int32_t on);
+int32_t sphgpu_sim_kernel_stats(sphgpu_ctx *ctx, sphgpu_kernel_stat *out, int32_t n);
+
 /* Interaction-list export for the current particle state (sorts and builds
  * the tree first).  kind: 0 gather (density/derivative), 1 symmetric
  * (hydro), 2 gravity.  First call with NULL arrays to get the sizes. */
