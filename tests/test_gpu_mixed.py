@@ -26,9 +26,14 @@ def ctx32():
 
 
 def test_mixed_functors(ctx32, oracle):
+    """Errors are measured against the physical scale of each output: sigma_v / h
+    for velocity gradients, P / (rho h) for the pressure acceleration and
+    (P / rho) (sigma_v / h) for du/dt -- these sums cancel strongly for the
+    random velocities used here, so their own value is no scale."""
     print("[mixed functors]", flush=True)
     c = H.cloud(700, 21)
     walks = H.walks_from(c, [64, 100, 3], 21, j_frac=0.7)
+    sig_v = float(np.std(c["vel"]))
     res = ctx32.calc_density([H.epi_dens(c, i) for i, _ in walks], [H.epj_dens(c, j) for _, j in walks])
     for (i, j), r in zip(walks, res):
         ref = oracle.calc_density(H.epi_dens(c, i), H.epj_dens(c, j))
@@ -37,16 +42,16 @@ def test_mixed_functors(ctx32, oracle):
     res = ctx32.calc_derivative([H.epi_drvt(c, i) for i, _ in walks], [H.epj_drvt(c, j) for _, j in walks])
     for (i, j), r in zip(walks, res):
         ref = oracle.calc_derivative(H.epi_drvt(c, i), H.epj_drvt(c, j))
-        sc = 1e-3 * (np.abs(ref["div_v"]).max() + np.abs(ref["rot_v"]).max())
+        sc = sig_v / c["smth"][i]
         assert H.rel_err(r["div_v"], ref["div_v"], sc) < 1e-5
-        assert H.rel_err(r["rot_v"], ref["rot_v"], sc) < 1e-5
+        assert H.rel_err(r["rot_v"], ref["rot_v"], sc[:, None]) < 1e-5
         assert H.rel_err(r["grad_smth"], ref["grad_smth"]) < 1e-5
     res = ctx32.calc_hydro([H.epi_hydro(c, i) for i, _ in walks], [H.epj_hydro(c, j) for _, j in walks])
     for (i, j), r in zip(walks, res):
         ref = oracle.calc_hydro(H.epi_hydro(c, i), H.epj_hydro(c, j))
-        sa = 1e-3 * np.abs(ref["acc"]).max()
-        assert H.rel_err(r["acc"], ref["acc"], sa) < 1e-5
-        assert H.rel_err(r["eng_dot"], ref["eng_dot"], 1e-3 * np.abs(ref["eng_dot"]).max()) < 1e-5
+        pa = c["pres"][i] / (c["dens"][i] * c["smth"][i])
+        assert H.rel_err(r["acc"], ref["acc"], pa[:, None]) < 1e-5
+        assert H.rel_err(r["eng_dot"], ref["eng_dot"], c["pres"][i] / c["dens"][i] * sig_v / c["smth"][i]) < 1e-5
         assert H.rel_err(r["dt"], ref["dt"]) < 1e-6
     res = ctx32.calc_gravity([H.epi_grav(c, i) for i, _ in walks], [H.epj_grav(c, j) for _, j in walks])
     for (i, j), r in zip(walks, res):
