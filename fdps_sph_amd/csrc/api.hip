@@ -489,20 +489,19 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
     hipStream_t s = c->stream;
     const int64_t n = S.n;
     const bool mixed = c->precision == SPHGPU_PREC_MIXED;
-    auto rec = [&](int k) {
+    auto rec_ev = [&](int k) {
         if (timed) SPH_HIP(hipEventRecord(S.ev[k], s));
     };
     tree_sort_and_build(S.tree, S.tp, n, S.P, S.Palt, s);
-    rec(2);
+    rec_ev(2);
     PArrays &P = S.P;
     const int G = (int) S.tree.ngroups;
     const int32_t *gs = S.tree.gstart.get();
     int kslot = 0;
-    auto kbegin = [&](const Lists &L, bool spj) {
+    auto kbegin = [&](const Lists *Lp, bool spj) {
         if (!S.stats) return;
-        const int kind = S.kkind[kslot];
-        hipLaunchKernelGGL(k_list_counts, dim3(nblk(G, 256)), dim3(256), 0, s, G, gs, L.cnt.get(),
-                           spj ? L.scnt.get() : nullptr, S.kcount.get() + 4 * kind);
+        hipLaunchKernelGGL(k_list_counts, dim3(nblk(G, 256)), dim3(256), 0, s, G, gs, Lp->cnt.get(),
+                           spj ? Lp->scnt.get() : nullptr, S.kcount.get() + 4 * S.kkind[kslot]);
         SPH_LAUNCH_CHECK();
         SPH_HIP(hipEventRecord(S.kev[kslot][0], s));
     };
@@ -511,43 +510,42 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
         SPH_HIP(hipEventRecord(S.kev[kslot][1], s));
         ++kslot;
     };
+    Lists &L = S.lsph;
+    auto lref = [&]() { return ListRef{L.idx.get(), nullptr, L.cnt.get(), L.cap}; };
     // 4 density passes, each with lists from the current h (src/main.cpp:138-140)
     for (int loop = 0; loop <= 3; ++loop) {
-        build_lists_sph(S.tree, S.tp, P, 0, S.lsph, s);
-        DensArgs a{G, gs, P.x, P.y, P.z, P.m, P.h, P.m, P.x, P.y, P.z,
-                   ListRef{S.lsph.idx.get(), nullptr, S.lsph.cnt.get(), S.lsph.cap}, P.rho, P.h, P.ngb};
-        kbegin(S.lsph, false);
+        build_lists_sph(S.tree, S.tp, P, 0, L, s);
+        DensArgs a{G, gs, P.x, P.y, P.z, P.m, P.h, P.m, P.x, P.y, P.z, lref(), P.rho, P.h, P.ngb};
+        kbegin(&L, false);
         if (mixed) launch_dens_mixed(a, s);
         else launch_dens_fp64(a, s);
         kend();
     }
-    rec(3);
+    rec_ev(3);
     launch_eos_pressure(n, P.mat, P.rho, P.u, c->d_mats.get(), SPHGPU_MAX_MATERIALS, P.p, P.cs, s);
-    rec(4);
-    build_lists_sph(S.tree, S.tp, P, 0, S.lsph, s);
+    rec_ev(4);
+    build_lists_sph(S.tree, S.tp, P, 0, L, s);
     {
         DrvtArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.cs, P.m, P.x, P.y, P.z, P.vx, P.vy, P.vz,
-                   ListRef{S.lsph.idx.get(), nullptr, S.lsph.cnt.get(), S.lsph.cap},
-                   P.divv, P.rotx, P.roty, P.rotz, P.gs, P.bal};
-        kbegin(S.lsph, false);
+                   lref(), P.divv, P.rotx, P.roty, P.rotz, P.gs, P.bal};
+        kbegin(&L, false);
         if (mixed) launch_drvt_mixed(a, s);
         else launch_drvt_fp64(a, s);
         kend();
     }
-    rec(5);
+    rec_ev(5);
     tree_update_maxH(S.tree, P.h, s);
-    build_lists_sph(S.tree, S.tp, P, 1, S.lsph, s);
+    build_lists_sph(S.tree, S.tp, P, 1, L, s);
     {
         HydroArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal,
                     P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal, P.m,
-                    ListRef{S.lsph.idx.get(), nullptr, S.lsph.cnt.get(), S.lsph.cap},
-                    P.ax, P.ay, P.az, P.du, P.dti};
-        kbegin(S.lsph, false);
+                    lref(), P.ax, P.ay, P.az, P.du, P.dti};
+        kbegin(&L, false);
         if (mixed) launch_hydro_mixed(a, s);
         else launch_hydro_fp64(a, s);
         kend();
     }
-    rec(6);
+    rec_ev(6);
     build_lists_grav(S.tree, S.tp, P, S.lgrav, s);
     {
         const Tree &t = S.tree;
@@ -556,12 +554,12 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
                    t.comx.get(), t.comy.get(), t.comz.get(), t.mass.get(),
                    ListRef{S.lgrav.sidx.get(), nullptr, S.lgrav.scnt.get(), S.lgrav.scap},
                    P.gx, P.gy, P.gz, P.pot};
-        kbegin(S.lgrav, true);
+        kbegin(&S.lgrav, true);
         if (mixed) launch_grav_mixed(a, s);
         else launch_grav_fp64(a, s);
         kend();
     }
-    rec(7);
+    rec_ev(7);
     if (S.stats) {
         SPH_HIP(hipStreamSynchronize(s));
         for (int k = 0; k < kslot; ++k) {

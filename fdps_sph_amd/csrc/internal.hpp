@@ -207,6 +207,7 @@ struct Tree {
     // i-groups
     int64_t ngroups = 0;
     DevBuf<int32_t> gstart;
+    int64_t n = 0;  // particles of the last build
 };
 
 void tree_sort_and_build(Tree &t, const TreeParams &tp, int64_t n, PArrays &P, PArrays &Palt, hipStream_t s);

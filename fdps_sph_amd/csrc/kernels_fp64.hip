@@ -37,6 +37,20 @@ __device__ __forceinline__ double gradW_factor_(double s, double r, double H, do
     return rv;
 }
 
+// Bit k set when j-tile entry k lies inside this lane's widened support.  The
+// kernels then evaluate only the set bits, in ascending order; entries left out
+// contribute exactly 0 to every sum, so the sums equal the full loop's.
+__device__ __forceinline__ uint64_t support_mask_(const double *sx, const double *sy, const double *sz, int nt,
+                                                  double xi, double yi, double zi, double rcut2) {
+    uint64_t mask = 0;
+    for (int jj = 0; jj < nt; ++jj) {
+        const double dx = sx[jj] - xi, dy = sy[jj] - yi, dz = sz[jj] - zi;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        mask |= (r2 <= rcut2) ? (1ULL << jj) : 0ULL;
+    }
+    return mask;
+}
+
 // ---------------------------------------------------------------- density
 // CalcDensity::operator() (src/force.h:9-23)
 __global__ __launch_bounds__(kWave) void k_dens(DensArgs a) {
@@ -65,14 +79,15 @@ __global__ __launch_bounds__(kWave) void k_dens(DensArgs a) {
         }
         __syncthreads();
         const int nt = min(kWave, nj - t);
-        for (int jj = 0; jj < nt; ++jj) {
+        // pass 1: this lane's in-support j of the tile; pass 2: only those, in order
+        const uint64_t mask = support_mask_(sx, sy, sz, nt, xi, yi, zi, rcut2);
+        for (uint64_t m = mask; m; m &= m - 1) {
+            const int jj = __builtin_ctzll(m);
             const double dx = sx[jj] - xi, dy = sy[jj] - yi, dz = sz[jj] - zi;
             const double r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 <= rcut2) {
-                const double s = sqrt(r2) / H;
-                dens += sm[jj] * W_(s, norm);
-                ngb += (plus_(1.0 - s) > 0) ? 1 : 0;
-            }
+            const double s = sqrt(r2) / H;
+            dens += sm[jj] * W_(s, norm);
+            ngb += (plus_(1.0 - s) > 0) ? 1 : 0;
         }
         __syncthreads();
     }
@@ -117,10 +132,12 @@ __global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
         }
         __syncthreads();
         const int nt = min(kWave, nj - t);
-        for (int jj = 0; jj < nt; ++jj) {
+        const uint64_t mask = support_mask_(sx, sy, sz, nt, xi, yi, zi, rcut2);
+        for (uint64_t msk = mask; msk; msk &= msk - 1) {
+            const int jj = __builtin_ctzll(msk);
             const double drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
             const double r2 = drx * drx + dry * dry + drz * drz;
-            if (r2 <= rcut2) {
+            {
                 const double dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
                 const double r = sqrt(r2);
                 const double s = r / H;
@@ -205,15 +222,40 @@ __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
         }
         __syncthreads();
         const int nt = min(kWave, nj - t);
+        // pass 1, every j: v_sig_max, and the mask of pairs inside either support.
+        // v_sig = (c_i + c_j) - 3w with |w| = |dv.dr|/|dr| <= |dv|; when even that
+        // bound stays below the current maximum (with a margin far above the
+        // rounding of either side) the square root and division are skipped --
+        // the maximum cannot change, so it stays bit-identical.
+        uint64_t mask = 0;
         for (int jj = 0; jj < nt; ++jj) {
+            const double drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
+            const double dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
+            const double dvdr = dvx * drx + dvy * dry + dvz * drz;
+            const double r2 = drx * drx + dry * dry + drz * drz;
+            const double cij = csi + scs[jj];
+            if (dvdr < 0) {
+                const double room = vmax - cij;
+                const double dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
+                if (!(room > 1.0e-4 * vmax && 9.0 * dv2 < (room * room) * (1.0 - 1.0e-10))) {
+                    const double w = dvdr / sqrt(r2);
+                    vmax = dmax(vmax, cij - 3.0 * w);
+                }
+            } else {
+                vmax = dmax(vmax, cij - 3.0 * 0.0);
+            }
+            mask |= (r2 <= rcut2i || (act && r2 <= scut[jj])) ? (1ULL << jj) : 0ULL;
+        }
+        // pass 2: the force terms of this lane's in-support pairs, in list order
+        for (uint64_t msk = mask; msk; msk &= msk - 1) {
+            const int jj = __builtin_ctzll(msk);
             const double drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
             const double dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
             const double dvdr = dvx * drx + dvy * dry + dvz * drz;
             const double r2 = drx * drx + dry * dry + drz * drz;
             const double w = (dvdr < 0) ? dvdr / sqrt(r2) : 0;
             const double v_sig = csi + scs[jj] - 3.0 * w;
-            vmax = dmax(vmax, v_sig);
-            if (r2 <= rcut2i || (act && r2 <= scut[jj])) {
+            {
                 const double rhoj = srho[jj];
                 const double AV = -kAV * 0.5 * v_sig * w / (0.5 * (rhoi + rhoj)) * 0.5 * (bali + sbal[jj]);
                 const double r = sqrt(r2);

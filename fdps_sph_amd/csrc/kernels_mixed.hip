@@ -50,6 +50,18 @@ __device__ __forceinline__ double wave_centre(double v, bool act) {
 
 constexpr float kNorm = (float) (1365.0 / 64.0 / kPi);
 
+// bit k set when j-tile entry k lies inside this lane's (widened) support
+__device__ __forceinline__ uint64_t support_mask(const float *sx, const float *sy, const float *sz, int nt, float xi,
+                                                 float yi, float zi, float rcut2) {
+    uint64_t mask = 0;
+    for (int jj = 0; jj < nt; ++jj) {
+        const float dx = sx[jj] - xi, dy = sy[jj] - yi, dz = sz[jj] - zi;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        mask |= (r2 < rcut2) ? (1ULL << jj) : 0ULL;
+    }
+    return mask;
+}
+
 // ---------------------------------------------------------------- density
 __global__ __launch_bounds__(kWave) void k_dens_mixed(DensArgs a) {
     const int g = xcd_remap(blockIdx.x, a.ngroups);
@@ -82,10 +94,12 @@ __global__ __launch_bounds__(kWave) void k_dens_mixed(DensArgs a) {
         __syncthreads();
         const int nt = min(kWave, nj - t);
         float part = 0.f;
-        for (int jj = 0; jj < nt; ++jj) {
+        const uint64_t mask = support_mask(sx, sy, sz, nt, xi, yi, zi, rcut2);
+        for (uint64_t msk = mask; msk; msk &= msk - 1) {
+            const int jj = __builtin_ctzll(msk);
             const float dx = sx[jj] - xi, dy = sy[jj] - yi, dz = sz[jj] - zi;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 < rcut2) {
+            {
                 const float r = r2 * rsq(r2 + 1e-30f);
                 const float s = r * invH;
                 const float om = fmaxf(1.f - s, 0.f);
@@ -145,10 +159,12 @@ __global__ __launch_bounds__(kWave) void k_drvt_mixed(DrvtArgs a) {
         __syncthreads();
         const int nt = min(kWave, nj - t);
         float pd = 0.f, px = 0.f, py = 0.f, pz = 0.f, pg = 0.f;
-        for (int jj = 0; jj < nt; ++jj) {
+        const uint64_t mask = support_mask(sx, sy, sz, nt, xi, yi, zi, rcut2);
+        for (uint64_t msk = mask; msk; msk &= msk - 1) {
+            const int jj = __builtin_ctzll(msk);
             const float drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
             const float r2 = drx * drx + dry * dry + drz * drz;
-            if (r2 < rcut2) {
+            {
                 const float dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
                 const float r = r2 * rsq(r2 + 1e-30f);
                 const float s = r * invH;
@@ -244,7 +260,20 @@ __global__ __launch_bounds__(kWave) void k_hydro_mixed(HydroArgs a) {
         __syncthreads();
         const int nt = min(kWave, nj - t);
         float px = 0.f, py = 0.f, pz = 0.f, pu = 0.f;
+        // pass 1, every j: v_sig_max and the mask of pairs inside either support
+        uint64_t mask = 0;
         for (int jj = 0; jj < nt; ++jj) {
+            const float drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
+            const float dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
+            const float dvdr = dvx * drx + dvy * dry + dvz * drz;
+            const float r2 = drx * drx + dry * dry + drz * drz;
+            const float w = (dvdr < 0.f) ? dvdr * rsq(r2 + 1e-30f) : 0.f;
+            vmax = fmaxf(vmax, csi + scs[jj] - 3.f * w);
+            mask |= (r2 < rcut2i || (act && r2 < scut[jj])) ? (1ULL << jj) : 0ULL;
+        }
+        // pass 2: force terms of this lane's in-support pairs
+        for (uint64_t msk = mask; msk; msk &= msk - 1) {
+            const int jj = __builtin_ctzll(msk);
             const float drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
             const float dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
             const float dvdr = dvx * drx + dvy * dry + dvz * drz;
@@ -252,8 +281,7 @@ __global__ __launch_bounds__(kWave) void k_hydro_mixed(HydroArgs a) {
             const float rinv = rsq(r2 + 1e-30f);
             const float w = (dvdr < 0.f) ? dvdr * rinv : 0.f;
             const float v_sig = csi + scs[jj] - 3.f * w;
-            vmax = fmaxf(vmax, v_sig);
-            if (r2 < rcut2i || (act && r2 < scut[jj])) {
+            {
                 const float r = r2 * rinv;
                 const float AV = (-0.5f * v_sig * w) * rcp(rhoi + srho[jj]) * (bali + sbal[jj]);
                 // W'(s) factors for h_i and h_j

@@ -5,11 +5,14 @@
 // pair (k-1, k) the particle count of the smallest octree cell holding both
 // is computed by two binary searches on the sorted keys; a node boundary at
 // threshold T sits exactly where that count exceeds T.  Leaves use
-// T = n_leaf, tree level l uses n_leaf * 8^l, the top level holds one node
-// per system, and i-groups use T = n_group.  Every such node is an octree cell
-// (or a run of equal keys), nodes nest level by level, and node moments are
-// summed sequentially in index order so that the CPU oracle reproduces them
-// bit for bit.  This file is compiled with -ffp-contract=off.
+// T = n_leaf, tree level l uses n_leaf * 8^l and the top level holds one node
+// per system.  Every such node is an octree cell (or a run of equal keys),
+// nodes nest level by level, and node moments are summed sequentially in index
+// order so that the CPU oracle reproduces them bit for bit.  i-groups pack
+// consecutive leaves greedily (up to n_group particles) inside each level-2
+// node, so a 64-lane wavefront holds ~57 compact i-particles on average
+// instead of the ~25 of one octree cell.  This file is compiled with
+// -ffp-contract=off.
 #include <hipcub/hipcub.hpp>
 
 #include "internal.hpp"
@@ -268,6 +271,32 @@ __global__ void k_sys_root(int64_t cnt_top, int64_t off_top, const int32_t *nsta
     const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= cnt_top) return;
     sys_root[sys[nstart[off_top + q]]] = (int32_t) (off_top + q);
+}
+
+// i-groups: inside node q of the packing level, consecutive leaves are packed
+// greedily while a group stays within n_group particles; flags mark each
+// group's first particle
+__global__ void k_pack_groups(int64_t cnt_l, int64_t off_l, const int32_t *nstart, const int32_t *nend, int nleaf,
+                              int n_group, uint8_t *flags) {
+    const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= cnt_l) return;
+    const int64_t id = off_l + q;
+    const int a = lb_i32(nstart, nleaf, nstart[id]);
+    const int b = lb_i32(nstart, nleaf, nend[id]);
+    int run = 0;
+    for (int c = a; c < b; ++c) {
+        const int sz = nend[c] - nstart[c];
+        if (c == a || run + sz > n_group) {
+            flags[nstart[c]] = 1;
+            run = 0;
+        }
+        run += sz;
+    }
+}
+
+__global__ void k_fill_u8(int64_t n, uint8_t *a) {
+    const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] = 0;
 }
 
 __global__ void k_fill_i32(int64_t n, int32_t *a, int32_t v) {
@@ -577,6 +606,7 @@ static void compute_sys_boxes(Tree &t, const TreeParams &tp, int64_t n, const PA
 }
 
 static void build_levels(Tree &t, const TreeParams &tp, int64_t n, const PArrays &P, hipStream_t s) {
+    t.n = n;
     t.cnt.ensure(n);
     hipLaunchKernelGGL(k_lca, dim3(nblk(n, 256)), dim3(256), 0, s, n, t.key.get(), tp.bits, tp.sysbits, t.cnt.get());
     SPH_LAUNCH_CHECK();
@@ -592,10 +622,9 @@ static void build_levels(Tree &t, const TreeParams &tp, int64_t n, const PArrays
         t.sel_tmp_bytes = need;
     }
     int64_t T = tp.n_leaf;
-    for (int l = 0; l <= L; ++l) {
+    for (int l = 0; l < L; ++l) {
         int64_t thr;
-        if (l == L) thr = tp.n_group;
-        else if (l == L - 1) thr = INT64_MAX - 1;
+        if (l == L - 1) thr = INT64_MAX - 1;
         else thr = T;
         hipLaunchKernelGGL(k_flags, dim3(nblk(n, 256)), dim3(256), 0, s, n, t.cnt.get(), thr, t.flags.get());
         SPH_LAUNCH_CHECK();
@@ -605,7 +634,7 @@ static void build_levels(Tree &t, const TreeParams &tp, int64_t n, const PArrays
         if (l < L - 1) T *= 8;
     }
     std::vector<int32_t> cnts(L + 1);
-    SPH_HIP(hipMemcpyAsync(cnts.data(), t.counts.get(), sizeof(int32_t) * (L + 1), hipMemcpyDeviceToHost, s));
+    SPH_HIP(hipMemcpyAsync(cnts.data(), t.counts.get(), sizeof(int32_t) * L, hipMemcpyDeviceToHost, s));
     SPH_HIP(hipStreamSynchronize(s));
     t.nlevels = L;
     t.level_off[0] = 0;
@@ -620,13 +649,6 @@ static void build_levels(Tree &t, const TreeParams &tp, int64_t n, const PArrays
     for (int l = 0; l < L; ++l)
         SPH_HIP(hipMemcpyAsync(t.nstart.get() + t.level_off[l], t.sel_out.get() + (size_t) l * n,
                                sizeof(int32_t) * cnts[l], hipMemcpyDeviceToDevice, s));
-    t.ngroups = cnts[L];
-    t.gstart.ensure(t.ngroups + 1);
-    SPH_HIP(hipMemcpyAsync(t.gstart.get(), t.sel_out.get() + (size_t) L * n, sizeof(int32_t) * t.ngroups,
-                           hipMemcpyDeviceToDevice, s));
-    const int32_t nn32 = (int32_t) n;
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s, (int64_t) 1, t.gstart.get() + t.ngroups, nn32);
-    SPH_LAUNCH_CHECK();
     for (int l = 0; l < L; ++l) {
         const int64_t cl = cnts[l];
         const int64_t offb = (l == 0) ? -1 : t.level_off[l - 1];
@@ -651,6 +673,27 @@ static void build_levels(Tree &t, const TreeParams &tp, int64_t n, const PArrays
     SPH_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_sys_root, dim3(nblk(cnts[L - 1], 64)), dim3(64), 0, s, (int64_t) cnts[L - 1],
                        t.level_off[L - 1], t.nstart.get(), P.sys, t.sys_root.get());
+    SPH_LAUNCH_CHECK();
+    // i-groups: leaves packed greedily inside each level-2 node (or the top level)
+    const int pl = std::min(2, L - 1);
+    hipLaunchKernelGGL(k_fill_u8, dim3(nblk(n, 256)), dim3(256), 0, s, n, t.flags.get());
+    SPH_LAUNCH_CHECK();
+    const int64_t cpl = cnts[pl];
+    hipLaunchKernelGGL(k_pack_groups, dim3(nblk(cpl, 64)), dim3(64), 0, s, cpl, t.level_off[pl], t.nstart.get(),
+                       t.nend.get(), (int) cnts[0], tp.n_group, t.flags.get());
+    SPH_LAUNCH_CHECK();
+    size_t bytes = t.sel_tmp_bytes;
+    SPH_HIP(hipcub::DeviceSelect::Flagged(t.sel_tmp.get(), bytes, it, t.flags.get(), t.sel_out.get() + (size_t) L * n,
+                                          t.counts.get() + L, (int) n, s));
+    int32_t ng = 0;
+    SPH_HIP(hipMemcpyAsync(&ng, t.counts.get() + L, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SPH_HIP(hipStreamSynchronize(s));
+    t.ngroups = ng;
+    t.gstart.ensure(t.ngroups + 1);
+    SPH_HIP(hipMemcpyAsync(t.gstart.get(), t.sel_out.get() + (size_t) L * n, sizeof(int32_t) * t.ngroups,
+                           hipMemcpyDeviceToDevice, s));
+    const int32_t nn32 = (int32_t) n;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s, (int64_t) 1, t.gstart.get() + t.ngroups, nn32);
     SPH_LAUNCH_CHECK();
 }
 

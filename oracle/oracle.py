@@ -70,8 +70,12 @@ def lib(kind: str = "strict") -> C.CDLL:
     if kind in _LIBS:
         return _LIBS[kind]
     path = os.path.join(HERE, f"liboracle_{kind}.so")
-    if not os.path.exists(path):
+    # make rebuilds only what is older than sphgpu_oracle.c/.h
+    try:
         build()
+    except (OSError, subprocess.CalledProcessError):
+        if not os.path.exists(path):
+            raise
     L = C.CDLL(path)
     P = C.c_void_p
     d = C.c_double
@@ -123,6 +127,9 @@ def lib(kind: str = "strict") -> C.CDLL:
     L.orc_list_grav.argtypes = [P, C.POINTER(_State), i64, d, P, i64, P, P, i64, P]
     L.orc_morton.restype = C.c_uint64
     L.orc_morton.argtypes = [d, d, d, P, d, i]
+    L.orc_hydro_term_scale.argtypes = [P, i, P, i, P, P]
+    L.orc_drvt_term_scale.argtypes = [P, i, P, i, P, P]
+    L.orc_state_term_scales.argtypes = [C.POINTER(_State), C.POINTER(_Params), P, P, P, P, P]
     L.orc_wtime.restype = d
     L.orc_num_threads.restype = i
     _LIBS[kind] = L
@@ -188,6 +195,33 @@ def calc_grav(epi, epj, spj=None, kind="strict"):
         spj = np.ascontiguousarray(spj)
         L.orc_calc_grav_sp(_p(epi), len(epi), _p(spj), len(spj), _p(res))
     return res
+
+
+def hydro_term_scale(epi, epj, kind="strict"):
+    epi = np.ascontiguousarray(epi)
+    epj = np.ascontiguousarray(epj)
+    a = np.zeros(len(epi))
+    u = np.zeros(len(epi))
+    lib(kind).orc_hydro_term_scale(_p(epi), len(epi), _p(epj), len(epj), _p(a), _p(u))
+    return a, u
+
+
+def drvt_term_scale(epi, epj, kind="strict"):
+    epi = np.ascontiguousarray(epi)
+    epj = np.ascontiguousarray(epj)
+    d = np.zeros(len(epi))
+    r = np.zeros(len(epi))
+    lib(kind).orc_drvt_term_scale(_p(epi), len(epi), _p(epj), len(epj), _p(d), _p(r))
+    return d, r
+
+
+def state_term_scales(state, prm, kind="strict"):
+    """Per-particle sizes of the summed terms: acc, du, div, rot, grav."""
+    n = state.n
+    out = [np.zeros(n) for _ in range(5)]
+    st = state.cstruct()
+    lib(kind).orc_state_term_scales(C.byref(st), C.byref(prm), *[_p(a) for a in out])
+    return dict(zip(["acc", "du", "div", "rot", "grav"], out))
 
 
 def balsara(div_v, rot_v, snds, smth, kind="strict") -> float:

@@ -207,6 +207,76 @@ void orc_calc_grav_sp(const orc_epi_grav *epi, int nip, const orc_spj_mono *spj,
     }
 }
 
+static inline double absmax3_(const double v[3])
+{
+    double m = fabs(v[0]);
+    if (fabs(v[1]) > m) m = fabs(v[1]);
+    if (fabs(v[2]) > m) m = fabs(v[2]);
+    return m;
+}
+
+void orc_hydro_term_scale(const orc_epi_hydro *epi, int nip, const orc_epj_hydro *epj, int njp, double *acc_scale,
+                          double *du_scale)
+{
+    for (int i = 0; i < nip; ++i) {
+        const orc_epi_hydro *ith = &epi[i];
+        const double Ai = ith->grad_smth * ith->pres / (ith->dens * ith->dens);
+        double sa = 0, su = 0;
+        for (int j = 0; j < njp; ++j) {
+            const orc_epj_hydro *jth = &epj[j];
+            double dr[3], dv[3], gwi[3], gwj[3], gw[3], t[3];
+            for (int k = 0; k < 3; ++k) { dr[k] = ith->pos[k] - jth->pos[k]; dv[k] = ith->vel[k] - jth->vel[k]; }
+            const double dvdr = dot3_(dv, dr);
+            const double w_ij = (dvdr < 0) ? dvdr / sqrt(dot3_(dr, dr)) : 0;
+            const double v_sig = ith->snds + jth->snds - 3.0 * w_ij;
+            const double AV = -0.5 * v_sig * w_ij / (0.5 * (ith->dens + jth->dens)) * 0.5 * (ith->Bal + jth->Bal);
+            orc_gradW(dr, ith->smth, gwi);
+            orc_gradW(dr, jth->smth, gwj);
+            for (int k = 0; k < 3; ++k) gw[k] = 0.5 * (gwi[k] * ith->grad_smth + gwj[k] * jth->grad_smth);
+            const double Aj = jth->grad_smth * jth->pres / (jth->dens * jth->dens);
+            /* dAV/dw: w_ij inherits the rounding of the velocities it is built from */
+            const double dAV = (dvdr < 0) ? 0.5 * fabs(ith->snds + jth->snds - 6.0 * w_ij) /
+                                                (0.5 * (ith->dens + jth->dens)) * 0.5 * (ith->Bal + jth->Bal)
+                                          : 0.0;
+            const double vs = sqrt(dot3_(ith->vel, ith->vel)) + sqrt(dot3_(jth->vel, jth->vel));
+            for (int k = 0; k < 3; ++k)
+                t[k] = jth->mass * (fabs(Ai * gwi[k]) + fabs(Aj * gwj[k]) + (fabs(AV) + dAV * vs) * fabs(gw[k]));
+            sa += absmax3_(t);
+            /* dv = v_i - v_j cancels for bodies in bulk motion: a 1-ulp change of
+               either velocity moves dv by ~eps (|v_i| + |v_j|) */
+            su += jth->mass * ((fabs(Ai) + 0.5 * fabs(AV)) * vs * sqrt(dot3_(gw, gw)) +
+                               0.5 * dAV * vs * fabs(dot3_(dv, gw)));
+        }
+        acc_scale[i] = sa;
+        du_scale[i] = su;
+    }
+}
+
+void orc_drvt_term_scale(const orc_epi_drvt *epi, int nip, const orc_epj_drvt *epj, int njp, double *div_scale,
+                         double *rot_scale)
+{
+    for (int i = 0; i < nip; ++i) {
+        const orc_epi_drvt *ith = &epi[i];
+        double sd = 0, sr = 0;
+        for (int j = 0; j < njp; ++j) {
+            const orc_epj_drvt *jth = &epj[j];
+            double dr[3], dv[3], gw[3], c[3];
+            for (int k = 0; k < 3; ++k) { dr[k] = ith->pos[k] - jth->pos[k]; dv[k] = ith->vel[k] - jth->vel[k]; }
+            orc_gradW(dr, ith->smth, gw);
+            /* conditioning of dv = v_i - v_j, as for du in orc_hydro_term_scale */
+            const double vs = sqrt(dot3_(ith->vel, ith->vel)) + sqrt(dot3_(jth->vel, jth->vel));
+            const double gn = sqrt(dot3_(gw, gw));
+            sd += jth->mass * (fabs(dot3_(dv, gw)) + vs * gn);
+            c[0] = fabs(dv[1] * gw[2]) + fabs(dv[2] * gw[1]);
+            c[1] = fabs(dv[2] * gw[0]) + fabs(dv[0] * gw[2]);
+            c[2] = fabs(dv[0] * gw[1]) + fabs(dv[1] * gw[0]);
+            sr += jth->mass * (absmax3_(c) + vs * gn);
+        }
+        div_scale[i] = sd / ith->dens;
+        rot_scale[i] = sr / ith->dens;
+    }
+}
+
 /* RealPtcl::copyFromForce(const RESULT::Drvt&) Balsara switch (src/class.h:136-137). */
 double orc_balsara(double div_v, const double rot_v[3], double snds, double smth)
 {
@@ -703,7 +773,28 @@ orc_tree *orc_tree_build(const orc_state *s, const orc_params *prm)
         const int sy = s->sys ? s->sys[t->nstart[id]] : 0;
         t->sys_root[sy] = id;
     }
-    t->gstart = starts_from_counts_(cnt, n, prm->n_group, &t->ngroups);
+    /* i-groups: inside each level-2 node (the top level when the tree is
+     * shallower), consecutive leaves are packed greedily while the group stays
+     * within n_group particles */
+    {
+        int64_t *gs = (int64_t *) malloc(sizeof(int64_t) * (n + 1));
+        int64_t ng = 0;
+        const int pl = (nl - 1 < 2) ? nl - 1 : 2;
+        /* leaf range of every level-pl node: leaves whose start lies inside it */
+        for (int64_t q = t->level_off[pl]; q < t->level_off[pl + 1]; ++q) {
+            const int64_t a = lower_bound_u64_((const uint64_t *) t->nstart, t->level_off[1], (uint64_t) t->nstart[q]);
+            const int64_t b = lower_bound_u64_((const uint64_t *) t->nstart, t->level_off[1], (uint64_t) t->nend[q]);
+            int64_t run = 0;
+            for (int64_t c = a; c < b; ++c) {
+                const int64_t sz = t->nend[c] - t->nstart[c];
+                if (c == a || run + sz > prm->n_group) { gs[ng++] = t->nstart[c]; run = 0; }
+                run += sz;
+            }
+        }
+        gs[ng] = n;
+        t->gstart = gs;
+        t->ngroups = ng;
+    }
     for (int l = 0; l < nl; ++l) free(lst[l]);
     free(lst); free(lcnt); free(cnt); free(key);
     return t;
@@ -1169,6 +1260,72 @@ int orc_sim_step(orc_state *s, const orc_eos_set *eos, const orc_params *prm, do
         s->vx[p] *= prm->damping; s->vy[p] *= prm->damping; s->vz[p] *= prm->damping;
     }
     timestep_(s, nsys, dt_sys);
+    return 0;
+}
+
+int orc_state_term_scales(const orc_state *s, const orc_params *prm, double *acc_scale, double *du_scale,
+                          double *div_scale, double *rot_scale, double *grav_scale)
+{
+    orc_tree *t = orc_tree_build(s, prm);
+    orc_tree_update_maxH(t, s);
+    int32_t *lst = (int32_t *) malloc(sizeof(int32_t) * LIST_CAP_);
+    int32_t *ls = (int32_t *) malloc(sizeof(int32_t) * LIST_CAP_);
+    orc_epj_hydro *eh = (orc_epj_hydro *) malloc(sizeof(orc_epj_hydro) * LIST_CAP_);
+    orc_epj_drvt *ed = (orc_epj_drvt *) malloc(sizeof(orc_epj_drvt) * LIST_CAP_);
+    for (int64_t g = 0; g < t->ngroups; ++g) {
+        const int64_t i0 = t->gstart[g], ni = t->gstart[g + 1] - i0;
+        orc_epi_hydro ih[64];
+        orc_epi_drvt id[64];
+        for (int64_t a = 0; a < ni; ++a) {
+            const int64_t p = i0 + a;
+            ih[a].pos[0] = s->x[p]; ih[a].pos[1] = s->y[p]; ih[a].pos[2] = s->z[p];
+            ih[a].vel[0] = s->vx[p]; ih[a].vel[1] = s->vy[p]; ih[a].vel[2] = s->vz[p];
+            ih[a].smth = s->h[p]; ih[a].dens = s->rho[p]; ih[a].pres = s->p[p]; ih[a].snds = s->cs[p];
+            ih[a].grad_smth = s->gs[p]; ih[a].Bal = s->bal[p]; ih[a].AVa = 1.0; ih[a].id = p;
+            memcpy(id[a].pos, ih[a].pos, sizeof id[a].pos);
+            memcpy(id[a].vel, ih[a].vel, sizeof id[a].vel);
+            id[a].smth = s->h[p]; id[a].dens = s->rho[p];
+        }
+        int64_t nj = orc_list_sph(t, s, g, 1, 0, lst, LIST_CAP_);
+        if (nj < 0) list_fail_("scale");
+        for (int64_t q = 0; q < nj; ++q) {
+            const int64_t p = lst[q];
+            orc_epj_hydro *e = &eh[q];
+            e->pos[0] = s->x[p]; e->pos[1] = s->y[p]; e->pos[2] = s->z[p];
+            e->vel[0] = s->vx[p]; e->vel[1] = s->vy[p]; e->vel[2] = s->vz[p];
+            e->dens = s->rho[p]; e->mass = s->m[p]; e->smth = s->h[p]; e->pres = s->p[p];
+            e->grad_smth = s->gs[p]; e->snds = s->cs[p]; e->Bal = s->bal[p]; e->AVa = 1.0; e->id = p;
+        }
+        orc_hydro_term_scale(ih, (int) ni, eh, (int) nj, acc_scale + i0, du_scale + i0);
+        nj = orc_list_sph(t, s, g, 0, 0, lst, LIST_CAP_);
+        if (nj < 0) list_fail_("scale");
+        for (int64_t q = 0; q < nj; ++q) {
+            const int64_t p = lst[q];
+            ed[q].mass = s->m[p]; ed[q].pos[0] = s->x[p]; ed[q].pos[1] = s->y[p]; ed[q].pos[2] = s->z[p];
+            ed[q].vel[0] = s->vx[p]; ed[q].vel[1] = s->vy[p]; ed[q].vel[2] = s->vz[p]; ed[q].smth = s->h[p];
+        }
+        orc_drvt_term_scale(id, (int) ni, ed, (int) nj, div_scale + i0, rot_scale + i0);
+        int64_t ngj = 0, ngs = 0;
+        if (orc_list_grav(t, s, g, prm->theta, lst, LIST_CAP_, &ngj, ls, LIST_CAP_, &ngs)) list_fail_("scale");
+        for (int64_t a = 0; a < ni; ++a) {
+            const int64_t p = i0 + a;
+            const double e2 = s->h[p] * s->h[p];
+            double sg = 0;
+            for (int64_t q = 0; q < ngj; ++q) {
+                const int64_t j = lst[q];
+                const double d[3] = {s->x[p] - s->x[j], s->y[p] - s->y[j], s->z[p] - s->z[j]};
+                sg += ORC_G * s->m[j] / (dot3_(d, d) + e2);
+            }
+            for (int64_t q = 0; q < ngs; ++q) {
+                const int64_t k = ls[q];
+                const double d[3] = {s->x[p] - t->com[3 * k], s->y[p] - t->com[3 * k + 1], s->z[p] - t->com[3 * k + 2]};
+                sg += ORC_G * t->mass[k] / (dot3_(d, d) + e2);
+            }
+            grav_scale[p] = sg;
+        }
+    }
+    free(lst); free(ls); free(eh); free(ed);
+    orc_tree_free(t);
     return 0;
 }
 

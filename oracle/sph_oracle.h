@@ -126,6 +126,24 @@ int64_t orc_list_sph(const orc_tree *t, const orc_state *s, int64_t g, int kind,
 int orc_list_grav(const orc_tree *t, const orc_state *s, int64_t g, double theta,
                   int32_t *epj, int64_t cap_j, int64_t *nj, int32_t *spj, int64_t cap_s, int64_t *ns);
 
+/* Size of the summed terms, per i (test diagnostics: the scale on which sums
+ * that cancel are compared).  hydro: sum_j |m_j (A_i gWi + A_j gWj + AV gW)|_max
+ * and sum_j m_j (|A_i| + |AV|/2) (|v_i| + |v_j|) |gW|; derivative:
+ * sum_j m_j (|dv.gW| + (|v_i| + |v_j|) |gW|) / rho_i and the same with
+ * |dv x gW|_max; gravity: sum_j G m_j / (r^2 + eps_i^2).  The (|v_i| + |v_j|)
+ * factors bound how far dv = v_i - v_j moves under a 1-ulp change of either
+ * velocity (bodies in bulk motion make dv a cancelling difference); the
+ * AV terms add the same through w_ij. */
+void orc_hydro_term_scale(const orc_epi_hydro *epi, int nip, const orc_epj_hydro *epj, int njp, double *acc_scale,
+                          double *du_scale);
+void orc_drvt_term_scale(const orc_epi_drvt *epi, int nip, const orc_epj_drvt *epj, int njp, double *div_scale,
+                         double *rot_scale);
+/* The same scales for every particle of a sorted state, over the lists the
+ * step would build (symmetric for hydro, gather for the derivative, the
+ * monopole tree for gravity). */
+int orc_state_term_scales(const orc_state *s, const orc_params *prm, double *acc_scale, double *du_scale,
+                          double *div_scale, double *rot_scale, double *grav_scale);
+
 /* Morton key of one position inside a cube [lo, lo+side) with `bits` per axis. */
 uint64_t orc_morton(double x, double y, double z, const double lo[3], double side, int bits);
 
