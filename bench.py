@@ -1,25 +1,43 @@
 #!/usr/bin/env python3
-"""SPH + self-gravity steps/s on one MI355X: the 1,000,000-particle giant
-impact (900k-particle target of iron core + dunite mantle, 100k-particle
-impactor of the same structure, ANEOS iron / dunite tables regridded to
-200 x 200 log-spaced cells, 1.1 v_esc at 45 deg, theta = 0.5).
+"""SPH + self-gravity steps/s on one MI355X, plain FP64 path and the
+mixed-precision form, with the mixed form's error against FP64.
 
 One step is src/main.cpp:125-173: kick/drift/predict, tree + interaction
 lists, 4 density passes, EoS, derivative (+ Balsara), hydro force, tree
 gravity, external force, final kick, dt reduction -- all on the GPU with the
-particles resident in HBM.  The FP64 path is timed first (`value`), then the
-mixed-precision path on the same initial state; both report their dominant
-kernel against the MI355X roofline.  The CPU baseline is the oracle (the
-reference's functors restated in C, OpenMP over i-groups, the reference's
-compiler flags) timed for one step of the same workload on the host cores.
+particles resident in HBM.  Each step is timed on its own (the step call
+returns after the context's stream has drained); `value` is the FP64 path's
+steps/s over the K timed steps, `mixed` carries the same figures for the
+mixed-precision form.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Workloads (--workload):
+  impact1m     1,000,000 particles: 900k iron-core/dunite-mantle target + 100k
+               impactor of the same structure, ANEOS tables regridded to
+               200 x 200 log-spaced cells, 1.1 v_esc at 45 deg, theta 0.5,
+               10 warm-up + 90 timed steps (default)
+  granite10k   9,000 + 1,000 granite particles, 9 km/s at 45 deg, 50 steps;
+               the CPU baseline runs all 50 steps
+  theta-sweep  impact1m at theta 0.3, 0.5 and 0.7, 100 steps each precision
+  batch64      64 independent granite impacts, log-uniform sizes in
+               [5k, 200k] (3.1M particles), speeds U[5,15] km/s, angles
+               U[0,75] deg, one launch per stage for all systems; n_group and
+               n_leaf picked by fdps_sph_amd.autotune
+  impact16m    12M iron/dunite target + 4M granite impactor, 20 steps
+
+Mixed-precision error (`mixed.err`): per-particle relative error of density,
+acceleration (hydro, gravity, total) and du/dt against the FP64 path from the
+same state, at the initial state and at the FP64 state after the last step;
+`energy_drift` is |E_mixed - E_fp64| / |E_fp64| after the same number of
+steps.  The CPU baseline is the oracle (the reference's functors restated in
+C with the reference's compiler flags, OpenMP over i-groups) on the same
+workload, on the host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -29,105 +47,253 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
+METRIC = "SPH+gravity steps/s at 1M particles: mixed-precision vs FP64, with err vs FP64"
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6    # MI355X vector FP64
-FP32_PEAK_TFLOPS = 157.3   # MI355X vector FP32 (published; includes packed FP32)
+FP32_PEAK_TFLOPS = 157.3   # MI355X vector FP32
 GRAV_FLOPS = 38            # flops per P2P / P2M interaction (acc + potential)
 # algorithmic bytes per i-particle (read + result) and per j-list entry, FP64 fields
-SPH_BYTES = {"dens": (40 + 16, 32), "drvt": (64 + 40, 56), "hydro": (88 + 40, 104)}
+SPH_BYTES = {"dens": (40 + 16, 32), "drvt": (64 + 40, 56), "hydro": (88 + 40, 104), "dens_fp64": (40 + 16, 32)}
+# kernel -> (symbol prefix, FP64 kernel even in the precision form)
+KERNEL_SYMBOL = {"dens": ("k_dens", False), "drvt": ("k_drvt", False), "hydro": ("k_hydro", False),
+                 "grav": ("k_grav", False), "dens_fp64": ("k_dens", True)}
+
+
+def kernel_symbol(kernel, mixed):
+    sym, always64 = KERNEL_SYMBOL[kernel]
+    return f"{sym}_{'mixed' if mixed and not always64 else 'fp64'}"
+STAGES = ["kick_drift", "sort_tree", "density_x4", "eos", "derivative", "hydro", "gravity", "final_dt"]
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+
+DEFAULTS = {  # workload: (warmup, steps)
+    "impact1m": (10, 90), "granite10k": (5, 45), "theta-sweep": (10, 90), "batch64": (5, 20),
+    "impact16m": (2, 18)}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def load_tables(ctx, grid):
+# ------------------------------------------------------------------ workloads
+def make_workload(name, n=None):
+    from fdps_sph_amd import ic as IC
+    t0 = time.perf_counter()
+    if name in ("impact1m", "theta-sweep"):
+        n = n or 1_000_000
+        nt = int(round(0.9 * n))
+        ic = IC.iron_dunite_impact(nt, n - nt)
+        desc = (f"giant impact, {n:,} particles: {nt:,}-particle iron-core/dunite-mantle target + {n - nt:,}-particle "
+                "impactor, ANEOS iron/dunite tables regridded to 200x200, 1.1 v_esc at 45 deg")
+        tables = {"granite": None, "iron": 200, "dunite": 200}
+    elif name == "granite10k":
+        ic = IC.granite_impact(9000, 1000)
+        desc = ("giant impact, 10,000 granite particles: 9,000-particle target (R 6,371 km) + 1,000-particle "
+                "impactor (R 3,390 km), granite table, 9 km/s at 45 deg")
+        tables = {"granite": None, "iron": None, "dunite": None}
+    elif name == "batch64":
+        ic = IC.batch_of_impacts(64)
+        desc = (f"64 independent granite impacts, {int(ic['counts'].sum()):,} particles (log-uniform "
+                f"{int(ic['counts'].min()):,}..{int(ic['counts'].max()):,} per system), speeds U[5,15] km/s, "
+                "angles U[0,75] deg")
+        tables = {"granite": None, "iron": None, "dunite": None}
+    elif name == "impact16m":
+        n = n or 16_000_000
+        nt = int(round(0.75 * n))
+        ic = IC.three_material_impact(nt, n - nt)
+        desc = (f"giant impact, {n:,} particles: {nt:,}-particle iron-core/dunite-mantle target + {n - nt:,}-particle "
+                "granite impactor, ANEOS tables regridded to 200x200, 1.1 v_esc at 45 deg")
+        tables = {"granite": 200, "iron": 200, "dunite": 200}
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    log(f"[{name}] initial conditions: {len(ic['mass']):,} particles in {time.perf_counter() - t0:.1f} s")
+    return ic, desc, tables
+
+
+def load_tables(ctx, tables):
     from fdps_sph_amd import eos_tables as E
-    ctx.load_table(0, E.PATHS["granite"])
-    ctx.set_table(1, E.table("iron", grid))
-    ctx.set_table(2, E.table("dunite", grid))
+    for mat, name in enumerate(("granite", "iron", "dunite")):
+        ctx.set_table(mat, E.table(name, tables[name]))
 
 
-def energy(sim):
+def energy_parts(sim):
+    """Kinetic, internal and potential (1/2 sum m phi) energy of all particles."""
     m = sim.get("MASS")
     v = sim.get("VEL")
     u = sim.get("ENG")
     pot = sim.get("POT")
-    return float(np.sum(m * (u + 0.5 * np.sum(v * v, 1))) + 0.5 * np.sum(m * pot))
+    return {"kinetic": float(np.sum(m * 0.5 * np.sum(v * v, 1))), "internal": float(np.sum(m * u)),
+            "potential": float(0.5 * np.sum(m * pot))}
 
 
+def energy(sim):
+    return sum(energy_parts(sim).values())
+
+
+def snapshot(sim):
+    """The particle state of a simulation as initial conditions."""
+    return {"pos": sim.get("POS"), "vel": sim.get("VEL"), "mass": sim.get("MASS"), "dens": sim.get("DENS"),
+            "eng": sim.get("ENG"), "mat": sim.get("MAT"), "sys": sim.get("SYS"), "id": sim.get("ID")}
+
+
+FORCE_FIELDS = ("ID", "DENS", "ACC", "GRAV", "ENGDOT", "PRES", "DT", "DIVV", "SMTH")
+
+
+def force_fields(sim):
+    return {k: sim.get(k) for k in FORCE_FIELDS}
+
+
+# ------------------------------------------------------------------ roofline
 def kernel_roofline(stats, mixed):
-    """Achieved throughput of every interaction kernel from its live stats."""
+    """Achieved throughput of every interaction kernel from its live HIP-event stats."""
     out = {}
     for k, s in stats.items():
         if s["launches"] == 0 or s["ms"] <= 0:
             continue
         sec = s["ms"] * 1e-3
+        L = s["launches"]
         if k == "grav":
             flops = GRAV_FLOPS * s["pairs"]
             peak = FP32_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS
             ach = flops / sec / 1e12
             out[k] = {"bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
-                      "avg_ms": s["ms"] / s["launches"], "per_launch": {"interactions": s["pairs"] / s["launches"],
-                                                                        "flops": flops / s["launches"]}}
+                      "avg_ms": s["ms"] / L, "launches": L,
+                      "per_launch": {"interactions": s["pairs"] / L, "flops": flops / L, "n_i": s["n_i"] / L,
+                                     "n_epj": s["n_j"] / L, "n_spj": s["n_s"] / L}}
         else:
             bi, bj = SPH_BYTES[k]
             nbytes = bi * s["n_i"] + bj * s["n_j"]
             ach = nbytes / sec / 1e9
-            out[k] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                      "avg_ms": s["ms"] / s["launches"], "per_launch": {"bytes": nbytes / s["launches"],
-                                                                        "n_i": s["n_i"] / s["launches"],
-                                                                        "n_j": s["n_j"] / s["launches"],
-                                                                        "pairs": s["pairs"] / s["launches"]}}
+            out[k] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": ach / HBM_PEAK_GBS, "avg_ms": s["ms"] / L, "launches": L,
+                      "per_launch": {"bytes": nbytes / L, "n_i": s["n_i"] / L, "n_j": s["n_j"] / L,
+                                     "pairs_in_tiles": s["pairs"] / L}}
     return out
 
 
-def run_gpu(ic, precision, steps, warmup, theta, grid):
-    from fdps_sph_amd import Context, Simulation
-    ctx = Context(0, precision)
-    load_tables(ctx, grid)
+def measured_traffic(kernel, precision, workload):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = d.get("workloads", {}).get(workload, {}).get(kernel_symbol(kernel, precision == "mixed"))
+    return None if e is None else e.get("hbm_bytes_per_launch")
+
+
+# ------------------------------------------------------------------ runs
+def run(ctx, ic, warmup, steps, label, **params):
+    """Simulation from `ic`: warm-up steps, then `steps` steps timed one by one."""
+    from fdps_sph_amd import Simulation
     t0 = time.perf_counter()
-    sim = Simulation(ctx, ic, theta=theta)
-    log(f"[{precision}] init (upload + first force evaluation) {time.perf_counter() - t0:.2f} s")
-    init_state = {k: sim.get(k) for k in ("ID", "DENS", "ACC", "GRAV", "ENGDOT", "SMTH")}
-    e0 = energy(sim)
-    if warmup:
-        sim.step(warmup)
+    sim = Simulation(ctx, ic, **params)
+    t_init = time.perf_counter() - t0
+    init = force_fields(sim)
+    parts0 = energy_parts(sim)
+    e0 = sum(parts0.values())
+    dts = []
+    for _ in range(warmup):
+        sim.step(1)
+        dts.append(float(np.min(sim.time()[0])))
     sim.set_stats(True)
-    # sphgpu_sim_step synchronises the context's stream before it returns, so the
-    # wall clock around it brackets exactly the K steps
-    t0 = time.perf_counter()
-    sim.step(steps)
-    wall = time.perf_counter() - t0
+    ms = []
+    t_all = time.perf_counter()
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        sim.step(1)  # returns after the context's stream has drained
+        ms.append(1e3 * (time.perf_counter() - t0))
+    wall = time.perf_counter() - t_all
     stats = sim.kernel_stats()
     stages = sim.stage_ms()
-    e1 = energy(sim)
+    parts1 = energy_parts(sim)
+    e1 = sum(parts1.values())
     dt, t = sim.time()
-    log(f"[{precision}] {steps} steps in {wall:.3f} s ({1e3 * wall / steps:.2f} ms/step), dt {dt[0]:.4g} s")
-    res = {"wall": wall, "ms_per_step": 1e3 * wall / steps, "steps_per_s": steps / wall, "stats": stats,
-           "stages": stages, "init": init_state, "E0": e0, "E1": e1, "time": float(t[0]), "ngroups": None}
-    ctx.close()
-    return res
+    gb, fe = sim.list_builds()
+    ms = np.array(ms)
+    log(f"[{label}] init {t_init:.2f} s; {steps} steps in {wall:.3f} s: mean {ms.mean():.2f} ms/step, "
+        f"median {np.median(ms):.2f}, min {ms.min():.2f}; dt {float(np.min(dt)):.4g} s, t {float(np.max(t)):.4g} s")
+    return {"sim": sim, "wall": wall, "ms_per_step": 1e3 * wall / steps, "median_ms": float(np.median(ms)),
+            "steps_per_s": steps / wall, "stats": stats, "stages": stages, "init": init, "E0": e0, "E1": e1,
+            "parts0": parts0, "parts1": parts1, "warmup_dts": dts, "gather_builds_per_force_eval": gb / max(fe, 1),
+            "t": float(np.max(t)), "steps_total": warmup + steps}
 
 
-def rel_errors(a, b, ids_a, ids_b):
-    """Per-particle relative error of a against b (matched by particle id)."""
-    ia = np.argsort(ids_a)
-    ib = np.argsort(ids_b)
-    a = a[ia]
-    b = b[ib]
+def err_stats(a, b, ids_a, ids_b, scale=None):
+    """Per-particle relative error of a against b, matched by particle id:
+    max / p99 / rms of |a - b| / |b|, the norm-wise error ||a - b|| / ||b||
+    over all particles and, with `scale`, max |a - b| / scale."""
+    ia, ib = np.argsort(ids_a), np.argsort(ids_b)
+    a, b = a[ia], b[ib]
     if a.ndim == 2:
         num = np.linalg.norm(a - b, axis=1)
         den = np.linalg.norm(b, axis=1)
     else:
         num = np.abs(a - b)
         den = np.abs(b)
-    e = num / np.maximum(den, 1e-300)
-    return {"max": float(e.max()), "p99": float(np.percentile(e, 99)), "rms": float(np.sqrt(np.mean(e * e)))}
+    ok = den > 0
+    e = np.divide(num[ok], den[ok], out=np.zeros(int(ok.sum())), where=num[ok] > 0)
+    if e.size == 0:
+        e = np.zeros(1)
+    out = {"max": float(e.max()), "p99": float(np.percentile(e, 99)), "rms": float(np.sqrt(np.mean(e * e))),
+           "norm": float(np.sqrt(np.sum(num * num)) / max(np.sqrt(np.sum(den * den)), 1e-300)),
+           "nonzero_where_fp64_zero": int(np.sum((num > 0) & ~ok))}
+    if scale is not None:
+        s = scale[ib]
+        out["max_on_force_scale"] = float(np.max(np.divide(num, s, out=np.zeros_like(num), where=num > 0)))
+    return out
 
 
-def cpu_baseline(ic, grid, theta):
-    """One step of the same workload on the CPU oracle (reference flags, OpenMP)."""
+def force_errors(m, f):
+    """Errors of the mixed force evaluation m against the FP64 one f."""
+    ids_m, ids_f = m["ID"], f["ID"]
+    # |a_hydro| + |a_grav|: the size of the two forces that sum to the acceleration
+    fscale = np.linalg.norm(f["ACC"], axis=1) + np.linalg.norm(f["GRAV"], axis=1)
+    return {"dens": err_stats(m["DENS"], f["DENS"], ids_m, ids_f),
+            "acc": err_stats(m["ACC"] + m["GRAV"], f["ACC"] + f["GRAV"], ids_m, ids_f, fscale),
+            "acc_hydro": err_stats(m["ACC"], f["ACC"], ids_m, ids_f),
+            "acc_grav": err_stats(m["GRAV"], f["GRAV"], ids_m, ids_f),
+            "du_dt": err_stats(m["ENGDOT"], f["ENGDOT"], ids_m, ids_f),
+            "pres": err_stats(m["PRES"], f["PRES"], ids_m, ids_f),
+            "dt": err_stats(m["DT"], f["DT"], ids_m, ids_f),
+            "div_v": err_stats(m["DIVV"], f["DIVV"], ids_m, ids_f)}
+
+
+def compare_precisions(ctx64, ctx32, ic, warmup, steps, **params):
+    """FP64 and mixed runs of the same workload plus the mixed form's errors."""
+    F = run(ctx64, ic, warmup, steps, "fp64", **params)
+    M = run(ctx32, ic, warmup, steps, "mixed", **params)
+    err0 = force_errors(M["init"], F["init"])
+    # one force evaluation of each precision from the FP64 state after the last step
+    state = snapshot(F["sim"])
+    from fdps_sph_amd import Simulation
+    fe = force_fields(Simulation(ctx64, state, **params))
+    me = force_errors(force_fields(Simulation(ctx32, state, **params)), fe)
+    drift = abs(M["E1"] - F["E1"]) / abs(F["E1"])
+    log(f"[mixed vs fp64] speedup {M['steps_per_s'] / F['steps_per_s']:.2f}x; acc err at the evolved state "
+        f"max {me['acc']['max']:.2e} p99 {me['acc']['p99']:.2e} norm {me['acc']['norm']:.2e}; energy drift over "
+        f"{F['steps_total']} steps {drift:.2e}")
+    return F, M, {"initial_state": err0, "after_last_step": me}, drift
+
+
+def stage_dict(stages):
+    return dict(zip(STAGES, [round(float(x), 3) for x in stages]))
+
+
+def precision_block(R, mixed, workload):
+    roof = kernel_roofline(R["stats"], mixed)
+    for k, v in roof.items():
+        v["traffic"] = measured_traffic(k, "mixed" if mixed else "fp64", workload)
+    return {"steps_per_s": R["steps_per_s"], "ms_per_step": R["ms_per_step"], "median_ms_per_step": R["median_ms"],
+            "gather_builds_per_force_eval": R["gather_builds_per_force_eval"],
+            "kernels": {k: {kk: vv for kk, vv in v.items() if kk != "per_launch"} for k, v in roof.items()},
+            "per_launch": {k: v["per_launch"] for k, v in roof.items()},
+            "stages_ms": stage_dict(R["stages"])}, roof
+
+
+# ------------------------------------------------------------------ CPU baseline
+def cpu_baseline(ic, tables, theta, steps, nsys=1):
+    """The oracle on the same workload: initial force evaluation + `steps` steps."""
     from fdps_sph_amd import eos_tables as E
     from oracle import oracle as O
     n = len(ic["mass"])
@@ -138,101 +304,219 @@ def cpu_baseline(ic, grid, theta):
     s.rho[:] = ic["dens"]
     s.u[:] = ic["eng"]
     s.mat[:] = ic["mat"]
-    tabs = [O.Table(E.PATHS["granite"], kind="fast"), O.Table(cells=E.table("iron", grid), kind="fast"),
-            O.Table(cells=E.table("dunite", grid), kind="fast")]
-    sim = O.Sim(s, tabs, O.params(theta=theta), kind="fast")
+    if "sys" in ic:
+        s.sys[:] = ic["sys"]
+    tabs = [O.Table(cells=E.table(name, tables[name]), kind="fast") for name in ("granite", "iron", "dunite")]
+    sim = O.Sim(s, tabs, O.params(theta=theta, nsys=nsys), kind="fast")
     t0 = time.perf_counter()
     sim.init()
     t_init = time.perf_counter() - t0
     t0 = time.perf_counter()
-    sim.step()
-    t_step = time.perf_counter() - t0
-    log(f"[cpu oracle] {O.num_threads()} threads: init {t_init:.2f} s, one step {t_step:.2f} s")
+    for _ in range(steps):
+        sim.step()
+    t_step = (time.perf_counter() - t0) / max(steps, 1)
+    log(f"[cpu oracle] {O.num_threads()} threads: init {t_init:.2f} s, {steps} step(s) {t_step:.3f} s/step")
     return {"value": 1.0 / t_step, "unit": "steps/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"one full step of the same {n:,}-particle workload after the initial force evaluation "
-                      f"({t_step:.2f} s); CPU oracle = the reference's functors restated in C, -O3 -ffast-math "
+            "sample": f"{steps} full step(s) of the same {n:,}-particle workload after the initial force evaluation "
+                      f"({t_step:.3f} s/step); CPU oracle = the reference's functors restated in C, -O3 -ffast-math "
                       "-funroll-loops -fopenmp, OpenMP over i-groups"}
 
 
+def cpu_force_eval(ic, tables, theta):
+    """The oracle's initial force evaluation (4 density passes, EoS, derivative,
+    hydro, gravity: a step without its O(N) kick/drift) -- the bounded CPU
+    sample for workloads whose full CPU step would take minutes."""
+    from fdps_sph_amd import eos_tables as E
+    from oracle import oracle as O
+    n = len(ic["mass"])
+    s = O.State(n)
+    s.x[:], s.y[:], s.z[:] = ic["pos"].T
+    s.vx[:], s.vy[:], s.vz[:] = ic["vel"].T
+    s.m[:] = ic["mass"]
+    s.rho[:] = ic["dens"]
+    s.u[:] = ic["eng"]
+    s.mat[:] = ic["mat"]
+    tabs = [O.Table(cells=E.table(name, tables[name]), kind="fast") for name in ("granite", "iron", "dunite")]
+    sim = O.Sim(s, tabs, O.params(theta=theta), kind="fast")
+    t0 = time.perf_counter()
+    sim.init()
+    t = time.perf_counter() - t0
+    log(f"[cpu oracle] {O.num_threads()} threads: one force evaluation {t:.2f} s")
+    return {"value": 1.0 / t, "unit": "steps/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"one force evaluation (the initial one: 4 density passes, EoS, derivative, hydro, gravity) "
+                      f"of the same {n:,}-particle workload ({t:.2f} s), timed as one step"}
+
+
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """A plain copy kernel moving `nbytes` in and `nbytes` out per launch."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        torch.mul(src, 1.0, out=dst)  # an elementwise kernel: reads nbytes, writes nbytes
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": nbytes, "bytes_written": nbytes, "launches": reps}}))
+
+
+# ------------------------------------------------------------------ main
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=1_000_000, help="total particles (90/10 target/impactor)")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", choices=list(DEFAULTS), default="impact1m")
+    ap.add_argument("--n", type=int, default=None, help="total particles (impact1m / impact16m)")
     ap.add_argument("--theta", type=float, default=0.5)
-    ap.add_argument("--grid", type=int, default=200, help="EoS table size (grid x grid)")
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--eos-density-fp32", action="store_true",
+                    help="precision form: run the last density pass (the EoS input) in FP32 too")
+    ap.add_argument("--sensitivity", type=float, default=0.0, metavar="EPS",
+                    help="also run FP64 from positions perturbed by a seeded relative EPS (e.g. 1e-7, the FP32 "
+                         "rounding level) and report its energy difference to the unperturbed FP64 run")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB elementwise copy kernel 5 times (the known-size reference "
+                         "for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this "
-              "path is a design in DESIGN.md, not a measured configuration", file=sys.stderr)
+              "path is a design, not a measured configuration", file=sys.stderr)
         sys.exit(2)
+    warmup, steps = DEFAULTS[args.workload]
+    warmup = warmup if args.warmup is None else args.warmup
+    steps = steps if args.steps is None else args.steps
+    if steps < 1:
+        raise SystemExit("--steps must be >= 1")
 
-    from fdps_sph_amd import ic as IC
-    nt = int(round(0.9 * args.n))
-    t0 = time.perf_counter()
-    ic = IC.iron_dunite_impact(nt, args.n - nt)
-    log(f"initial conditions: {args.n:,} particles in {time.perf_counter() - t0:.1f} s")
+    from fdps_sph_amd import Context
+    ic, desc, tables = make_workload(args.workload, args.n)
+    n = len(ic["mass"])
+    nsys = int(ic["sys"].max()) + 1 if "sys" in ic else 1
+    precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    ctx = {}
+    for p in precs:
+        ctx[p] = Context(0, p)
+        load_tables(ctx[p], tables)
 
-    runs = {}
-    for prec in (["fp64", "mixed"] if args.precision == "both" else [args.precision]):
-        runs[prec] = run_gpu(ic, prec, args.steps, args.warmup, args.theta, args.grid)
+    params = {"theta": args.theta, "nsys": nsys, "eos_density_fp64": 0 if args.eos_density_fp32 else 1}
+    extra = {}
+    if args.workload == "batch64":
+        from fdps_sph_amd import autotune
+        tctx = ctx.get("mixed", ctx.get("fp64"))
+        tr = autotune.tune(tctx, ic, warmup=1, steps=3, log=log, **params)
+        params.update(n_group=tr.n_group, n_leaf=tr.n_leaf)
+        extra["autotune"] = {"precision": "mixed" if "mixed" in ctx else "fp64", "best": {
+            "n_group": tr.n_group, "n_leaf": tr.n_leaf, "ms_per_step": tr.ms_per_step},
+            "table": [{"n_group": a, "n_leaf": b, "ms_per_step": c} for a, b, c in tr.table]}
 
-    main_prec = "fp64" if "fp64" in runs else "mixed"
-    R = runs[main_prec]
-    roof = kernel_roofline(R["stats"], main_prec == "mixed")
-    dominant = max(roof, key=lambda k: roof[k]["avg_ms"] * R["stats"][k]["launches"])
+    sweep = [0.3, 0.5, 0.7] if args.workload == "theta-sweep" else [args.theta]
+    results = {}
+    for th in sweep:
+        params["theta"] = th
+        if len(precs) == 2:
+            F, M, err, drift = compare_precisions(ctx["fp64"], ctx["mixed"], ic, warmup, steps, **params)
+            results[th] = {"fp64": F, "mixed": M, "err": err, "drift": drift}
+        else:
+            R = run(ctx[precs[0]], ic, warmup, steps, precs[0], **params)
+            results[th] = {precs[0]: R}
+
+    th0 = 0.5 if 0.5 in results else sweep[0]
+    R0 = results[th0]
+    main_prec = "fp64" if "fp64" in R0 else "mixed"
+    R = R0[main_prec]
+    block, roof = precision_block(R, main_prec == "mixed", args.workload)
+    dominant = max(roof, key=lambda k: roof[k]["avg_ms"] * roof[k]["launches"])
     rl = roof[dominant]
     roofline = {"bound": rl["bound"], "achieved": rl["achieved"], "peak": rl["peak"], "unit": rl["unit"],
-                "frac": rl["frac"], "traffic": None, "kernel": dominant, "avg_ms": rl["avg_ms"],
-                "per_launch": rl["per_launch"]}
-    stage_names = ["kick_drift", "sort_tree", "density_x4", "eos", "derivative", "hydro", "gravity", "final_dt"]
+                "frac": rl["frac"], "traffic": rl["traffic"], "kernel": kernel_symbol(dominant, main_prec == "mixed"),
+                "avg_ms": rl["avg_ms"], "per_launch": rl["per_launch"]}
     out = {
-        "metric": "SPH+gravity steps/s at 1M particles: mixed-precision vs FP64, with err vs FP64",
+        "metric": METRIC,
         "value": R["steps_per_s"],
         "unit": "steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": R["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if main_prec == "fp64" else "f32",
         "data": "synthetic (seeded lattice bodies)",
-        "config": {"workload": f"giant impact, {args.n:,} particles: {nt:,}-particle iron-core/dunite-mantle "
-                               f"target + {args.n - nt:,}-particle impactor, ANEOS {args.grid}x{args.grid} tables, "
-                               f"1.1 v_esc at 45 deg, theta={args.theta}", "precision": main_prec,
-                   "n_group": 64, "n_leaf": 16},
+        "config": {"workload": f"{args.workload}: {desc}", "particles": n, "systems": nsys, "theta": th0,
+                   "precision": main_prec, "n_group": params.get("n_group", 64),
+                   "n_leaf": params.get("n_leaf", 16),
+                   "eos_density": "fp32" if args.eos_density_fp32 else "fp64"},
         "roofline": roofline,
-        "kernels": {k: {kk: vv for kk, vv in v.items() if kk != "per_launch"} for k, v in roof.items()},
-        "stages_ms": dict(zip(stage_names, [round(float(x), 3) for x in R["stages"]])),
+        main_prec: block,
     }
-    if "mixed" in runs and "fp64" in runs:
-        M, F = runs["mixed"], runs["fp64"]
-        ids_m, ids_f = M["init"]["ID"], F["init"]["ID"]
-        acc_m = M["init"]["ACC"] + M["init"]["GRAV"]
-        acc_f = F["init"]["ACC"] + F["init"]["GRAV"]
-        err = {"dens": rel_errors(M["init"]["DENS"], F["init"]["DENS"], ids_m, ids_f),
-               "acc": rel_errors(acc_m, acc_f, ids_m, ids_f),
-               "acc_hydro": rel_errors(M["init"]["ACC"], F["init"]["ACC"], ids_m, ids_f),
-               "acc_grav": rel_errors(M["init"]["GRAV"], F["init"]["GRAV"], ids_m, ids_f),
-               "du_dt": rel_errors(M["init"]["ENGDOT"], F["init"]["ENGDOT"], ids_m, ids_f)}
-        mroof = kernel_roofline(M["stats"], True)
-        out["mixed"] = {
-            "steps_per_s": M["steps_per_s"], "ms_per_step": M["ms_per_step"],
-            "speedup_vs_fp64": M["steps_per_s"] / F["steps_per_s"],
-            "err_vs_fp64_first_force_eval": err,
-            "energy_drift_vs_fp64": abs(M["E1"] - F["E1"]) / abs(F["E1"]),
-            "energy_fp64_rel_change": abs(F["E1"] - F["E0"]) / abs(F["E0"]),
-            "kernels": {k: {kk: vv for kk, vv in v.items() if kk != "per_launch"} for k, v in mroof.items()},
-            "stages_ms": dict(zip(stage_names, [round(float(x), 3) for x in M["stages"]])),
-        }
+    if nsys > 1:
+        out["particle_steps_per_s"] = R["steps_per_s"] * n
+    if "mixed" in R0 and "fp64" in R0:
+        mb, _ = precision_block(R0["mixed"], True, args.workload)
+        mb["dtype"] = "f32 pair arithmetic relative to the group centre, Kahan-summed; f64 state, EoS and integrator"
+        mb["speedup_vs_fp64"] = R0["mixed"]["steps_per_s"] / R0["fp64"]["steps_per_s"]
+        mb["err_vs_fp64"] = R0["err"]
+        mb["energy_drift_vs_fp64"] = R0["drift"]
+        mb["energy_drift_steps"] = R0["fp64"]["steps_total"]
+        mb["energy_fp64_rel_change"] = abs(R0["fp64"]["E1"] - R0["fp64"]["E0"]) / abs(R0["fp64"]["E0"])
+        F0, M0 = R0["fp64"], R0["mixed"]
+        mb["energy_diagnostics"] = {
+            "initial_offset": abs(M0["E0"] - F0["E0"]) / abs(F0["E0"]),
+            "change_difference": abs((M0["E1"] - M0["E0"]) - (F0["E1"] - F0["E0"])) / abs(F0["E0"]),
+            "fp64_parts_initial": F0["parts0"], "fp64_parts_final": F0["parts1"],
+            "mixed_minus_fp64_parts_initial": {k: M0["parts0"][k] - F0["parts0"][k] for k in F0["parts0"]},
+            "mixed_minus_fp64_parts_final": {k: M0["parts1"][k] - F0["parts1"][k] for k in F0["parts1"]},
+            "warmup_dt_rel_diff_max": max([abs(a - b) / b for a, b in zip(M0["warmup_dts"], F0["warmup_dts"])],
+                                          default=0.0),
+            "final_time": {"fp64": F0["t"], "mixed": M0["t"]}}
+        if nsys > 1:
+            mb["particle_steps_per_s"] = R0["mixed"]["steps_per_s"] * n
+        out["mixed"] = mb
+        out["mixed_steps_per_s"] = mb["steps_per_s"]
+        out["speedup_mixed_vs_fp64"] = mb["speedup_vs_fp64"]
+    if len(sweep) > 1:
+        out["theta_sweep"] = []
+        for th in sweep:
+            r = results[th]
+            e = {"theta": th, "fp64_steps_per_s": r["fp64"]["steps_per_s"],
+                 "mixed_steps_per_s": r["mixed"]["steps_per_s"],
+                 "speedup": r["mixed"]["steps_per_s"] / r["fp64"]["steps_per_s"],
+                 "energy_drift_vs_fp64": r["drift"], "steps": r["fp64"]["steps_total"],
+                 "err_vs_fp64": r["err"]}
+            out["theta_sweep"].append(e)
+    if args.sensitivity > 0 and "fp64" in R0:
+        from fdps_sph_amd.ic import SplitMix64
+        pic = dict(ic)
+        rnd = SplitMix64(4242).uniform(3 * n).reshape(n, 3) * 2.0 - 1.0
+        pic["pos"] = ic["pos"] * (1.0 + args.sensitivity * rnd)
+        P = run(ctx["fp64"], pic, warmup, steps, f"fp64, positions perturbed by {args.sensitivity:g}", **params)
+        F0 = R0["fp64"]
+        out["fp64_sensitivity"] = {
+            "perturbation": args.sensitivity,
+            "energy_rel_diff": abs(P["E1"] - F0["E1"]) / abs(F0["E1"]),
+            "initial_offset": abs(P["E0"] - F0["E0"]) / abs(F0["E0"]),
+            "parts_diff_final": {k: P["parts1"][k] - F0["parts1"][k] for k in F0["parts1"]},
+            "steps": F0["steps_total"]}
+        log(f"[fp64 sensitivity] perturbation {args.sensitivity:g}: energy difference after {F0['steps_total']} "
+            f"steps {out['fp64_sensitivity']['energy_rel_diff']:.2e}")
+    out.update(extra)
+    out["graph_captured"] = False
+
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(ic, args.grid, args.theta)
+        if args.workload == "granite10k":
+            out["cpu_baseline"] = cpu_baseline(ic, tables, th0, warmup + steps, nsys)
+        elif args.workload == "impact16m":
+            out["cpu_baseline"] = cpu_force_eval(ic, tables, th0)
+        else:
+            out["cpu_baseline"] = cpu_baseline(ic, tables, th0, 1, nsys)
     print(json.dumps(out), flush=True)
+    for c in ctx.values():
+        c.close()
 
 
 if __name__ == "__main__":

@@ -126,6 +126,8 @@ constexpr int kStages = 8;
 }  // namespace
 
 // ------------------------------------------------------------------ simulation state
+constexpr int kStatKinds = 5;
+
 struct SimState {
     sphgpu_sim_params prm{};
     TreeParams tp{};
@@ -144,11 +146,15 @@ struct SimState {
     bool ready = false;
     // per-kernel statistics (density x4, derivative, hydro, gravity per step)
     bool stats = false;
+    // kinds: 0 density, 1 derivative, 2 hydro, 3 gravity, 4 density pass run in
+    // FP64 by the precision form
     hipEvent_t kev[7][2] = {};
     int kkind[7] = {0, 0, 0, 0, 1, 2, 3};
-    double kms[4] = {0};
-    int64_t klaunch[4] = {0};
-    DevBuf<long long> kcount;  // [4][4]: n_i, n_j, n_s, pairs
+    double kms[kStatKinds] = {0};
+    int64_t klaunch[kStatKinds] = {0};
+    DevBuf<long long> kcount;  // [kStatKinds][4]: n_i, n_j, n_s, pairs
+    int64_t gather_builds = 0; // gather-list constructions since init
+    int64_t force_evals = 0;   // force evaluations (sim_forces calls) since init
     ~SimState() {
         for (auto &e : ev)
             if (e) (void) hipEventDestroy(e);
@@ -498,10 +504,11 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
     const int G = (int) S.tree.ngroups;
     const int32_t *gs = S.tree.gstart.get();
     int kslot = 0;
-    auto kbegin = [&](const Lists *Lp, bool spj) {
+    auto kbegin = [&](const Lists *Lp, bool spj, int kind) {
         if (!S.stats) return;
+        S.kkind[kslot] = kind;
         hipLaunchKernelGGL(k_list_counts, dim3(nblk(G, 256)), dim3(256), 0, s, G, gs, Lp->cnt.get(),
-                           spj ? Lp->scnt.get() : nullptr, S.kcount.get() + 4 * S.kkind[kslot]);
+                           spj ? Lp->scnt.get() : nullptr, S.kcount.get() + 4 * kind);
         SPH_LAUNCH_CHECK();
         SPH_HIP(hipEventRecord(S.kev[kslot][0], s));
     };
@@ -511,24 +518,37 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
         ++kslot;
     };
     Lists &L = S.lsph;
+    ++S.force_evals;
     auto lref = [&]() { return ListRef{L.idx.get(), nullptr, L.cnt.get(), L.cap}; };
-    // 4 density passes, each with lists from the current h (src/main.cpp:138-140)
+    // 4 density passes, each over lists that cover the current h
+    // (src/main.cpp:138-140).  One gather list, built with a margin on h, serves
+    // every pass (and the derivative pass) while each group's search box still
+    // fits inside the box it was built for; see gather_lists_valid.
     for (int loop = 0; loop <= 3; ++loop) {
-        build_lists_sph(S.tree, S.tp, P, 0, L, s);
+        if (loop == 0 || !gather_lists_valid(S.tree, P, L, s)) {
+            build_lists_sph(S.tree, S.tp, P, 0, L, s, kGatherMargin);
+            ++S.gather_builds;
+        }
         DensArgs a{G, gs, P.x, P.y, P.z, P.m, P.h, P.m, P.x, P.y, P.z, lref(), P.rho, P.h, P.ngb};
-        kbegin(&L, false);
-        if (mixed) launch_dens_mixed(a, s);
-        else launch_dens_fp64(a, s);
+        // the density of the last pass feeds the EoS; the precision form keeps
+        // it FP64 unless asked otherwise (the earlier passes only steer h)
+        const bool d64 = !mixed || (loop == 3 && S.prm.eos_density_fp64);
+        kbegin(&L, false, (mixed && d64) ? 4 : 0);
+        if (d64) launch_dens_fp64(a, s);
+        else launch_dens_mixed(a, s);
         kend();
     }
     rec_ev(3);
     launch_eos_pressure(n, P.mat, P.rho, P.u, c->d_mats.get(), SPHGPU_MAX_MATERIALS, P.p, P.cs, s);
     rec_ev(4);
-    build_lists_sph(S.tree, S.tp, P, 0, L, s);
+    if (!gather_lists_valid(S.tree, P, L, s)) {
+        build_lists_sph(S.tree, S.tp, P, 0, L, s, kGatherMargin);
+        ++S.gather_builds;
+    }
     {
         DrvtArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.cs, P.m, P.x, P.y, P.z, P.vx, P.vy, P.vz,
                    lref(), P.divv, P.rotx, P.roty, P.rotz, P.gs, P.bal};
-        kbegin(&L, false);
+        kbegin(&L, false, 1);
         if (mixed) launch_drvt_mixed(a, s);
         else launch_drvt_fp64(a, s);
         kend();
@@ -540,7 +560,7 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
         HydroArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal,
                     P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal, P.m,
                     lref(), P.ax, P.ay, P.az, P.du, P.dti};
-        kbegin(&L, false);
+        kbegin(&L, false, 2);
         if (mixed) launch_hydro_mixed(a, s);
         else launch_hydro_fp64(a, s);
         kend();
@@ -554,7 +574,7 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
                    t.comx.get(), t.comy.get(), t.comz.get(), t.mass.get(),
                    ListRef{S.lgrav.sidx.get(), nullptr, S.lgrav.scnt.get(), S.lgrav.scap},
                    P.gx, P.gy, P.gz, P.pot};
-        kbegin(&S.lgrav, true);
+        kbegin(&S.lgrav, true, 3);
         if (mixed) launch_grav_mixed(a, s);
         else launch_grav_fp64(a, s);
         kend();
@@ -846,6 +866,7 @@ void sphgpu_sim_default_params(sphgpu_sim_params *p) {
     p->nsys = 1;
     p->list_cap_sph = 4096;
     p->list_cap_epj = 8192;
+    p->eos_density_fp64 = 1;
     p->list_cap_spj = 8192;
 }
 
@@ -1045,6 +1066,16 @@ int32_t sphgpu_sim_stage_ms(sphgpu_ctx *c, double *ms, int32_t n) {
     });
 }
 
+int32_t sphgpu_sim_list_builds(sphgpu_ctx *c, int64_t *gather_builds, int64_t *force_evals) {
+    return guard([&] {
+        check_ctx(c);
+        SimState &S = sim_of(c);
+        if (!gather_builds || !force_evals) throw Error(SPHGPU_ERR_ARG, "null output");
+        *gather_builds = S.gather_builds;
+        *force_evals = S.force_evals;
+    });
+}
+
 int32_t sphgpu_sim_set_stats(sphgpu_ctx *c, int32_t on) {
     return guard([&] {
         check_ctx(c);
@@ -1054,11 +1085,11 @@ int32_t sphgpu_sim_set_stats(sphgpu_ctx *c, int32_t on) {
         for (auto &p : S.kev)
             for (auto &e : p)
                 if (!e) SPH_HIP(hipEventCreate(&e));
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < kStatKinds; ++k) {
             S.kms[k] = 0;
             S.klaunch[k] = 0;
         }
-        SPH_HIP(hipMemset(S.kcount.ensure(16), 0, sizeof(long long) * 16));
+        SPH_HIP(hipMemset(S.kcount.ensure(4 * kStatKinds), 0, sizeof(long long) * 4 * kStatKinds));
     });
 }
 
@@ -1068,9 +1099,9 @@ int32_t sphgpu_sim_kernel_stats(sphgpu_ctx *c, sphgpu_kernel_stat *out, int32_t 
         SimState &S = sim_of(c);
         if (!out || n < 0) throw Error(SPHGPU_ERR_ARG, "bad output");
         SPH_HIP(hipStreamSynchronize(c->stream));
-        long long cnt[16] = {0};
+        long long cnt[4 * kStatKinds] = {0};
         if (S.kcount.get()) SPH_HIP(hipMemcpy(cnt, S.kcount.get(), sizeof(cnt), hipMemcpyDeviceToHost));
-        for (int k = 0; k < std::min(n, 4); ++k) {
+        for (int k = 0; k < std::min(n, kStatKinds); ++k) {
             out[k].ms = S.kms[k];
             out[k].launches = S.klaunch[k];
             out[k].n_i = cnt[4 * k + 0];

@@ -219,10 +219,22 @@ struct Lists {
     DevBuf<int32_t> sidx, scnt;
     int64_t cap = 0, scap = 0;
     DevBuf<int32_t> frontier;
-    DevBuf<int32_t> err;
+    int64_t fcap = 0;        // per-workgroup BFS frontier capacity
+    DevBuf<int32_t> err, flag;
+    DevBuf<double> sbox;     // gather lists: per-group search box (lo[3], hi[3])
+    int kind = -1;           // kind of the lists held (-1 none, 0 gather, 1 symmetric, 2 gravity)
 };
-// kind 0 gather, 1 symmetric.  Returns false on capacity overflow.
-bool build_lists_sph(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, hipStream_t s);
+// Search-radius factor of the gather lists that the density passes reuse:
+// 3 % on h covers the change of h over the density iterations of a step.
+constexpr double kGatherMargin = 1.03;
+// kind 0 gather, 1 symmetric.  Gather lists use the search radius 2.5 h hfac
+// (hfac >= 1: a superset that stays usable while h grows); symmetric lists are
+// always exact.  Capacities grow until the lists fit.
+bool build_lists_sph(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, hipStream_t s,
+                     double hfac = 1.0);
+// True when the gather lists in L still cover every particle's current support
+// (same tree, positions unchanged, h possibly changed).  Synchronises s.
+bool gather_lists_valid(const Tree &t, const PArrays &P, Lists &L, hipStream_t s);
 bool build_lists_grav(const Tree &t, const TreeParams &tp, const PArrays &P, Lists &L, hipStream_t s);
 int64_t lists_max_count(const Lists &L, int64_t ngroups, bool spj, hipStream_t s);
 

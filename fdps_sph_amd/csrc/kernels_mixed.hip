@@ -6,7 +6,11 @@
 // from that centre, formed in FP64 and rounded once, so dr and dv carry
 // relative errors of order 2^-24 of the group's extent rather than of the
 // absolute coordinates.  Pair arithmetic is FP32 with the hardware rsqrt and
-// reciprocal; the kernel polynomials use the algebraically equal forms
+// reciprocal; the per-i kernel constants (1/H, norm) are formed in FP64 and
+// rounded once.  The error floor of this form is the FP32 rounding of the
+// pair offsets themselves: half an ulp of s = r/H, amplified by
+// |d ln W / d ln s| (2 to 13 over the support), gives about 1e-7 rms relative
+// error in the density sums.  The kernel polynomials use the algebraically equal forms
 //   W'(s) = -22 s (1 + 7 s + 16 s^2) (1 - s)^7 * norm
 // that avoid the cancellation of the reference's (src/kernel.h:28-29) form.
 // Sums: each 64-j tile is accumulated in FP32 and the tile sums are combined
@@ -49,6 +53,18 @@ __device__ __forceinline__ double wave_centre(double v, bool act) {
 }
 
 constexpr float kNorm = (float) (1365.0 / 64.0 / kPi);
+constexpr double kNormD = 1365.0 / 64.0 / kPi;
+
+// Per-particle kernel constants, formed in FP64 and rounded once: every pair
+// term of particle i is scaled by norm, so an error there would bias the
+// whole sum rather than average out over the pairs.
+struct KernelConsts {
+    float H, invH, norm;
+};
+__device__ __forceinline__ KernelConsts kernel_consts(double h) {
+    const double Hd = kSupport * h;
+    return {(float) Hd, (float) (1.0 / Hd), (float) (kNormD / (Hd * Hd * Hd))};
+}
 
 // bit k set when j-tile entry k lies inside this lane's (widened) support
 __device__ __forceinline__ uint64_t support_mask(const float *sx, const float *sy, const float *sz, int nt, float xi,
@@ -73,9 +89,8 @@ __global__ __launch_bounds__(kWave) void k_dens_mixed(DensArgs a) {
     const double cx = wave_centre(xd, act), cy = wave_centre(yd, act), cz = wave_centre(zd, act);
     const float xi = (float) (xd - cx), yi = (float) (yd - cy), zi = (float) (zd - cz);
     const double hid = a.hi[ip];
-    const float H = (float) (kSupport * hid);
-    const float invH = rcp(H);
-    const float norm = kNorm * invH * invH * invH;
+    const KernelConsts kc = kernel_consts(hid);
+    const float H = kc.H, invH = kc.invH, norm = kc.norm;
     const float rcut2 = act ? H * H * 1.0001f : -1.f;
     Kahan dens;
     int ngb = 0;
@@ -134,11 +149,9 @@ __global__ __launch_bounds__(kWave) void k_drvt_mixed(DrvtArgs a) {
     const float xi = (float) (xd - cx), yi = (float) (yd - cy), zi = (float) (zd - cz);
     const float vxi = (float) (vxd - cvx), vyi = (float) (vyd - cvy), vzi = (float) (vzd - cvz);
     const double hid = a.hi[ip], rhoi = a.rhoi[ip];
-    const float hi = (float) hid;
-    const float H = (float) (kSupport * hid);
-    const float invH = rcp(H);
-    const float norm = kNorm * invH * invH * invH;
-    const float reg = 1.0e-6f * hi;
+    const KernelConsts kc = kernel_consts(hid);
+    const float H = kc.H, invH = kc.invH, norm = kc.norm;
+    const float reg = (float) (1.0e-6 * hid);
     const float rcut2 = act ? H * H * 1.0001f : -1.f;
     Kahan divv, rx, ry, rz, gsum;
     const int64_t lb = list_begin(a.L, g);
@@ -218,9 +231,8 @@ __global__ __launch_bounds__(kWave) void k_hydro_mixed(HydroArgs a) {
     const float xi = (float) (xd - cx), yi = (float) (yd - cy), zi = (float) (zd - cz);
     const float vxi = (float) (vxd - cvx), vyi = (float) (vyd - cvy), vzi = (float) (vzd - cvz);
     const double hid = a.hi[ip], rhoid = a.rhoi[ip];
-    const float Hi = (float) (kSupport * hid);
-    const float invHi = rcp(Hi);
-    const float normi = kNorm * invHi * invHi * invHi;
+    const KernelConsts kc = kernel_consts(hid);
+    const float Hi = kc.H, invHi = kc.invH, normi = kc.norm;
     const float regi = (float) (1.0e-6 * hid);
     const float rcut2i = act ? Hi * Hi * 1.0001f : -1.f;
     const float rhoi = (float) rhoid, csi = (float) a.csi[ip], bali = (float) a.bali[ip];

@@ -21,8 +21,8 @@ namespace sphgpu {
 namespace {
 
 constexpr int kChunk = 4096;       // bbox partial-reduction chunk
-constexpr int kListSlots = 2048;   // persistent list-builder workgroups
-constexpr int64_t kFrontier = 16384;  // per-workgroup BFS frontier capacity
+constexpr int kListSlots = 8192;      // persistent list-builder workgroups (32 waves per CU)
+constexpr int64_t kFrontier0 = 4096;  // initial per-workgroup BFS frontier capacity (doubled on overflow)
 
 __device__ __forceinline__ uint64_t expand_bits(uint64_t v) {
     v &= 0x1fffffULL;
@@ -323,6 +323,8 @@ struct ListArgs {
     int32_t *frontier;
     int64_t fcap;
     double theta2;
+    double hfac;    // gather search radius = 2.5 h hfac (hfac > 1 makes a reusable superset list)
+    double *sbox;   // optional, per group: the search box the list was built for (lo[3], hi[3])
     int32_t *err;
 };
 
@@ -356,7 +358,7 @@ __device__ __forceinline__ void group_box(const ListArgs &a, int i0, int ni, GBo
     const bool act = lane < ni;
     const int p = i0 + (act ? lane : 0);
     const double c[3] = {a.x[p], a.y[p], a.z[p]};
-    const double H = kSupport * a.h[p];
+    const double H = kSupport * a.h[p] * a.hfac;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         b.glo[k] = wave_min(act ? c[k] : INFINITY);
@@ -515,6 +517,7 @@ __global__ __launch_bounds__(kWave) void k_list_sph(ListArgs a) {
         group_box(a, i0, ni, b);
         int32_t *fa = fa0, *fb = fb0;
         if (lane == 0) fa[0] = a.sys_root[a.sys[i0]];
+        if (a.sbox && lane < 6) a.sbox[(int64_t) g * 6 + lane] = (lane < 3) ? b.slo[lane] : b.shi[lane - 3];
         __syncthreads();
         int nf = 1;
         auto test = [&](int node) { return node_test<KIND>(a, node, b); };
@@ -580,6 +583,33 @@ __global__ __launch_bounds__(kWave) void k_list_grav(ListArgs a) {
         }
         __syncthreads();
     }
+}
+
+// A gather list built for the search box sbox[g] still holds every particle
+// within 2.5 h_i of each i of group g when the group's current search box
+// (h_i as now) lies inside sbox[g]: the list is the sorted set of particle
+// indices inside that box, so it is then a superset of the exact list with the
+// in-support particles in the same order.  Sets *flag when some group's box
+// has outgrown its list.
+__global__ __launch_bounds__(kWave) void k_list_box_check(int ngroups, const int32_t *gstart, const double *x,
+                                                          const double *y, const double *z, const double *h,
+                                                          const double *sbox, int32_t *flag) {
+    const int g = blockIdx.x;
+    if (g >= ngroups) return;
+    const int lane = threadIdx.x;
+    const int i0 = gstart[g], ni = gstart[g + 1] - i0;
+    const bool act = lane < ni;
+    const int p = i0 + (act ? lane : 0);
+    const double c[3] = {x[p], y[p], z[p]};
+    const double H = kSupport * h[p];
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double lo = wave_min(act ? c[k] - H : INFINITY);
+        const double hi = wave_max(act ? c[k] + H : -INFINITY);
+        bad = bad || (lo < sbox[(int64_t) g * 6 + k]) || (hi > sbox[(int64_t) g * 6 + 3 + k]);
+    }
+    if (lane == 0 && bad) atomicOr(flag, 1);
 }
 
 inline int nblk(int64_t n, int b) { return (int) ((n + b - 1) / b); }
@@ -774,8 +804,10 @@ static ListArgs list_args(const Tree &t, const TreeParams &tp, const PArrays &P,
     a.out = L.idx.get(); a.cnt = L.cnt.get(); a.cap = L.cap;
     a.sout = L.sidx.get(); a.scnt = L.scnt.get(); a.scap = L.scap;
     a.frontier = L.frontier.get();
-    a.fcap = kFrontier;
+    a.fcap = L.fcap;
     a.theta2 = tp.theta * tp.theta;
+    a.hfac = 1.0;
+    a.sbox = nullptr;
     a.err = L.err.get();
     return a;
 }
@@ -786,7 +818,8 @@ static int list_grid(int64_t ngroups) {
     return (int) std::max<int64_t>(8, b);
 }
 
-static bool run_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, hipStream_t s) {
+static bool run_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, double hfac,
+                      hipStream_t s) {
     const int64_t G = t.ngroups;
     L.idx.ensure((size_t) G * L.cap);
     L.cnt.ensure(G);
@@ -794,11 +827,17 @@ static bool run_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int
         L.sidx.ensure((size_t) G * L.scap);
         L.scnt.ensure(G);
     }
+    if (L.fcap <= 0) L.fcap = kFrontier0;
     const int grid = list_grid(G);
-    L.frontier.ensure((size_t) grid * 2 * kFrontier);
+    L.frontier.ensure((size_t) grid * 2 * L.fcap);
     L.err.ensure(1);
     SPH_HIP(hipMemsetAsync(L.err.get(), 0, sizeof(int32_t), s));
     ListArgs a = list_args(t, tp, P, L);
+    if (kind == 0) {  // gather lists remember their search boxes (reuse check)
+        L.sbox.ensure((size_t) G * 6);
+        a.sbox = L.sbox.get();
+        a.hfac = hfac;
+    }
     if (kind == 0) hipLaunchKernelGGL(k_list_sph<0>, dim3(grid), dim3(kWave), 0, s, a);
     else if (kind == 1) hipLaunchKernelGGL(k_list_sph<1>, dim3(grid), dim3(kWave), 0, s, a);
     else hipLaunchKernelGGL(k_list_grav, dim3(grid), dim3(kWave), 0, s, a);
@@ -806,21 +845,39 @@ static bool run_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int
     int32_t err = 0;
     SPH_HIP(hipMemcpyAsync(&err, L.err.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
     SPH_HIP(hipStreamSynchronize(s));
-    if (err & 4) throw Error(-2, "tree-walk frontier exceeds its capacity (16384 nodes per level)");
-    if (err & 1) { L.cap *= 2; return false; }
-    if (err & 2) { L.scap *= 2; return false; }
-    return true;
+    bool again = false;
+    if (err & 4) { L.fcap *= 2; again = true; }
+    if (err & 1) { L.cap *= 2; again = true; }
+    if (err & 2) { L.scap *= 2; again = true; }
+    L.kind = again ? -1 : kind;
+    return !again;
 }
 
-bool build_lists_sph(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, hipStream_t s) {
-    for (int attempt = 0; attempt < 8; ++attempt)
-        if (run_lists(t, tp, P, kind, L, s)) return true;
+bool build_lists_sph(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, hipStream_t s,
+                     double hfac) {
+    if (kind != 0) hfac = 1.0;  // symmetric lists are exact: v_sig_max runs over the whole list
+    for (int attempt = 0; attempt < 12; ++attempt)
+        if (run_lists(t, tp, P, kind, L, hfac, s)) return true;
     throw Error(-2, "interaction list capacity could not be grown enough");
 }
 
+bool gather_lists_valid(const Tree &t, const PArrays &P, Lists &L, hipStream_t s) {
+    if (L.kind != 0 || t.ngroups <= 0) return false;
+    L.flag.ensure(1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s, (int64_t) 1, L.flag.get(), 0);
+    SPH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_list_box_check, dim3((unsigned) t.ngroups), dim3(kWave), 0, s, (int) t.ngroups,
+                       t.gstart.get(), P.x, P.y, P.z, P.h, L.sbox.get(), L.flag.get());
+    SPH_LAUNCH_CHECK();
+    int32_t flag = 0;
+    SPH_HIP(hipMemcpyAsync(&flag, L.flag.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SPH_HIP(hipStreamSynchronize(s));
+    return flag == 0;
+}
+
 bool build_lists_grav(const Tree &t, const TreeParams &tp, const PArrays &P, Lists &L, hipStream_t s) {
-    for (int attempt = 0; attempt < 8; ++attempt)
-        if (run_lists(t, tp, P, 2, L, s)) return true;
+    for (int attempt = 0; attempt < 12; ++attempt)
+        if (run_lists(t, tp, P, 2, L, 1.0, s)) return true;
     throw Error(-2, "gravity list capacity could not be grown enough");
 }
 

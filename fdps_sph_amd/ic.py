@@ -51,13 +51,20 @@ class SplitMix64:
         return (self.next_u64(n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
 
 
-def _lattice(dx: float, rmax: float, rmin: float = 0.0) -> np.ndarray:
+def _lattice_r(dx: float, rmax: float, rmin: float = 0.0):
+    """Cubic lattice points (cell centres, spacing dx) with rmin < r <= rmax,
+    in lattice (x, y, z index) order, and their radii."""
     k = int(math.ceil(rmax / dx)) + 1
     a = (np.arange(-k, k + 1) + 0.5) * dx
     X, Y, Z = np.meshgrid(a, a, a, indexing="ij")
     p = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
     r = np.sqrt((p * p).sum(1))
-    return p[(r <= rmax) & (r > rmin)]
+    sel = (r <= rmax) & (r > rmin)
+    return p[sel], r[sel]
+
+
+def _lattice(dx: float, rmax: float, rmin: float = 0.0) -> np.ndarray:
+    return _lattice_r(dx, rmax, rmin)[0]
 
 
 def _trim(p: np.ndarray, n: int, rng: SplitMix64) -> np.ndarray:
@@ -76,11 +83,21 @@ def _shell(n: int, rho: float, m: float, rmin: float, rng: SplitMix64, jitter=0.
     vol = n * m / rho
     rmax = (rmin ** 3 + 3.0 * vol / (4.0 * math.pi)) ** (1.0 / 3.0)
     dx = (m / rho) ** (1.0 / 3.0)
-    p = _lattice(dx, rmax, rmin)
+    # boundary effects: widen the outer radius in steps of 0.2 % until the shell
+    # holds n points.  One lattice wide enough for every radius tried is cut
+    # once; a point's coordinates and its lattice order do not depend on the
+    # extent of the lattice it was generated in.
+    wide = 1.25
+    pw, rw = _lattice_r(dx, rmax * wide, rmin)
+    rs = np.sort(rw)
     grow = 1.0
-    while len(p) < n:  # boundary effects: widen the outer radius a little
+    while np.searchsorted(rs, rmax * grow, side="right") < n:
         grow += 0.002
-        p = _lattice(dx, rmax * grow, rmin)
+        if rmax * grow > rmax * wide:
+            wide *= 1.25
+            pw, rw = _lattice_r(dx, rmax * wide, rmin)
+            rs = np.sort(rw)
+    p = pw[rw <= rmax * grow]
     p = _trim(p, n, rng)
     p = p + (rng.uniform(3 * n).reshape(n, 3) - 0.5) * (2.0 * jitter * dx)
     return p, rmax, dx
@@ -143,6 +160,17 @@ def iron_dunite_impact(n_target=900_000, n_impactor=100_000, M_target=5.97e24, c
     nc_i = int(round(core_mass_frac * n_impactor))
     return impact([(nc_t, IRON), (n_target - nc_t, DUNITE)], [(nc_i, IRON), (n_impactor - nc_i, DUNITE)], m, m,
                   vesc_factor, angle, seed, speed_in_vesc=True)
+
+
+def three_material_impact(n_target=12_000_000, n_impactor=4_000_000, M_target=5.97e24, core_mass_frac=0.3,
+                          vesc_factor=1.1, angle=45.0, seed=1004):
+    """Iron core + dunite mantle target and a granite impactor, equal particle
+    mass (the impactor mass follows from its particle count), 1.1 v_esc at
+    45 deg: the 16M-particle three-material workload."""
+    m = M_target / n_target
+    nc_t = int(round(core_mass_frac * n_target))
+    return impact([(nc_t, IRON), (n_target - nc_t, DUNITE)], [(n_impactor, GRANITE)], m, m, vesc_factor, angle, seed,
+                  speed_in_vesc=True)
 
 
 def batch_of_impacts(n_systems=64, n_min=5_000, n_max=200_000, seed=1003):

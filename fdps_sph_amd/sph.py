@@ -181,14 +181,20 @@ class Simulation:
         check(self._L.sphgpu_sim_stage_ms(self.ctx.handle, ptr(ms), 8))
         return ms
 
+    def list_builds(self) -> tuple[int, int]:
+        """(gather-list constructions, force evaluations) since init."""
+        g, f = C.c_int64(), C.c_int64()
+        check(self._L.sphgpu_sim_list_builds(self.ctx.handle, C.byref(g), C.byref(f)))
+        return g.value, f.value
+
     def set_stats(self, on: bool = True):
         check(self._L.sphgpu_sim_set_stats(self.ctx.handle, int(bool(on))))
 
     def kernel_stats(self) -> dict:
-        arr = (KernelStat * 4)()
-        check(self._L.sphgpu_sim_kernel_stats(self.ctx.handle, arr, 4))
-        names = ["dens", "drvt", "hydro", "grav"]
-        return {names[k]: {f: getattr(arr[k], f) for f, _ in KernelStat._fields_} for k in range(4)}
+        names = ["dens", "drvt", "hydro", "grav", "dens_fp64"]
+        arr = (KernelStat * len(names))()
+        check(self._L.sphgpu_sim_kernel_stats(self.ctx.handle, arr, len(names)))
+        return {names[k]: {f: getattr(arr[k], f) for f, _ in KernelStat._fields_} for k in range(len(names))}
 
     def lists(self, kind: int):
         ng, nj, ns = C.c_int64(), C.c_int64(), C.c_int64()

@@ -148,6 +148,10 @@ typedef struct {
     int32_t list_cap_sph;    /* per-group j-list capacity, default 4096 */
     int32_t list_cap_epj;    /* per-group gravity particle-list capacity, default 8192 */
     int32_t list_cap_spj;    /* per-group gravity node-list capacity, default 8192 */
+    int32_t eos_density_fp64; /* precision form only: run the last density pass, whose
+                                 density feeds the EoS, in FP64 (1, default) or FP32 (0).
+                                 The EoS tables amplify density errors by
+                                 d ln P / d ln rho (20 and more near the cold curve). */
 } sphgpu_sim_params;
 
 void sphgpu_sim_default_params(sphgpu_sim_params *p);
@@ -181,12 +185,20 @@ int32_t sphgpu_sim_get_time(sphgpu_ctx *ctx, double *dt_sys, double *time_sys);
  * [4] derivative, [5] hydro, [6] gravity, [7] final kick + dt; n <= 8. */
 int32_t sphgpu_sim_stage_ms(sphgpu_ctx *ctx, double *ms, int32_t n);
 
+/* Gather-list constructions and force evaluations since sphgpu_sim_init.  A
+ * force evaluation runs 4 density passes and one derivative pass; they share
+ * one gather list (built with a 3 % margin on h) for as long as it covers every
+ * particle's support, so gather_builds / force_evals is 1 when h settles. */
+int32_t sphgpu_sim_list_builds(sphgpu_ctx *ctx, int64_t *gather_builds, int64_t *force_evals);
+
 /* Per-kernel statistics of the interaction kernels over the steps run since
  * statistics were switched on: summed kernel time (hipEvents on the
  * context's stream, one pair per launch), launches, and the interaction-list
  * sizes those launches processed: n_i = i-particles, n_j = j-list entries
  * (particles), n_s = monopole entries, pairs = sum over groups of
- * n_i * (n_j + n_s).  Index 0 density, 1 derivative, 2 hydro, 3 gravity. */
+ * n_i * (n_j + n_s).  Index 0 density, 1 derivative, 2 hydro, 3 gravity,
+ * 4 the density passes the precision form runs in FP64 (eos_density_fp64);
+ * n may be 4 or 5. */
 typedef struct {
     double ms;
     int64_t launches, n_i, n_j, n_s, pairs;

@@ -278,3 +278,16 @@ def test_full_step_granite_10k(ctx64, oracle):
         _print(f"   step {step + 1}: dt gpu {dt[0]!r} oracle {osim.dt[0]!r}")
         assert abs(dt[0] - osim.dt[0]) <= TOL * osim.dt[0]
         _compare(sim.state(), os_)
+
+
+def test_gather_list_reuse(ctx64):
+    """The density and derivative passes of a force evaluation share one
+    gather list while it covers every support; the full-step parity tests
+    above run through the same path."""
+    ic = IC.granite_impact(9000, 1000, seed=1000)
+    sim = Simulation(ctx64, ic)
+    sim.step(3)
+    builds, evals = sim.list_builds()
+    _print(f"[gather lists] {builds} builds for {evals} force evaluations")
+    assert evals == 4
+    assert evals <= builds <= 2 * evals
