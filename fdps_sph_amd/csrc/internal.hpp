@@ -257,6 +257,20 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
     return x * q + (x < r ? x : r) + k;
 }
 
+// Tiling of a group's j-list: T = ceil(nj / 64) tiles, tile t holding the
+// entries t, t + T, t + 2T, ... (lane l loads entry t + T l).  Interleaving
+// spreads every tile over the whole search region, so each lane finds about
+// the same number of in-support j per tile and the per-lane pair loops stay
+// balanced (consecutive entries are one compact Morton chunk, close to some
+// lanes and far from others).  Each lane still visits every entry once, so the
+// sums run over the same terms, associated in tile order.
+struct Tiles {
+    int nj, T;
+    __device__ __forceinline__ explicit Tiles(int n) : nj(n), T((n + kWave - 1) / kWave) {}
+    __device__ __forceinline__ int entry(int t, int lane) const { return t + T * lane; }
+    __device__ __forceinline__ int count(int t) const { return (nj - t + T - 1) / T; }
+};
+
 __device__ __forceinline__ int64_t list_begin(const ListRef &L, int g) {
     return L.off ? L.off[g] : (int64_t) g * L.cap;
 }

@@ -3,11 +3,11 @@
 // One 64-lane wavefront per i-group: lane l owns i-particle gstart[g]+l, and
 // the group's j-list is streamed through LDS in tiles of 64 (each lane
 // gathers one j, then every lane reads the tile back as an LDS broadcast).
-// The j-order is the list order, and every expression keeps the operand
-// order of src/force.h / src/kernel.h; this file is compiled with
-// -ffp-contract=off so no multiply-add is fused.  Pairs outside the support
-// radius (r^2 > (2.5h)^2 (1+1e-10)) are skipped: there W and gradW are
-// exactly zero, so the skip leaves every sum bit-identical.
+// Every pair expression keeps the operand order of src/force.h /
+// src/kernel.h; this file is compiled with -ffp-contract=off so no
+// multiply-add is fused.  Each lane sums its pairs sequentially in tile order
+// (see Tiles).  Pairs outside the support are skipped: there W and gradW are
+// exactly zero, so the skip changes no sum.
 #include "internal.hpp"
 
 namespace sphgpu {
@@ -37,16 +37,67 @@ __device__ __forceinline__ double gradW_factor_(double s, double r, double H, do
     return rv;
 }
 
-// Bit k set when j-tile entry k lies inside this lane's widened support.  The
-// kernels then evaluate only the set bits, in ascending order; entries left out
-// contribute exactly 0 to every sum, so the sums equal the full loop's.
-__device__ __forceinline__ uint64_t support_mask_(const double *sx, const double *sy, const double *sz, int nt,
-                                                  double xi, double yi, double zi, double rcut2) {
+__device__ __forceinline__ double wave_min_(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = dmin(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+__device__ __forceinline__ double wave_max_(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = dmax(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// The support test runs in FP32 on offsets o = (float)(x - c) from the centre c
+// of the group's box; the FP64 pair code then evaluates the pairs it admits.
+// The FP32 test admits a superset of the in-support pairs: each offset
+// component rounds by at most 2^-24 |o|, and the FP32 difference and r^2 by a
+// few ulp more, so r_fp32 <= r + 2^-24 (|o_i|_1 + |o_j|_1) + 1e-6 r.  The cut
+// radius is H (1 + 1e-4) + pad(o_i) + pad(o_j) with pad(o) = 4 * 2^-24 |o|_1.
+// Pairs admitted beyond the support give W = gradW = 0 exactly, so every sum
+// is the one over the in-support pairs.
+struct Frame {
+    double cx, cy, cz;
+};
+__device__ __forceinline__ Frame group_frame(double x, double y, double z, bool act) {
+    const double lx = wave_min_(act ? x : INFINITY), hx = wave_max_(act ? x : -INFINITY);
+    const double ly = wave_min_(act ? y : INFINITY), hy = wave_max_(act ? y : -INFINITY);
+    const double lz = wave_min_(act ? z : INFINITY), hz = wave_max_(act ? z : -INFINITY);
+    return {0.5 * (lx + hx), 0.5 * (ly + hy), 0.5 * (lz + hz)};
+}
+__device__ __forceinline__ float pad_(float ox, float oy, float oz) {
+    return 2.384185791015625e-7f * (fabsf(ox) + fabsf(oy) + fabsf(oz));  // 4 * 2^-24 |o|_1
+}
+__device__ __forceinline__ float wave_maxf_(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// Bit k set when j-tile entry k may lie inside this lane's support (cut2 < 0:
+// none).  o: FP32 offsets of the tile entries from the frame centre; cut2 is
+// the lane's padded cut radius (with the tile's largest j pad) squared.
+__device__ __forceinline__ uint64_t support_mask_(const float4 *o, int nt, float xi, float yi, float zi,
+                                                  float cut2) {
     uint64_t mask = 0;
     for (int jj = 0; jj < nt; ++jj) {
-        const double dx = sx[jj] - xi, dy = sy[jj] - yi, dz = sz[jj] - zi;
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        mask |= (r2 <= rcut2) ? (1ULL << jj) : 0ULL;
+        const float4 q = o[jj];
+        const float dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        mask |= (r2 <= cut2) ? (1ULL << jj) : 0ULL;
+    }
+    return mask;
+}
+// symmetric search: inside the lane's support or inside j's (o.w: j's padded
+// cut radius squared, with the group's largest i pad)
+__device__ __forceinline__ uint64_t support_mask_sym_(const float4 *o, int nt, float xi, float yi, float zi,
+                                                      float cut2, bool act) {
+    uint64_t mask = 0;
+    for (int jj = 0; jj < nt; ++jj) {
+        const float4 q = o[jj];
+        const float dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        mask |= (r2 <= cut2 || (act && r2 <= q.w)) ? (1ULL << jj) : 0ULL;
     }
     return mask;
 }
@@ -62,31 +113,35 @@ __global__ __launch_bounds__(kWave) void k_dens(DensArgs a) {
     const double xi = a.xi[ip], yi = a.yi[ip], zi = a.zi[ip], hi = a.hi[ip], mi = a.mi[ip];
     const double H = kSupport * hi;
     const double norm = wnorm_(H);
-    const double rcut2 = act ? (H * H) * kCutWiden : -1.0;
+    const Frame fr = group_frame(xi, yi, zi, act);
+    const float oxi = (float) (xi - fr.cx), oyi = (float) (yi - fr.cy), ozi = (float) (zi - fr.cz);
+    const float cuti = (float) H * 1.0001f + pad_(oxi, oyi, ozi);
     double dens = 0.0;
     int ngb = 0;
     const int64_t lb = list_begin(a.L, g);
-    const int nj = a.L.cnt[g];
-    __shared__ double sm[kWave], sx[kWave], sy[kWave], sz[kWave];
-    for (int t = 0; t < nj; t += kWave) {
-        const int k = t + lane;
-        if (k < nj) {
+    const Tiles tl(a.L.cnt[g]);
+    __shared__ double4 sp[kWave];  // x, y, z, m
+    __shared__ float4 so[kWave];   // FP32 offsets from the frame centre
+    for (int t = 0; t < tl.T; ++t) {
+        const int k = tl.entry(t, lane);
+        float padj = 0.f;
+        if (k < tl.nj) {
             const int q = a.L.idx[lb + k];
-            sm[lane] = a.mj[q];
-            sx[lane] = a.xj[q];
-            sy[lane] = a.yj[q];
-            sz[lane] = a.zj[q];
+            const double x = a.xj[q], y = a.yj[q], z = a.zj[q];
+            sp[lane] = make_double4(x, y, z, a.mj[q]);
+            const float4 o = make_float4((float) (x - fr.cx), (float) (y - fr.cy), (float) (z - fr.cz), 0.f);
+            so[lane] = o;
+            padj = pad_(o.x, o.y, o.z);
         }
+        const float cut = cuti + wave_maxf_(padj);
         __syncthreads();
-        const int nt = min(kWave, nj - t);
-        // pass 1: this lane's in-support j of the tile; pass 2: only those, in order
-        const uint64_t mask = support_mask_(sx, sy, sz, nt, xi, yi, zi, rcut2);
+        const uint64_t mask = support_mask_(so, tl.count(t), oxi, oyi, ozi, act ? cut * cut : -1.f);
         for (uint64_t m = mask; m; m &= m - 1) {
-            const int jj = __builtin_ctzll(m);
-            const double dx = sx[jj] - xi, dy = sy[jj] - yi, dz = sz[jj] - zi;
+            const double4 pj = sp[__builtin_ctzll(m)];
+            const double dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
             const double r2 = dx * dx + dy * dy + dz * dz;
             const double s = sqrt(r2) / H;
-            dens += sm[jj] * W_(s, norm);
+            dens += pj.w * W_(s, norm);
             ngb += (plus_(1.0 - s) > 0) ? 1 : 0;
         }
         __syncthreads();
@@ -113,39 +168,45 @@ __global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
     const double hi = a.hi[ip], rhoi = a.rhoi[ip];
     const double H = kSupport * hi;
     const double norm = wnorm_(H);
-    const double rcut2 = act ? (H * H) * kCutWiden : -1.0;
+    const Frame fr = group_frame(xi, yi, zi, act);
+    const float oxi = (float) (xi - fr.cx), oyi = (float) (yi - fr.cy), ozi = (float) (zi - fr.cz);
+    const float cuti = (float) H * 1.0001f + pad_(oxi, oyi, ozi);
     double divv = 0.0, rx = 0.0, ry = 0.0, rz = 0.0, gs = 0.0;
     const int64_t lb = list_begin(a.L, g);
-    const int nj = a.L.cnt[g];
-    __shared__ double sm[kWave], sx[kWave], sy[kWave], sz[kWave], svx[kWave], svy[kWave], svz[kWave];
-    for (int t = 0; t < nj; t += kWave) {
-        const int k = t + lane;
-        if (k < nj) {
+    const Tiles tl(a.L.cnt[g]);
+    __shared__ double4 sp[kWave];  // x, y, z, m
+    __shared__ double4 sv[kWave];  // vx, vy, vz, -
+    __shared__ float4 so[kWave];
+    for (int t = 0; t < tl.T; ++t) {
+        const int k = tl.entry(t, lane);
+        float padj = 0.f;
+        if (k < tl.nj) {
             const int q = a.L.idx[lb + k];
-            sm[lane] = a.mj[q];
-            sx[lane] = a.xj[q];
-            sy[lane] = a.yj[q];
-            sz[lane] = a.zj[q];
-            svx[lane] = a.vxj[q];
-            svy[lane] = a.vyj[q];
-            svz[lane] = a.vzj[q];
+            const double x = a.xj[q], y = a.yj[q], z = a.zj[q];
+            sp[lane] = make_double4(x, y, z, a.mj[q]);
+            sv[lane] = make_double4(a.vxj[q], a.vyj[q], a.vzj[q], 0.0);
+            const float4 o = make_float4((float) (x - fr.cx), (float) (y - fr.cy), (float) (z - fr.cz), 0.f);
+            so[lane] = o;
+            padj = pad_(o.x, o.y, o.z);
         }
+        const float cut = cuti + wave_maxf_(padj);
         __syncthreads();
-        const int nt = min(kWave, nj - t);
-        const uint64_t mask = support_mask_(sx, sy, sz, nt, xi, yi, zi, rcut2);
+        const uint64_t mask = support_mask_(so, tl.count(t), oxi, oyi, ozi, act ? cut * cut : -1.f);
         for (uint64_t msk = mask; msk; msk &= msk - 1) {
             const int jj = __builtin_ctzll(msk);
-            const double drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
+            const double4 pj = sp[jj];
+            const double4 vj = sv[jj];
+            const double drx = xi - pj.x, dry = yi - pj.y, drz = zi - pj.z;
             const double r2 = drx * drx + dry * dry + drz * drz;
             {
-                const double dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
+                const double dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
                 const double r = sqrt(r2);
                 const double s = r / H;
                 double inv;
                 const double f = gradW_factor_(s, r, H, hi, norm, inv);
                 const double gwx = (drx * f) * inv, gwy = (dry * f) * inv, gwz = (drz * f) * inv;
                 const double w = W_(s, norm);
-                const double mj = sm[jj];
+                const double mj = pj.w;
                 const double mneg = -mj;
                 const double ax = mneg * dvx, ay = mneg * dvy, az = mneg * dvz;
                 divv += ax * gwx + ay * gwy + az * gwz;
@@ -177,6 +238,9 @@ __global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
 // ---------------------------------------------------------------- hydro force
 // CalcHydroForce::operator() (src/force.h:112-147), `#if 1` branch.  v_sig_max
 // runs over every j of the list; the force terms only inside either support.
+struct HydroJ {   // per-j record of a tile (LDS)
+    double x, y, z, vx, vy, vz, m, rho, cs, bal, gs, A, h, H, norm;
+};
 __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
     const int g = xcd_remap(blockIdx.x, a.ngroups);
     const int lane = threadIdx.x;
@@ -189,87 +253,93 @@ __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
                  bali = a.bali[ip];
     const double Hi = kSupport * hi;
     const double normi = wnorm_(Hi);
-    const double rcut2i = act ? (Hi * Hi) * kCutWiden : -1.0;
     const double Ai = gsi * pi / (rhoi * rhoi);
-    double ax = 0.0, ay = 0.0, az = 0.0, du = 0.0, vmax = 0.0;
     const int64_t lb = list_begin(a.L, g);
-    const int nj = a.L.cnt[g];
-    __shared__ double sx[kWave], sy[kWave], sz[kWave], svx[kWave], svy[kWave], svz[kWave], sm[kWave];
-    __shared__ double srho[kWave], scs[kWave], sbal[kWave], sgs[kWave], sA[kWave], sh[kWave], sH[kWave],
-        snorm[kWave], scut[kWave];
-    for (int t = 0; t < nj; t += kWave) {
-        const int k = t + lane;
-        if (k < nj) {
+    const Tiles tl(a.L.cnt[g]);
+    const Frame fr = group_frame(xi, yi, zi, act);
+    const float oxi = (float) (xi - fr.cx), oyi = (float) (yi - fr.cy), ozi = (float) (zi - fr.cz);
+    const float padi = pad_(oxi, oyi, ozi);
+    const float cuti = (float) Hi * 1.0001f + padi;
+    const float padi_max = wave_maxf_(act ? padi : 0.f);
+    double ax = 0.0, ay = 0.0, az = 0.0, du = 0.0, vmax = 0.0;
+    __shared__ HydroJ sj[kWave];
+    __shared__ float4 so[kWave];  // FP32 offsets, cut2 of j
+    for (int t = 0; t < tl.T; ++t) {
+        const int k = tl.entry(t, lane);
+        float padj = 0.f;
+        if (k < tl.nj) {
             const int q = a.L.idx[lb + k];
-            sx[lane] = a.xj[q];
-            sy[lane] = a.yj[q];
-            sz[lane] = a.zj[q];
-            svx[lane] = a.vxj[q];
-            svy[lane] = a.vyj[q];
-            svz[lane] = a.vzj[q];
-            sm[lane] = a.mj[q];
-            const double rj = a.rhoj[q], gj = a.gsj[q], hj = a.hj[q];
-            srho[lane] = rj;
-            scs[lane] = a.csj[q];
-            sbal[lane] = a.balj[q];
-            sgs[lane] = gj;
-            sA[lane] = gj * a.pj[q] / (rj * rj);
-            sh[lane] = hj;
-            const double Hj = kSupport * hj;
-            sH[lane] = Hj;
-            snorm[lane] = wnorm_(Hj);
-            scut[lane] = (Hj * Hj) * kCutWiden;
+            HydroJ r;
+            r.x = a.xj[q];
+            r.y = a.yj[q];
+            r.z = a.zj[q];
+            r.vx = a.vxj[q];
+            r.vy = a.vyj[q];
+            r.vz = a.vzj[q];
+            r.m = a.mj[q];
+            r.rho = a.rhoj[q];
+            r.cs = a.csj[q];
+            r.bal = a.balj[q];
+            r.gs = a.gsj[q];
+            r.A = r.gs * a.pj[q] / (r.rho * r.rho);
+            r.h = a.hj[q];
+            r.H = kSupport * r.h;
+            r.norm = wnorm_(r.H);
+            sj[lane] = r;
+            const float ox = (float) (r.x - fr.cx), oy = (float) (r.y - fr.cy), oz = (float) (r.z - fr.cz);
+            padj = pad_(ox, oy, oz);
+            const float cutj = (float) r.H * 1.0001f + padj + padi_max;
+            so[lane] = make_float4(ox, oy, oz, cutj * cutj);
         }
+        const float cut = cuti + wave_maxf_(padj);
         __syncthreads();
-        const int nt = min(kWave, nj - t);
-        // pass 1, every j: v_sig_max, and the mask of pairs inside either support.
-        // v_sig = (c_i + c_j) - 3w with |w| = |dv.dr|/|dr| <= |dv|; when even that
-        // bound stays below the current maximum (with a margin far above the
-        // rounding of either side) the square root and division are skipped --
-        // the maximum cannot change, so it stays bit-identical.
-        uint64_t mask = 0;
+        const int nt = tl.count(t);
+        // pass 1, every j: v_sig_max.  v_sig = (c_i + c_j) - 3w with |w| =
+        // |dv.dr|/|dr| <= |dv|; when even that bound stays below the current
+        // maximum (with a margin far above the rounding of either side) the
+        // square root and division are skipped -- the maximum cannot change.
         for (int jj = 0; jj < nt; ++jj) {
-            const double drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
-            const double dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
+            const double drx = xi - sj[jj].x, dry = yi - sj[jj].y, drz = zi - sj[jj].z;
+            const double dvx = vxi - sj[jj].vx, dvy = vyi - sj[jj].vy, dvz = vzi - sj[jj].vz;
             const double dvdr = dvx * drx + dvy * dry + dvz * drz;
-            const double r2 = drx * drx + dry * dry + drz * drz;
-            const double cij = csi + scs[jj];
+            const double cij = csi + sj[jj].cs;
             if (dvdr < 0) {
                 const double room = vmax - cij;
                 const double dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
                 if (!(room > 1.0e-4 * vmax && 9.0 * dv2 < (room * room) * (1.0 - 1.0e-10))) {
+                    const double r2 = drx * drx + dry * dry + drz * drz;
                     const double w = dvdr / sqrt(r2);
                     vmax = dmax(vmax, cij - 3.0 * w);
                 }
             } else {
                 vmax = dmax(vmax, cij - 3.0 * 0.0);
             }
-            mask |= (r2 <= rcut2i || (act && r2 <= scut[jj])) ? (1ULL << jj) : 0ULL;
         }
-        // pass 2: the force terms of this lane's in-support pairs, in list order
+        // pass 2: the force terms of the pairs inside either support
+        const uint64_t mask = support_mask_sym_(so, nt, oxi, oyi, ozi, act ? cut * cut : -1.f, act);
         for (uint64_t msk = mask; msk; msk &= msk - 1) {
-            const int jj = __builtin_ctzll(msk);
-            const double drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
-            const double dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
+            const HydroJ &J = sj[__builtin_ctzll(msk)];
+            const double drx = xi - J.x, dry = yi - J.y, drz = zi - J.z;
+            const double dvx = vxi - J.vx, dvy = vyi - J.vy, dvz = vzi - J.vz;
             const double dvdr = dvx * drx + dvy * dry + dvz * drz;
             const double r2 = drx * drx + dry * dry + drz * drz;
             const double w = (dvdr < 0) ? dvdr / sqrt(r2) : 0;
-            const double v_sig = csi + scs[jj] - 3.0 * w;
+            const double v_sig = csi + J.cs - 3.0 * w;
             {
-                const double rhoj = srho[jj];
-                const double AV = -kAV * 0.5 * v_sig * w / (0.5 * (rhoi + rhoj)) * 0.5 * (bali + sbal[jj]);
+                const double rhoj = J.rho;
+                const double AV = -kAV * 0.5 * v_sig * w / (0.5 * (rhoi + rhoj)) * 0.5 * (bali + J.bal);
                 const double r = sqrt(r2);
                 double invi, invj;
                 const double fi = gradW_factor_(r / Hi, r, Hi, hi, normi, invi);
-                const double Hj = sH[jj];
-                const double fj = gradW_factor_(r / Hj, r, Hj, sh[jj], snorm[jj], invj);
+                const double Hj = J.H;
+                const double fj = gradW_factor_(r / Hj, r, Hj, J.h, J.norm, invj);
                 const double gix = (drx * fi) * invi, giy = (dry * fi) * invi, giz = (drz * fi) * invi;
                 const double gjx = (drx * fj) * invj, gjy = (dry * fj) * invj, gjz = (drz * fj) * invj;
-                const double gsj = sgs[jj];
+                const double gsj = J.gs;
                 const double gx = 0.5 * (gix * gsi + gjx * gsj);
                 const double gy = 0.5 * (giy * gsi + gjy * gsj);
                 const double gz = 0.5 * (giz * gsi + gjz * gsj);
-                const double mj = sm[jj], Aj = sA[jj];
+                const double mj = J.m, Aj = J.A;
                 ax -= mj * (Ai * gix + Aj * gjx + AV * gx);
                 ay -= mj * (Ai * giy + Aj * gjy + AV * gy);
                 az -= mj * (Ai * giz + Aj * gjz + AV * gz);

@@ -52,7 +52,6 @@ __device__ __forceinline__ double wave_centre(double v, bool act) {
     return 0.5 * (lo + hi);
 }
 
-constexpr float kNorm = (float) (1365.0 / 64.0 / kPi);
 constexpr double kNormD = 1365.0 / 64.0 / kPi;
 
 // Per-particle kernel constants, formed in FP64 and rounded once: every pair
@@ -97,8 +96,9 @@ __global__ __launch_bounds__(kWave) void k_dens_mixed(DensArgs a) {
     const int64_t lb = list_begin(a.L, g);
     const int nj = a.L.cnt[g];
     __shared__ float sm[kWave], sx[kWave], sy[kWave], sz[kWave];
-    for (int t = 0; t < nj; t += kWave) {
-        const int k = t + lane;
+    const Tiles tl(nj);
+    for (int t = 0; t < tl.T; ++t) {
+        const int k = tl.entry(t, lane);
         if (k < nj) {
             const int q = a.L.idx[lb + k];
             sm[lane] = (float) a.mj[q];
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(kWave) void k_dens_mixed(DensArgs a) {
             sz[lane] = (float) (a.zj[q] - cz);
         }
         __syncthreads();
-        const int nt = min(kWave, nj - t);
+        const int nt = tl.count(t);
         float part = 0.f;
         const uint64_t mask = support_mask(sx, sy, sz, nt, xi, yi, zi, rcut2);
         for (uint64_t msk = mask; msk; msk &= msk - 1) {
@@ -157,8 +157,9 @@ __global__ __launch_bounds__(kWave) void k_drvt_mixed(DrvtArgs a) {
     const int64_t lb = list_begin(a.L, g);
     const int nj = a.L.cnt[g];
     __shared__ float sm[kWave], sx[kWave], sy[kWave], sz[kWave], svx[kWave], svy[kWave], svz[kWave];
-    for (int t = 0; t < nj; t += kWave) {
-        const int k = t + lane;
+    const Tiles tl(nj);
+    for (int t = 0; t < tl.T; ++t) {
+        const int k = tl.entry(t, lane);
         if (k < nj) {
             const int q = a.L.idx[lb + k];
             sm[lane] = (float) a.mj[q];
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(kWave) void k_drvt_mixed(DrvtArgs a) {
             svz[lane] = (float) (a.vzj[q] - cvz);
         }
         __syncthreads();
-        const int nt = min(kWave, nj - t);
+        const int nt = tl.count(t);
         float pd = 0.f, px = 0.f, py = 0.f, pz = 0.f, pg = 0.f;
         const uint64_t mask = support_mask(sx, sy, sz, nt, xi, yi, zi, rcut2);
         for (uint64_t msk = mask; msk; msk &= msk - 1) {
@@ -245,8 +246,9 @@ __global__ __launch_bounds__(kWave) void k_hydro_mixed(HydroArgs a) {
     __shared__ float sx[kWave], sy[kWave], sz[kWave], svx[kWave], svy[kWave], svz[kWave], sm[kWave];
     __shared__ float srho[kWave], scs[kWave], sbal[kWave], sgs[kWave], sA[kWave], sinvH[kWave], sH[kWave],
         sreg[kWave], snorm[kWave], scut[kWave];
-    for (int t = 0; t < nj; t += kWave) {
-        const int k = t + lane;
+    const Tiles tl(nj);
+    for (int t = 0; t < tl.T; ++t) {
+        const int k = tl.entry(t, lane);
         if (k < nj) {
             const int q = a.L.idx[lb + k];
             sx[lane] = (float) (a.xj[q] - cx);
@@ -270,7 +272,7 @@ __global__ __launch_bounds__(kWave) void k_hydro_mixed(HydroArgs a) {
             scut[lane] = (float) (Hj * Hj) * 1.0001f;
         }
         __syncthreads();
-        const int nt = min(kWave, nj - t);
+        const int nt = tl.count(t);
         float px = 0.f, py = 0.f, pz = 0.f, pu = 0.f;
         // pass 1, every j: v_sig_max and the mask of pairs inside either support
         uint64_t mask = 0;

@@ -40,7 +40,7 @@ def test_batch_matches_oracle(ctx64, oracle, batch_ic):
     tabs = [oracle.Table(E.PATHS[k]) for k in ("granite", "iron", "dunite")]
     osim = oracle.Sim(os_, tabs, oracle.params(nsys=3))
     osim.init()
-    _compare(sim.state(), os_)
+    _compare(sim.state(), os_, tabs)
     sim.step(1)
     osim.step()
     dt, t = sim.time()
@@ -48,4 +48,4 @@ def test_batch_matches_oracle(ctx64, oracle, batch_ic):
     assert len(set(dt.tolist())) == 3  # independent time steps
     assert H.rel_err(dt, osim.dt) <= TOL
     assert H.rel_err(t, osim.time) <= TOL
-    _compare(sim.state(), os_)
+    _compare(sim.state(), os_, tabs)
