@@ -271,6 +271,32 @@ struct Tiles {
     __device__ __forceinline__ int count(int t) const { return (nj - t + T - 1) / T; }
 };
 
+// Support mask of one j-tile in dot form.  q[jj] = (o.x, o.y, o.z, w) holds
+// FP32 offsets o_j of the tile's entries from the group centre and
+// w = |o_j|^2; with t = (-2 o_i) . o_j + w = |o_j - o_i|^2 - |o_i|^2, bit jj
+// of the result is the sign of t - c (set: t < c, taken as inside the cut;
+// c = cut^2 - |o_i|^2, c = -inf admits nothing).  Each test is three FMAs, a
+// subtraction and one funnel shift that moves the sign bit into the mask.
+__device__ __forceinline__ uint32_t dot_bit_(uint32_t m, const float4 &q, float tx, float ty, float tz, float c) {
+    float t = fmaf(tx, q.x, q.w);
+    t = fmaf(ty, q.y, t);
+    t = fmaf(tz, q.z, t);
+    return __builtin_amdgcn_alignbit(m, __float_as_uint(t - c), 31);
+}
+__device__ __forceinline__ uint64_t dot_mask(const float4 *q, int nt, float tx, float ty, float tz, float c) {
+    uint32_t lo = 0, hi = 0;
+    if (nt == kWave) {  // full tile: unrolled, constant LDS offsets
+#pragma unroll
+        for (int jj = kWave - 1; jj >= 32; --jj) hi = dot_bit_(hi, q[jj], tx, ty, tz, c);
+#pragma unroll
+        for (int jj = 31; jj >= 0; --jj) lo = dot_bit_(lo, q[jj], tx, ty, tz, c);
+    } else {
+        for (int jj = nt - 1; jj >= 32; --jj) hi = dot_bit_(hi, q[jj], tx, ty, tz, c);
+        for (int jj = (nt < 32 ? nt : 32) - 1; jj >= 0; --jj) lo = dot_bit_(lo, q[jj], tx, ty, tz, c);
+    }
+    return ((uint64_t) hi << 32) | lo;
+}
+
 __device__ __forceinline__ int64_t list_begin(const ListRef &L, int g) {
     return L.off ? L.off[g] : (int64_t) g * L.cap;
 }

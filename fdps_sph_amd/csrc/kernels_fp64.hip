@@ -65,6 +65,15 @@ __device__ __forceinline__ Frame group_frame(double x, double y, double z, bool 
     const double lz = wave_min_(act ? z : INFINITY), hz = wave_max_(act ? z : -INFINITY);
     return {0.5 * (lx + hx), 0.5 * (ly + hy), 0.5 * (lz + hz)};
 }
+// Dot-form mask threshold of one lane for one tile (see dot_mask): the
+// in-support test |x_j - x_i|^2 <= H^2 evaluated through FP32 offsets needs
+// room for the offsets' rounding (radius + 2^-22 (|o_i|_1 + N)) and for the
+// FMA chain's (2^-19 (|o_i|_1^2 + N^2)), N = largest |o_j|_1 of the tile.
+__device__ __forceinline__ float dot_cut_(float Hc, float wi, float ni, float nmax) {
+    const float r = Hc + 2.384185791015625e-7f * (ni + nmax);
+    return fmaf(r, r, -wi) + 1.9073486328125e-6f * (ni * ni + nmax * nmax);
+}
+__device__ __forceinline__ float norm1_(float ox, float oy, float oz) { return fabsf(ox) + fabsf(oy) + fabsf(oz); }
 __device__ __forceinline__ float pad_(float ox, float oy, float oz) {
     return 2.384185791015625e-7f * (fabsf(ox) + fabsf(oy) + fabsf(oz));  // 4 * 2^-24 |o|_1
 }
@@ -74,20 +83,6 @@ __device__ __forceinline__ float wave_maxf_(float v) {
     return v;
 }
 
-// Bit k set when j-tile entry k may lie inside this lane's support (cut2 < 0:
-// none).  o: FP32 offsets of the tile entries from the frame centre; cut2 is
-// the lane's padded cut radius (with the tile's largest j pad) squared.
-__device__ __forceinline__ uint64_t support_mask_(const float4 *o, int nt, float xi, float yi, float zi,
-                                                  float cut2) {
-    uint64_t mask = 0;
-    for (int jj = 0; jj < nt; ++jj) {
-        const float4 q = o[jj];
-        const float dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        mask |= (r2 <= cut2) ? (1ULL << jj) : 0ULL;
-    }
-    return mask;
-}
 // symmetric search: inside the lane's support or inside j's (o.w: j's padded
 // cut radius squared, with the group's largest i pad)
 __device__ __forceinline__ uint64_t support_mask_sym_(const float4 *o, int nt, float xi, float yi, float zi,
@@ -115,27 +110,29 @@ __global__ __launch_bounds__(kWave) void k_dens(DensArgs a) {
     const double norm = wnorm_(H);
     const Frame fr = group_frame(xi, yi, zi, act);
     const float oxi = (float) (xi - fr.cx), oyi = (float) (yi - fr.cy), ozi = (float) (zi - fr.cz);
-    const float cuti = (float) H * 1.0001f + pad_(oxi, oyi, ozi);
+    const float Hc = (float) H * 1.0001f;
+    const float wi = fmaf(oxi, oxi, fmaf(oyi, oyi, ozi * ozi)), n1i = norm1_(oxi, oyi, ozi);
+    const float tx = -2.f * oxi, ty = -2.f * oyi, tz = -2.f * ozi;
     double dens = 0.0;
     int ngb = 0;
     const int64_t lb = list_begin(a.L, g);
     const Tiles tl(a.L.cnt[g]);
     __shared__ double4 sp[kWave];  // x, y, z, m
-    __shared__ float4 so[kWave];   // FP32 offsets from the frame centre
+    __shared__ float4 so[kWave];   // FP32 offsets from the frame centre, |offset|^2
     for (int t = 0; t < tl.T; ++t) {
         const int k = tl.entry(t, lane);
-        float padj = 0.f;
+        float nj1 = 0.f;
         if (k < tl.nj) {
             const int q = a.L.idx[lb + k];
             const double x = a.xj[q], y = a.yj[q], z = a.zj[q];
             sp[lane] = make_double4(x, y, z, a.mj[q]);
-            const float4 o = make_float4((float) (x - fr.cx), (float) (y - fr.cy), (float) (z - fr.cz), 0.f);
-            so[lane] = o;
-            padj = pad_(o.x, o.y, o.z);
+            const float ox = (float) (x - fr.cx), oy = (float) (y - fr.cy), oz = (float) (z - fr.cz);
+            so[lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
+            nj1 = norm1_(ox, oy, oz);
         }
-        const float cut = cuti + wave_maxf_(padj);
+        const float c = act ? dot_cut_(Hc, wi, n1i, wave_maxf_(nj1)) : -INFINITY;
         __syncthreads();
-        const uint64_t mask = support_mask_(so, tl.count(t), oxi, oyi, ozi, act ? cut * cut : -1.f);
+        const uint64_t mask = dot_mask(so, tl.count(t), tx, ty, tz, c);
         for (uint64_t m = mask; m; m &= m - 1) {
             const double4 pj = sp[__builtin_ctzll(m)];
             const double dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
@@ -170,7 +167,9 @@ __global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
     const double norm = wnorm_(H);
     const Frame fr = group_frame(xi, yi, zi, act);
     const float oxi = (float) (xi - fr.cx), oyi = (float) (yi - fr.cy), ozi = (float) (zi - fr.cz);
-    const float cuti = (float) H * 1.0001f + pad_(oxi, oyi, ozi);
+    const float Hc = (float) H * 1.0001f;
+    const float wi = fmaf(oxi, oxi, fmaf(oyi, oyi, ozi * ozi)), n1i = norm1_(oxi, oyi, ozi);
+    const float tx = -2.f * oxi, ty = -2.f * oyi, tz = -2.f * ozi;
     double divv = 0.0, rx = 0.0, ry = 0.0, rz = 0.0, gs = 0.0;
     const int64_t lb = list_begin(a.L, g);
     const Tiles tl(a.L.cnt[g]);
@@ -179,19 +178,19 @@ __global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
     __shared__ float4 so[kWave];
     for (int t = 0; t < tl.T; ++t) {
         const int k = tl.entry(t, lane);
-        float padj = 0.f;
+        float nj1 = 0.f;
         if (k < tl.nj) {
             const int q = a.L.idx[lb + k];
             const double x = a.xj[q], y = a.yj[q], z = a.zj[q];
             sp[lane] = make_double4(x, y, z, a.mj[q]);
             sv[lane] = make_double4(a.vxj[q], a.vyj[q], a.vzj[q], 0.0);
-            const float4 o = make_float4((float) (x - fr.cx), (float) (y - fr.cy), (float) (z - fr.cz), 0.f);
-            so[lane] = o;
-            padj = pad_(o.x, o.y, o.z);
+            const float ox = (float) (x - fr.cx), oy = (float) (y - fr.cy), oz = (float) (z - fr.cz);
+            so[lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
+            nj1 = norm1_(ox, oy, oz);
         }
-        const float cut = cuti + wave_maxf_(padj);
+        const float c = act ? dot_cut_(Hc, wi, n1i, wave_maxf_(nj1)) : -INFINITY;
         __syncthreads();
-        const uint64_t mask = support_mask_(so, tl.count(t), oxi, oyi, ozi, act ? cut * cut : -1.f);
+        const uint64_t mask = dot_mask(so, tl.count(t), tx, ty, tz, c);
         for (uint64_t msk = mask; msk; msk &= msk - 1) {
             const int jj = __builtin_ctzll(msk);
             const double4 pj = sp[jj];
@@ -360,20 +359,26 @@ __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
 
 // ---------------------------------------------------------------- gravity
 // CalcGravityForce<EPJ::Grav> then CalcGravityForce<SPJMonopole>
-// (src/force.h:154-168): the particle list first, then the monopoles.
-__device__ __forceinline__ void grav_tile_(const double *sx, const double *sy, const double *sz, const double *sm,
-                                           int nt, double xi, double yi, double zi, double eps2, double &ax,
-                                           double &ay, double &az, double &pot) {
+// (src/force.h:154-168): the particle list first, then the monopoles.  The
+// reference's 1.0 / sqrt(r^2 + eps^2) is the FP64 rsqrt of the math library
+// (hardware estimate refined in FP64, within an ulp or so of the quotient of
+// the correctly rounded square root: a third of the instructions of the
+// IEEE divide and square root), and the accumulations are fused
+// multiply-adds.  Both change results only at the rounding level.
+__device__ __forceinline__ void grav_tile_(const double4 *sp, int nt, double xi, double yi, double zi, double eps2,
+                                           double &ax, double &ay, double &az, double &pot) {
+#pragma unroll 2
     for (int jj = 0; jj < nt; ++jj) {
-        const double drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
+        const double4 pj = sp[jj];  // x, y, z, m
+        const double drx = xi - pj.x, dry = yi - pj.y, drz = zi - pj.z;
         const double dr2 = drx * drx + dry * dry + drz * drz;
-        const double dr_inv = 1.0 / sqrt(dr2 + eps2);
-        const double m = sm[jj];
-        const double m_dr3_inv = m * (dr_inv * dr_inv * dr_inv);
-        ax -= kG * m_dr3_inv * drx;
-        ay -= kG * m_dr3_inv * dry;
-        az -= kG * m_dr3_inv * drz;
-        pot -= kG * m * dr_inv;
+        const double dr_inv = rsqrt(dr2 + eps2);
+        const double m_dr3_inv = pj.w * (dr_inv * dr_inv * dr_inv);
+        const double g3 = kG * m_dr3_inv;
+        ax = fma(-g3, drx, ax);
+        ay = fma(-g3, dry, ay);
+        az = fma(-g3, drz, az);
+        pot = fma(-(kG * pj.w), dr_inv, pot);
     }
 }
 
@@ -388,38 +393,22 @@ __global__ __launch_bounds__(kWave) void k_grav(GravArgs a) {
     if (a.eps2i) eps2 = a.eps2i[ip];
     else { const double h = a.hi[ip]; eps2 = h * h; }
     double ax = 0.0, ay = 0.0, az = 0.0, pot = 0.0;
-    __shared__ double sx[kWave], sy[kWave], sz[kWave], sm[kWave];
-    {
-        const int64_t lb = list_begin(a.Lj, g);
-        const int nj = a.Lj.cnt[g];
+    __shared__ double4 sp[kWave];
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass && !a.ms) break;
+        const ListRef &L = pass ? a.Ls : a.Lj;
+        const double *X = pass ? a.xs : a.xj, *Y = pass ? a.ys : a.yj, *Z = pass ? a.zs : a.zj,
+                     *Mm = pass ? a.ms : a.mj;
+        const int64_t lb = list_begin(L, g);
+        const int nj = L.cnt[g];
         for (int t = 0; t < nj; t += kWave) {
             const int k = t + lane;
             if (k < nj) {
-                const int q = a.Lj.idx[lb + k];
-                sx[lane] = a.xj[q];
-                sy[lane] = a.yj[q];
-                sz[lane] = a.zj[q];
-                sm[lane] = a.mj[q];
+                const int q = L.idx[lb + k];
+                sp[lane] = make_double4(X[q], Y[q], Z[q], Mm[q]);
             }
             __syncthreads();
-            grav_tile_(sx, sy, sz, sm, min(kWave, nj - t), xi, yi, zi, eps2, ax, ay, az, pot);
-            __syncthreads();
-        }
-    }
-    if (a.ms) {
-        const int64_t lb = list_begin(a.Ls, g);
-        const int ns = a.Ls.cnt[g];
-        for (int t = 0; t < ns; t += kWave) {
-            const int k = t + lane;
-            if (k < ns) {
-                const int q = a.Ls.idx[lb + k];
-                sx[lane] = a.xs[q];
-                sy[lane] = a.ys[q];
-                sz[lane] = a.zs[q];
-                sm[lane] = a.ms[q];
-            }
-            __syncthreads();
-            grav_tile_(sx, sy, sz, sm, min(kWave, ns - t), xi, yi, zi, eps2, ax, ay, az, pot);
+            grav_tile_(sp, min(kWave, nj - t), xi, yi, zi, eps2, ax, ay, az, pot);
             __syncthreads();
         }
     }

@@ -65,18 +65,6 @@ __device__ __forceinline__ KernelConsts kernel_consts(double h) {
     return {(float) Hd, (float) (1.0 / Hd), (float) (kNormD / (Hd * Hd * Hd))};
 }
 
-// bit k set when j-tile entry k lies inside this lane's (widened) support
-__device__ __forceinline__ uint64_t support_mask(const float *sx, const float *sy, const float *sz, int nt, float xi,
-                                                 float yi, float zi, float rcut2) {
-    uint64_t mask = 0;
-    for (int jj = 0; jj < nt; ++jj) {
-        const float dx = sx[jj] - xi, dy = sy[jj] - yi, dz = sz[jj] - zi;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        mask |= (r2 < rcut2) ? (1ULL << jj) : 0ULL;
-    }
-    return mask;
-}
-
 // ---------------------------------------------------------------- density
 __global__ __launch_bounds__(kWave) void k_dens_mixed(DensArgs a) {
     const int g = xcd_remap(blockIdx.x, a.ngroups);
@@ -90,29 +78,48 @@ __global__ __launch_bounds__(kWave) void k_dens_mixed(DensArgs a) {
     const double hid = a.hi[ip];
     const KernelConsts kc = kernel_consts(hid);
     const float H = kc.H, invH = kc.invH, norm = kc.norm;
-    const float rcut2 = act ? H * H * 1.0001f : -1.f;
+    // dot-form mask constants (see dot_mask): inside when |o_j - o_i|^2 < H^2 (1 + 1e-4)
+    const float cdot = act ? H * H * 1.0001f - fmaf(xi, xi, fmaf(yi, yi, zi * zi)) : -INFINITY;
+    const float tx = -2.f * xi, ty = -2.f * yi, tz = -2.f * zi;
     Kahan dens;
     int ngb = 0;
     const int64_t lb = list_begin(a.L, g);
     const int nj = a.L.cnt[g];
-    __shared__ float sm[kWave], sx[kWave], sy[kWave], sz[kWave];
+    // Software-pipelined tiles: while tile t is evaluated from LDS buffer t&1,
+    // the particle loads of tile t+1 and the list index of tile t+2 are in
+    // flight (their first use comes after the evaluation).
+    __shared__ float4 so[2][kWave];  // FP32 offset from the centre, |offset|^2
+    __shared__ float sm[2][kWave];
     const Tiles tl(nj);
+    int qn = 0;  // list entry of tile t+1 for this lane (0 if none)
+    double px = 0, py = 0, pz = 0, pm = 0;  // particle data of tile t (lane's entry)
+    if (tl.T > 0) {
+        const int k0 = tl.entry(0, lane);
+        const int q0 = (k0 < nj) ? a.L.idx[lb + k0] : 0;
+        px = a.xj[q0]; py = a.yj[q0]; pz = a.zj[q0]; pm = a.mj[q0];
+        const int k1 = tl.entry(1, lane);
+        qn = (tl.T > 1 && k1 < nj) ? a.L.idx[lb + k1] : 0;
+    }
     for (int t = 0; t < tl.T; ++t) {
-        const int k = tl.entry(t, lane);
-        if (k < nj) {
-            const int q = a.L.idx[lb + k];
-            sm[lane] = (float) a.mj[q];
-            sx[lane] = (float) (a.xj[q] - cx);
-            sy[lane] = (float) (a.yj[q] - cy);
-            sz[lane] = (float) (a.zj[q] - cz);
+        const int b = t & 1;
+        if (tl.entry(t, lane) < nj) {
+            sm[b][lane] = (float) pm;
+            const float ox = (float) (px - cx), oy = (float) (py - cy), oz = (float) (pz - cz);
+            so[b][lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
         }
         __syncthreads();
+        if (t + 1 < tl.T) {  // issue tile t+1's particle loads and tile t+2's index load
+            px = a.xj[qn]; py = a.yj[qn]; pz = a.zj[qn]; pm = a.mj[qn];
+            const int k2 = tl.entry(t + 2, lane);
+            qn = (t + 2 < tl.T && k2 < nj) ? a.L.idx[lb + k2] : 0;
+        }
         const int nt = tl.count(t);
         float part = 0.f;
-        const uint64_t mask = support_mask(sx, sy, sz, nt, xi, yi, zi, rcut2);
+        const uint64_t mask = dot_mask(so[b], nt, tx, ty, tz, cdot);
         for (uint64_t msk = mask; msk; msk &= msk - 1) {
             const int jj = __builtin_ctzll(msk);
-            const float dx = sx[jj] - xi, dy = sy[jj] - yi, dz = sz[jj] - zi;
+            const float4 oj = so[b][jj];
+            const float dx = oj.x - xi, dy = oj.y - yi, dz = oj.z - zi;
             const float r2 = dx * dx + dy * dy + dz * dz;
             {
                 const float r = r2 * rsq(r2 + 1e-30f);
@@ -120,12 +127,11 @@ __global__ __launch_bounds__(kWave) void k_dens_mixed(DensArgs a) {
                 const float om = fmaxf(1.f - s, 0.f);
                 const float om2 = om * om, om4 = om2 * om2;
                 const float w = (1.f + s * (8.f + s * (25.f + s * 32.f))) * (om4 * om4);
-                part += sm[jj] * w;
+                part += sm[b][jj] * w;
                 ngb += (om > 0.f) ? 1 : 0;
             }
         }
         dens.add(part * norm);
-        __syncthreads();
     }
     if (act) {
         const double d = dmax(5.0, dens.value());
@@ -152,20 +158,21 @@ __global__ __launch_bounds__(kWave) void k_drvt_mixed(DrvtArgs a) {
     const KernelConsts kc = kernel_consts(hid);
     const float H = kc.H, invH = kc.invH, norm = kc.norm;
     const float reg = (float) (1.0e-6 * hid);
-    const float rcut2 = act ? H * H * 1.0001f : -1.f;
+    const float cdot = act ? H * H * 1.0001f - fmaf(xi, xi, fmaf(yi, yi, zi * zi)) : -INFINITY;
+    const float tx = -2.f * xi, ty = -2.f * yi, tz = -2.f * zi;
     Kahan divv, rx, ry, rz, gsum;
     const int64_t lb = list_begin(a.L, g);
     const int nj = a.L.cnt[g];
-    __shared__ float sm[kWave], sx[kWave], sy[kWave], sz[kWave], svx[kWave], svy[kWave], svz[kWave];
+    __shared__ float4 so[kWave];  // FP32 offset from the centre, |offset|^2
+    __shared__ float sm[kWave], svx[kWave], svy[kWave], svz[kWave];
     const Tiles tl(nj);
     for (int t = 0; t < tl.T; ++t) {
         const int k = tl.entry(t, lane);
         if (k < nj) {
             const int q = a.L.idx[lb + k];
             sm[lane] = (float) a.mj[q];
-            sx[lane] = (float) (a.xj[q] - cx);
-            sy[lane] = (float) (a.yj[q] - cy);
-            sz[lane] = (float) (a.zj[q] - cz);
+            const float ox = (float) (a.xj[q] - cx), oy = (float) (a.yj[q] - cy), oz = (float) (a.zj[q] - cz);
+            so[lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
             svx[lane] = (float) (a.vxj[q] - cvx);
             svy[lane] = (float) (a.vyj[q] - cvy);
             svz[lane] = (float) (a.vzj[q] - cvz);
@@ -173,10 +180,11 @@ __global__ __launch_bounds__(kWave) void k_drvt_mixed(DrvtArgs a) {
         __syncthreads();
         const int nt = tl.count(t);
         float pd = 0.f, px = 0.f, py = 0.f, pz = 0.f, pg = 0.f;
-        const uint64_t mask = support_mask(sx, sy, sz, nt, xi, yi, zi, rcut2);
+        const uint64_t mask = dot_mask(so, nt, tx, ty, tz, cdot);
         for (uint64_t msk = mask; msk; msk &= msk - 1) {
             const int jj = __builtin_ctzll(msk);
-            const float drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
+            const float4 oj = so[jj];
+            const float drx = xi - oj.x, dry = yi - oj.y, drz = zi - oj.z;
             const float r2 = drx * drx + dry * dry + drz * drz;
             {
                 const float dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];

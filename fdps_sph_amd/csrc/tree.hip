@@ -324,6 +324,7 @@ struct ListArgs {
     int64_t fcap;
     double theta2;
     double hfac;    // gather search radius = 2.5 h hfac (hfac > 1 makes a reusable superset list)
+    int32_t *queue; // 8 per-XCD group counters, zero at launch
     double *sbox;   // optional, per group: the search box the list was built for (lo[3], hi[3])
     int32_t *err;
 };
@@ -414,20 +415,29 @@ __device__ __forceinline__ bool accept_node(const ListArgs &a, int id, const GBo
     return (sz * sz) < (a.theta2 * d2);
 }
 
-// groups of the XCD this workgroup runs on, walked by the workgroups of that XCD
-struct GroupRange {
-    int beg, end, step, first;
+// The groups are split into 8 contiguous ranges, one per XCD (workgroup b
+// runs on XCD b mod 8), so that neighbouring groups -- which walk the same
+// nodes and read the same particles -- share an L2.  Inside its range each
+// wave takes the next group from the XCD's counter: walk costs vary a lot
+// between groups, and a static split leaves a long tail of busy waves.
+struct GroupQueue {
+    int beg, end;
+    int32_t *ctr;
+    __device__ __forceinline__ int next() const {
+        int k = 0;
+        if (threadIdx.x == 0) k = atomicAdd(ctr, 1);
+        k = __shfl(k, 0, kWave);
+        return beg + k;
+    }
 };
-__device__ __forceinline__ GroupRange xcd_groups(int ngroups) {
-    const int x = blockIdx.x & 7, kb = blockIdx.x >> 3;
-    const int nbx = ((int) gridDim.x - x + 7) >> 3;
+__device__ __forceinline__ GroupQueue xcd_queue(int ngroups, int32_t *ctr8) {
+    const int x = blockIdx.x & 7;
     const int q = ngroups >> 3, r = ngroups & 7;
-    GroupRange R;
-    R.beg = x * q + (x < r ? x : r);
-    R.end = R.beg + q + (x < r ? 1 : 0);
-    R.step = nbx;
-    R.first = R.beg + kb;
-    return R;
+    GroupQueue Q;
+    Q.beg = x * q + (x < r ? x : r);
+    Q.end = Q.beg + q + (x < r ? 1 : 0);
+    Q.ctr = ctr8 + x;
+    return Q;
 }
 
 // expand the frontier of one level: children of the nodes that pass `test`
@@ -510,8 +520,8 @@ __global__ __launch_bounds__(kWave) void k_list_sph(ListArgs a) {
     int32_t *fb0 = fa0 + a.fcap;
     __shared__ int s_pref[kWave + 1];
     __shared__ int s_leaf[kWave];
-    const GroupRange R = xcd_groups(a.ngroups);
-    for (int g = R.first; g < R.end; g += R.step) {
+    const GroupQueue Q = xcd_queue(a.ngroups, a.queue);
+    for (int g = Q.next(); g < Q.end; g = Q.next()) {
         const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
         GBox b;
         group_box(a, i0, ni, b);
@@ -543,8 +553,8 @@ __global__ __launch_bounds__(kWave) void k_list_grav(ListArgs a) {
     int32_t *fb0 = fa0 + a.fcap;
     __shared__ int s_pref[kWave + 1];
     __shared__ int s_leaf[kWave];
-    const GroupRange R = xcd_groups(a.ngroups);
-    for (int g = R.first; g < R.end; g += R.step) {
+    const GroupQueue Q = xcd_queue(a.ngroups, a.queue);
+    for (int g = Q.next(); g < Q.end; g = Q.next()) {
         const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
         GBox b;
         group_box(a, i0, ni, b);
@@ -809,6 +819,7 @@ static ListArgs list_args(const Tree &t, const TreeParams &tp, const PArrays &P,
     a.hfac = 1.0;
     a.sbox = nullptr;
     a.err = L.err.get();
+    a.queue = L.err.get() + 1;
     return a;
 }
 
@@ -830,8 +841,8 @@ static bool run_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int
     if (L.fcap <= 0) L.fcap = kFrontier0;
     const int grid = list_grid(G);
     L.frontier.ensure((size_t) grid * 2 * L.fcap);
-    L.err.ensure(1);
-    SPH_HIP(hipMemsetAsync(L.err.get(), 0, sizeof(int32_t), s));
+    L.err.ensure(1 + 8);  // error flags, then the 8 per-XCD group counters
+    SPH_HIP(hipMemsetAsync(L.err.get(), 0, sizeof(int32_t) * (1 + 8), s));
     ListArgs a = list_args(t, tp, P, L);
     if (kind == 0) {  // gather lists remember their search boxes (reuse check)
         L.sbox.ensure((size_t) G * 6);
