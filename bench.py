@@ -494,16 +494,22 @@ def main():
         pic = dict(ic)
         rnd = SplitMix64(4242).uniform(3 * n).reshape(n, 3) * 2.0 - 1.0
         pic["pos"] = ic["pos"] * (1.0 + args.sensitivity * rnd)
-        P = run(ctx["fp64"], pic, warmup, steps, f"fp64, positions perturbed by {args.sensitivity:g}", **params)
-        F0 = R0["fp64"]
-        out["fp64_sensitivity"] = {
-            "perturbation": args.sensitivity,
-            "energy_rel_diff": abs(P["E1"] - F0["E1"]) / abs(F0["E1"]),
-            "initial_offset": abs(P["E0"] - F0["E0"]) / abs(F0["E0"]),
-            "parts_diff_final": {k: P["parts1"][k] - F0["parts1"][k] for k in F0["parts1"]},
-            "steps": F0["steps_total"]}
-        log(f"[fp64 sensitivity] perturbation {args.sensitivity:g}: energy difference after {F0['steps_total']} "
-            f"steps {out['fp64_sensitivity']['energy_rel_diff']:.2e}")
+        sens = []
+        for th in sweep:
+            params["theta"] = th
+            P = run(ctx["fp64"], pic, warmup, steps, f"fp64 theta {th}, positions perturbed by {args.sensitivity:g}",
+                    **params)
+            F0 = results[th]["fp64"]
+            e = {"theta": th, "perturbation": args.sensitivity,
+                 "energy_rel_diff": abs(P["E1"] - F0["E1"]) / abs(F0["E1"]),
+                 "initial_offset": abs(P["E0"] - F0["E0"]) / abs(F0["E0"]),
+                 "parts_diff_final": {k: P["parts1"][k] - F0["parts1"][k] for k in F0["parts1"]},
+                 "steps": F0["steps_total"]}
+            sens.append(e)
+            log(f"[fp64 sensitivity] theta {th}, perturbation {args.sensitivity:g}: energy difference after "
+                f"{F0['steps_total']} steps {e['energy_rel_diff']:.2e}")
+        out["fp64_sensitivity"] = sens if len(sens) > 1 else sens[0]
+        params["theta"] = th0
     out.update(extra)
     out["graph_captured"] = False
 

@@ -153,6 +153,8 @@ struct SimState {
     double kms[kStatKinds] = {0};
     int64_t klaunch[kStatKinds] = {0};
     DevBuf<long long> kcount;  // [kStatKinds][4]: n_i, n_j, n_s, pairs
+    DevBuf<HydroRec> hrec;     // j records of the hydro pass
+    DevBuf<KinRec> krec;       // j records of the density and derivative passes
     int64_t gather_builds = 0; // gather-list constructions since init
     int64_t force_evals = 0;   // force evaluations (sim_forces calls) since init
     ~SimState() {
@@ -503,6 +505,8 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
     PArrays &P = S.P;
     const int G = (int) S.tree.ngroups;
     const int32_t *gs = S.tree.gstart.get();
+    KinRec *krec = S.krec.ensure(n);  // j records of the density and derivative passes
+    launch_pack_kin(n, P.x, P.y, P.z, P.m, P.vx, P.vy, P.vz, krec, s);
     int kslot = 0;
     auto kbegin = [&](const Lists *Lp, bool spj, int kind) {
         if (!S.stats) return;
@@ -529,7 +533,7 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
             build_lists_sph(S.tree, S.tp, P, 0, L, s, kGatherMargin);
             ++S.gather_builds;
         }
-        DensArgs a{G, gs, P.x, P.y, P.z, P.m, P.h, P.m, P.x, P.y, P.z, lref(), P.rho, P.h, P.ngb};
+        DensArgs a{G, gs, P.x, P.y, P.z, P.m, P.h, P.m, P.x, P.y, P.z, lref(), P.rho, P.h, P.ngb, krec};
         // the density of the last pass feeds the EoS; the precision form keeps
         // it FP64 unless asked otherwise (the earlier passes only steer h)
         const bool d64 = !mixed || (loop == 3 && S.prm.eos_density_fp64);
@@ -547,7 +551,7 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
     }
     {
         DrvtArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.cs, P.m, P.x, P.y, P.z, P.vx, P.vy, P.vz,
-                   lref(), P.divv, P.rotx, P.roty, P.rotz, P.gs, P.bal};
+                   lref(), P.divv, P.rotx, P.roty, P.rotz, P.gs, P.bal, krec};
         kbegin(&L, false, 1);
         if (mixed) launch_drvt_mixed(a, s);
         else launch_drvt_fp64(a, s);
@@ -557,9 +561,11 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
     tree_update_maxH(S.tree, P.h, s);
     build_lists_sph(S.tree, S.tp, P, 1, L, s);
     {
+        HydroRec *rec = S.hrec.ensure(n);
+        launch_pack_hydro(n, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.m, P.rho, P.cs, P.bal, P.gs, P.p, P.h, rec, s);
         HydroArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal,
                     P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal, P.m,
-                    lref(), P.ax, P.ay, P.az, P.du, P.dti};
+                    lref(), P.ax, P.ay, P.az, P.du, P.dti, rec};
         kbegin(&L, false, 2);
         if (mixed) launch_hydro_mixed(a, s);
         else launch_hydro_fp64(a, s);

@@ -108,6 +108,13 @@ struct PArrays {
 };
 
 // ------------------------------------------------------------------ interaction kernel arguments
+// The j side of the density and derivative passes as one 64-byte record per
+// particle (positions, mass and velocities are fixed during a force evaluation).
+struct alignas(16) KinRec {
+    double x, y, z, m, vx, vy, vz, pad;
+};
+static_assert(sizeof(KinRec) == 64, "two records per cache line");
+
 struct DensArgs {
     int ngroups;
     const int32_t *gstart;
@@ -116,6 +123,7 @@ struct DensArgs {
     ListRef L;
     double *rho_out, *h_out;
     int32_t *ngb_out;
+    const KinRec *jrec = nullptr;  // if set: the j side is read from here
 };
 
 struct DrvtArgs {
@@ -126,7 +134,17 @@ struct DrvtArgs {
     const double *mj, *xj, *yj, *zj, *vxj, *vyj, *vzj;
     ListRef L;
     double *divv, *rotx, *roty, *rotz, *gs, *bal;
+    const KinRec *jrec = nullptr;  // if set: the j side is read from here
 };
+
+// The j side of the hydro pass as one 128-byte record per particle (one
+// cache line per list entry instead of 13 gathers from separate arrays):
+// A = f P / rho^2, H = 2.5 h and the kernel normalisation are formed by the
+// same FP64 operations the kernel would apply to the separate fields.
+struct alignas(16) HydroRec {
+    double x, y, z, vx, vy, vz, m, rho, cs, bal, gs, A, h, H, norm, pad;
+};
+static_assert(sizeof(HydroRec) == 128, "one cache line per record");
 
 struct HydroArgs {
     int ngroups;
@@ -135,6 +153,7 @@ struct HydroArgs {
     const double *xj, *yj, *zj, *vxj, *vyj, *vzj, *hj, *rhoj, *pj, *csj, *gsj, *balj, *mj;
     ListRef L;
     double *ax, *ay, *az, *du, *dt;
+    const HydroRec *jrec = nullptr;  // if set: the j side is read from here (list indices index it)
 };
 
 struct GravArgs {
@@ -149,6 +168,15 @@ struct GravArgs {
     ListRef Ls;
     double *ax, *ay, *az, *pot;
 };
+
+// Pack the kinematic records of particles [0, n).
+void launch_pack_kin(int64_t n, const double *x, const double *y, const double *z, const double *m, const double *vx,
+                     const double *vy, const double *vz, KinRec *out, hipStream_t s);
+// Pack the hydro records of particles [0, n) from the SoA fields.
+void launch_pack_hydro(int64_t n, const double *x, const double *y, const double *z, const double *vx,
+                       const double *vy, const double *vz, const double *m, const double *rho, const double *cs,
+                       const double *bal, const double *gs, const double *p, const double *h, HydroRec *out,
+                       hipStream_t s);
 
 // FP64 (plain) interaction kernels -- kernels_fp64.hip
 void launch_dens_fp64(const DensArgs &a, hipStream_t s);

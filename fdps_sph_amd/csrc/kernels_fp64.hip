@@ -124,8 +124,10 @@ __global__ __launch_bounds__(kWave) void k_dens(DensArgs a) {
         float nj1 = 0.f;
         if (k < tl.nj) {
             const int q = a.L.idx[lb + k];
-            const double x = a.xj[q], y = a.yj[q], z = a.zj[q];
-            sp[lane] = make_double4(x, y, z, a.mj[q]);
+            const double4 pq = a.jrec ? *reinterpret_cast<const double4 *>(a.jrec + q)
+                                      : make_double4(a.xj[q], a.yj[q], a.zj[q], a.mj[q]);
+            const double x = pq.x, y = pq.y, z = pq.z;
+            sp[lane] = pq;
             const float ox = (float) (x - fr.cx), oy = (float) (y - fr.cy), oz = (float) (z - fr.cz);
             so[lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
             nj1 = norm1_(ox, oy, oz);
@@ -181,9 +183,17 @@ __global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
         float nj1 = 0.f;
         if (k < tl.nj) {
             const int q = a.L.idx[lb + k];
-            const double x = a.xj[q], y = a.yj[q], z = a.zj[q];
-            sp[lane] = make_double4(x, y, z, a.mj[q]);
-            sv[lane] = make_double4(a.vxj[q], a.vyj[q], a.vzj[q], 0.0);
+            double4 pq, vq;
+            if (a.jrec) {
+                pq = *reinterpret_cast<const double4 *>(a.jrec + q);
+                vq = *(reinterpret_cast<const double4 *>(a.jrec + q) + 1);
+            } else {
+                pq = make_double4(a.xj[q], a.yj[q], a.zj[q], a.mj[q]);
+                vq = make_double4(a.vxj[q], a.vyj[q], a.vzj[q], 0.0);
+            }
+            const double x = pq.x, y = pq.y, z = pq.z;
+            sp[lane] = pq;
+            sv[lane] = vq;
             const float ox = (float) (x - fr.cx), oy = (float) (y - fr.cy), oz = (float) (z - fr.cz);
             so[lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
             nj1 = norm1_(ox, oy, oz);
@@ -237,9 +247,41 @@ __global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
 // ---------------------------------------------------------------- hydro force
 // CalcHydroForce::operator() (src/force.h:112-147), `#if 1` branch.  v_sig_max
 // runs over every j of the list; the force terms only inside either support.
-struct HydroJ {   // per-j record of a tile (LDS)
-    double x, y, z, vx, vy, vz, m, rho, cs, bal, gs, A, h, H, norm;
-};
+using HydroJ = HydroRec;  // per-j record of a tile (LDS)
+
+__device__ __forceinline__ HydroRec make_hydro_rec(double x, double y, double z, double vx, double vy, double vz,
+                                                   double m, double rho, double cs, double bal, double gs, double p,
+                                                   double h) {
+    HydroRec r;
+    r.x = x; r.y = y; r.z = z;
+    r.vx = vx; r.vy = vy; r.vz = vz;
+    r.m = m;
+    r.rho = rho;
+    r.cs = cs;
+    r.bal = bal;
+    r.gs = gs;
+    r.A = gs * p / (rho * rho);
+    r.h = h;
+    r.H = kSupport * h;
+    r.norm = wnorm_(r.H);
+    r.pad = 0.0;
+    return r;
+}
+
+__global__ void k_pack_kin(int64_t n, const double *x, const double *y, const double *z, const double *m,
+                           const double *vx, const double *vy, const double *vz, KinRec *out) {
+    const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = KinRec{x[k], y[k], z[k], m[k], vx[k], vy[k], vz[k], 0.0};
+}
+
+__global__ void k_pack_hydro(int64_t n, const double *x, const double *y, const double *z, const double *vx,
+                             const double *vy, const double *vz, const double *m, const double *rho,
+                             const double *cs, const double *bal, const double *gs, const double *p, const double *h,
+                             HydroRec *out) {
+    const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = make_hydro_rec(x[k], y[k], z[k], vx[k], vy[k], vz[k], m[k], rho[k], cs[k], bal[k], gs[k],
+                                       p[k], h[k]);
+}
 __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
     const int g = xcd_remap(blockIdx.x, a.ngroups);
     const int lane = threadIdx.x;
@@ -268,22 +310,9 @@ __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
         float padj = 0.f;
         if (k < tl.nj) {
             const int q = a.L.idx[lb + k];
-            HydroJ r;
-            r.x = a.xj[q];
-            r.y = a.yj[q];
-            r.z = a.zj[q];
-            r.vx = a.vxj[q];
-            r.vy = a.vyj[q];
-            r.vz = a.vzj[q];
-            r.m = a.mj[q];
-            r.rho = a.rhoj[q];
-            r.cs = a.csj[q];
-            r.bal = a.balj[q];
-            r.gs = a.gsj[q];
-            r.A = r.gs * a.pj[q] / (r.rho * r.rho);
-            r.h = a.hj[q];
-            r.H = kSupport * r.h;
-            r.norm = wnorm_(r.H);
+            const HydroJ r = a.jrec ? a.jrec[q]
+                                    : make_hydro_rec(a.xj[q], a.yj[q], a.zj[q], a.vxj[q], a.vyj[q], a.vzj[q], a.mj[q],
+                                                     a.rhoj[q], a.csj[q], a.balj[q], a.gsj[q], a.pj[q], a.hj[q]);
             sj[lane] = r;
             const float ox = (float) (r.x - fr.cx), oy = (float) (r.y - fr.cy), oz = (float) (r.z - fr.cz);
             padj = pad_(ox, oy, oz);
@@ -422,6 +451,21 @@ __global__ __launch_bounds__(kWave) void k_grav(GravArgs a) {
 
 }  // namespace
 
+void launch_pack_kin(int64_t n, const double *x, const double *y, const double *z, const double *m, const double *vx,
+                     const double *vy, const double *vz, KinRec *out, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_pack_kin, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, n, x, y, z, m, vx, vy, vz, out);
+    SPH_LAUNCH_CHECK();
+}
+void launch_pack_hydro(int64_t n, const double *x, const double *y, const double *z, const double *vx,
+                       const double *vy, const double *vz, const double *m, const double *rho, const double *cs,
+                       const double *bal, const double *gs, const double *p, const double *h, HydroRec *out,
+                       hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_pack_hydro, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, n, x, y, z, vx, vy, vz, m,
+                       rho, cs, bal, gs, p, h, out);
+    SPH_LAUNCH_CHECK();
+}
 void launch_dens_fp64(const DensArgs &a, hipStream_t s) {
     if (a.ngroups <= 0) return;
     hipLaunchKernelGGL(k_dens, dim3(a.ngroups), dim3(kWave), 0, s, a);

@@ -65,6 +65,16 @@ __device__ __forceinline__ KernelConsts kernel_consts(double h) {
     return {(float) Hd, (float) (1.0 / Hd), (float) (kNormD / (Hd * Hd * Hd))};
 }
 
+// j position and mass, from the packed record when there is one
+__device__ __forceinline__ void load_xyzm(const DensArgs &a, int q, double &x, double &y, double &z, double &m) {
+    if (a.jrec) {
+        const double4 r = *reinterpret_cast<const double4 *>(a.jrec + q);
+        x = r.x; y = r.y; z = r.z; m = r.w;
+    } else {
+        x = a.xj[q]; y = a.yj[q]; z = a.zj[q]; m = a.mj[q];
+    }
+}
+
 // ---------------------------------------------------------------- density
 __global__ __launch_bounds__(kWave) void k_dens_mixed(DensArgs a) {
     const int g = xcd_remap(blockIdx.x, a.ngroups);
@@ -96,7 +106,7 @@ __global__ __launch_bounds__(kWave) void k_dens_mixed(DensArgs a) {
     if (tl.T > 0) {
         const int k0 = tl.entry(0, lane);
         const int q0 = (k0 < nj) ? a.L.idx[lb + k0] : 0;
-        px = a.xj[q0]; py = a.yj[q0]; pz = a.zj[q0]; pm = a.mj[q0];
+        load_xyzm(a, q0, px, py, pz, pm);
         const int k1 = tl.entry(1, lane);
         qn = (tl.T > 1 && k1 < nj) ? a.L.idx[lb + k1] : 0;
     }
@@ -109,7 +119,7 @@ __global__ __launch_bounds__(kWave) void k_dens_mixed(DensArgs a) {
         }
         __syncthreads();
         if (t + 1 < tl.T) {  // issue tile t+1's particle loads and tile t+2's index load
-            px = a.xj[qn]; py = a.yj[qn]; pz = a.zj[qn]; pm = a.mj[qn];
+            load_xyzm(a, qn, px, py, pz, pm);
             const int k2 = tl.entry(t + 2, lane);
             qn = (t + 2 < tl.T && k2 < nj) ? a.L.idx[lb + k2] : 0;
         }
@@ -170,12 +180,20 @@ __global__ __launch_bounds__(kWave) void k_drvt_mixed(DrvtArgs a) {
         const int k = tl.entry(t, lane);
         if (k < nj) {
             const int q = a.L.idx[lb + k];
-            sm[lane] = (float) a.mj[q];
-            const float ox = (float) (a.xj[q] - cx), oy = (float) (a.yj[q] - cy), oz = (float) (a.zj[q] - cz);
+            double4 pq, vq;
+            if (a.jrec) {
+                pq = *reinterpret_cast<const double4 *>(a.jrec + q);
+                vq = *(reinterpret_cast<const double4 *>(a.jrec + q) + 1);
+            } else {
+                pq = make_double4(a.xj[q], a.yj[q], a.zj[q], a.mj[q]);
+                vq = make_double4(a.vxj[q], a.vyj[q], a.vzj[q], 0.0);
+            }
+            sm[lane] = (float) pq.w;
+            const float ox = (float) (pq.x - cx), oy = (float) (pq.y - cy), oz = (float) (pq.z - cz);
             so[lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
-            svx[lane] = (float) (a.vxj[q] - cvx);
-            svy[lane] = (float) (a.vyj[q] - cvy);
-            svz[lane] = (float) (a.vzj[q] - cvz);
+            svx[lane] = (float) (vq.x - cvx);
+            svy[lane] = (float) (vq.y - cvy);
+            svz[lane] = (float) (vq.z - cvz);
         }
         __syncthreads();
         const int nt = tl.count(t);
@@ -259,24 +277,34 @@ __global__ __launch_bounds__(kWave) void k_hydro_mixed(HydroArgs a) {
         const int k = tl.entry(t, lane);
         if (k < nj) {
             const int q = a.L.idx[lb + k];
-            sx[lane] = (float) (a.xj[q] - cx);
-            sy[lane] = (float) (a.yj[q] - cy);
-            sz[lane] = (float) (a.zj[q] - cz);
-            svx[lane] = (float) (a.vxj[q] - cvx);
-            svy[lane] = (float) (a.vyj[q] - cvy);
-            svz[lane] = (float) (a.vzj[q] - cvz);
-            sm[lane] = (float) a.mj[q];
-            const double rj = a.rhoj[q], gj = a.gsj[q], hj = a.hj[q];
+            double x, y, z, vx, vy, vz, m, rj, cj, bj, gj, Aj, hj, Hj, nj3;
+            if (a.jrec) {  // one 128-byte record per entry
+                const HydroRec r = a.jrec[q];
+                x = r.x; y = r.y; z = r.z; vx = r.vx; vy = r.vy; vz = r.vz; m = r.m;
+                rj = r.rho; cj = r.cs; bj = r.bal; gj = r.gs; Aj = r.A; hj = r.h; Hj = r.H; nj3 = r.norm;
+            } else {
+                x = a.xj[q]; y = a.yj[q]; z = a.zj[q]; vx = a.vxj[q]; vy = a.vyj[q]; vz = a.vzj[q]; m = a.mj[q];
+                rj = a.rhoj[q]; cj = a.csj[q]; bj = a.balj[q]; gj = a.gsj[q]; hj = a.hj[q];
+                Aj = gj * a.pj[q] / (rj * rj);
+                Hj = kSupport * hj;
+                nj3 = kNormD / (Hj * Hj * Hj);
+            }
+            sx[lane] = (float) (x - cx);
+            sy[lane] = (float) (y - cy);
+            sz[lane] = (float) (z - cz);
+            svx[lane] = (float) (vx - cvx);
+            svy[lane] = (float) (vy - cvy);
+            svz[lane] = (float) (vz - cvz);
+            sm[lane] = (float) m;
             srho[lane] = (float) rj;
-            scs[lane] = (float) a.csj[q];
-            sbal[lane] = (float) a.balj[q];
+            scs[lane] = (float) cj;
+            sbal[lane] = (float) bj;
             sgs[lane] = (float) gj;
-            sA[lane] = (float) (gj * a.pj[q] / (rj * rj));
-            const double Hj = kSupport * hj;
+            sA[lane] = (float) Aj;
             sH[lane] = (float) Hj;
             sinvH[lane] = (float) (1.0 / Hj);
             sreg[lane] = (float) (1.0e-6 * hj);
-            snorm[lane] = (float) (1365.0 / 64.0 / kPi / (Hj * Hj * Hj));
+            snorm[lane] = (float) nj3;
             scut[lane] = (float) (Hj * Hj) * 1.0001f;
         }
         __syncthreads();
