@@ -154,7 +154,7 @@ struct SimState {
     int64_t klaunch[kStatKinds] = {0};
     DevBuf<long long> kcount;  // [kStatKinds][4]: n_i, n_j, n_s, pairs
     DevBuf<HydroRec> hrec;     // j records of the hydro pass
-    DevBuf<KinRec> krec;       // j records of the density and derivative passes
+    DevBuf<double4> kpos, kvel; // j records of the density and derivative passes
     int64_t gather_builds = 0; // gather-list constructions since init
     int64_t force_evals = 0;   // force evaluations (sim_forces calls) since init
     ~SimState() {
@@ -505,8 +505,8 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
     PArrays &P = S.P;
     const int G = (int) S.tree.ngroups;
     const int32_t *gs = S.tree.gstart.get();
-    KinRec *krec = S.krec.ensure(n);  // j records of the density and derivative passes
-    launch_pack_kin(n, P.x, P.y, P.z, P.m, P.vx, P.vy, P.vz, krec, s);
+    double4 *kpos = S.kpos.ensure(n), *kvel = S.kvel.ensure(n);  // j records of the density / derivative passes
+    launch_pack_kin(n, P.x, P.y, P.z, P.m, P.vx, P.vy, P.vz, kpos, kvel, s);
     int kslot = 0;
     auto kbegin = [&](const Lists *Lp, bool spj, int kind) {
         if (!S.stats) return;
@@ -533,7 +533,7 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
             build_lists_sph(S.tree, S.tp, P, 0, L, s, kGatherMargin);
             ++S.gather_builds;
         }
-        DensArgs a{G, gs, P.x, P.y, P.z, P.m, P.h, P.m, P.x, P.y, P.z, lref(), P.rho, P.h, P.ngb, krec};
+        DensArgs a{G, gs, P.x, P.y, P.z, P.m, P.h, P.m, P.x, P.y, P.z, lref(), P.rho, P.h, P.ngb, kpos};
         // the density of the last pass feeds the EoS; the precision form keeps
         // it FP64 unless asked otherwise (the earlier passes only steer h)
         const bool d64 = !mixed || (loop == 3 && S.prm.eos_density_fp64);
@@ -551,7 +551,7 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
     }
     {
         DrvtArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.cs, P.m, P.x, P.y, P.z, P.vx, P.vy, P.vz,
-                   lref(), P.divv, P.rotx, P.roty, P.rotz, P.gs, P.bal, krec};
+                   lref(), P.divv, P.rotx, P.roty, P.rotz, P.gs, P.bal, kpos, kvel};
         kbegin(&L, false, 1);
         if (mixed) launch_drvt_mixed(a, s);
         else launch_drvt_fp64(a, s);

@@ -108,13 +108,6 @@ struct PArrays {
 };
 
 // ------------------------------------------------------------------ interaction kernel arguments
-// The j side of the density and derivative passes as one 64-byte record per
-// particle (positions, mass and velocities are fixed during a force evaluation).
-struct alignas(16) KinRec {
-    double x, y, z, m, vx, vy, vz, pad;
-};
-static_assert(sizeof(KinRec) == 64, "two records per cache line");
-
 struct DensArgs {
     int ngroups;
     const int32_t *gstart;
@@ -123,7 +116,8 @@ struct DensArgs {
     ListRef L;
     double *rho_out, *h_out;
     int32_t *ngb_out;
-    const KinRec *jrec = nullptr;  // if set: the j side is read from here
+    // if set: the j side packed as (x, y, z, m), one 32-byte record per particle
+    const double4 *jpos = nullptr;
 };
 
 struct DrvtArgs {
@@ -134,7 +128,8 @@ struct DrvtArgs {
     const double *mj, *xj, *yj, *zj, *vxj, *vyj, *vzj;
     ListRef L;
     double *divv, *rotx, *roty, *rotz, *gs, *bal;
-    const KinRec *jrec = nullptr;  // if set: the j side is read from here
+    // if set: the j side packed as (x, y, z, m) and (vx, vy, vz, 0)
+    const double4 *jpos = nullptr, *jvel = nullptr;
 };
 
 // The j side of the hydro pass as one 128-byte record per particle (one
@@ -169,9 +164,11 @@ struct GravArgs {
     double *ax, *ay, *az, *pot;
 };
 
-// Pack the kinematic records of particles [0, n).
+// Pack the j records of the density and derivative passes for particles
+// [0, n): (x, y, z, m) and (vx, vy, vz, 0); positions, masses and velocities
+// are fixed during a force evaluation.
 void launch_pack_kin(int64_t n, const double *x, const double *y, const double *z, const double *m, const double *vx,
-                     const double *vy, const double *vz, KinRec *out, hipStream_t s);
+                     const double *vy, const double *vz, double4 *pos, double4 *vel, hipStream_t s);
 // Pack the hydro records of particles [0, n) from the SoA fields.
 void launch_pack_hydro(int64_t n, const double *x, const double *y, const double *z, const double *vx,
                        const double *vy, const double *vz, const double *m, const double *rho, const double *cs,

@@ -124,8 +124,7 @@ __global__ __launch_bounds__(kWave) void k_dens(DensArgs a) {
         float nj1 = 0.f;
         if (k < tl.nj) {
             const int q = a.L.idx[lb + k];
-            const double4 pq = a.jrec ? *reinterpret_cast<const double4 *>(a.jrec + q)
-                                      : make_double4(a.xj[q], a.yj[q], a.zj[q], a.mj[q]);
+            const double4 pq = a.jpos ? a.jpos[q] : make_double4(a.xj[q], a.yj[q], a.zj[q], a.mj[q]);
             const double x = pq.x, y = pq.y, z = pq.z;
             sp[lane] = pq;
             const float ox = (float) (x - fr.cx), oy = (float) (y - fr.cy), oz = (float) (z - fr.cz);
@@ -184,9 +183,9 @@ __global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
         if (k < tl.nj) {
             const int q = a.L.idx[lb + k];
             double4 pq, vq;
-            if (a.jrec) {
-                pq = *reinterpret_cast<const double4 *>(a.jrec + q);
-                vq = *(reinterpret_cast<const double4 *>(a.jrec + q) + 1);
+            if (a.jpos) {
+                pq = a.jpos[q];
+                vq = a.jvel[q];
             } else {
                 pq = make_double4(a.xj[q], a.yj[q], a.zj[q], a.mj[q]);
                 vq = make_double4(a.vxj[q], a.vyj[q], a.vzj[q], 0.0);
@@ -269,9 +268,12 @@ __device__ __forceinline__ HydroRec make_hydro_rec(double x, double y, double z,
 }
 
 __global__ void k_pack_kin(int64_t n, const double *x, const double *y, const double *z, const double *m,
-                           const double *vx, const double *vy, const double *vz, KinRec *out) {
+                           const double *vx, const double *vy, const double *vz, double4 *pos, double4 *vel) {
     const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) out[k] = KinRec{x[k], y[k], z[k], m[k], vx[k], vy[k], vz[k], 0.0};
+    if (k < n) {
+        pos[k] = make_double4(x[k], y[k], z[k], m[k]);
+        vel[k] = make_double4(vx[k], vy[k], vz[k], 0.0);
+    }
 }
 
 __global__ void k_pack_hydro(int64_t n, const double *x, const double *y, const double *z, const double *vx,
@@ -452,9 +454,10 @@ __global__ __launch_bounds__(kWave) void k_grav(GravArgs a) {
 }  // namespace
 
 void launch_pack_kin(int64_t n, const double *x, const double *y, const double *z, const double *m, const double *vx,
-                     const double *vy, const double *vz, KinRec *out, hipStream_t s) {
+                     const double *vy, const double *vz, double4 *pos, double4 *vel, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_pack_kin, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, n, x, y, z, m, vx, vy, vz, out);
+    hipLaunchKernelGGL(k_pack_kin, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, n, x, y, z, m, vx, vy, vz, pos,
+                       vel);
     SPH_LAUNCH_CHECK();
 }
 void launch_pack_hydro(int64_t n, const double *x, const double *y, const double *z, const double *vx,

@@ -67,8 +67,8 @@ __device__ __forceinline__ KernelConsts kernel_consts(double h) {
 
 // j position and mass, from the packed record when there is one
 __device__ __forceinline__ void load_xyzm(const DensArgs &a, int q, double &x, double &y, double &z, double &m) {
-    if (a.jrec) {
-        const double4 r = *reinterpret_cast<const double4 *>(a.jrec + q);
+    if (a.jpos) {
+        const double4 r = a.jpos[q];
         x = r.x; y = r.y; z = r.z; m = r.w;
     } else {
         x = a.xj[q]; y = a.yj[q]; z = a.zj[q]; m = a.mj[q];
@@ -181,9 +181,9 @@ __global__ __launch_bounds__(kWave) void k_drvt_mixed(DrvtArgs a) {
         if (k < nj) {
             const int q = a.L.idx[lb + k];
             double4 pq, vq;
-            if (a.jrec) {
-                pq = *reinterpret_cast<const double4 *>(a.jrec + q);
-                vq = *(reinterpret_cast<const double4 *>(a.jrec + q) + 1);
+            if (a.jpos) {
+                pq = a.jpos[q];
+                vq = a.jvel[q];
             } else {
                 pq = make_double4(a.xj[q], a.yj[q], a.zj[q], a.mj[q]);
                 vq = make_double4(a.vxj[q], a.vyj[q], a.vzj[q], 0.0);
