@@ -952,6 +952,7 @@ int32_t sphgpu_sim_init(sphgpu_ctx *c, const sphgpu_sim_params *prm, int64_t n, 
             SPH_HIP(hipMemsetAsync(z, 0, sizeof(double) * n, s));
         SPH_HIP(hipMemcpy(S->tree.sys_start.ensure(nsys + 1), S->sys_start_h.data(), sizeof(int32_t) * (nsys + 1),
                           hipMemcpyHostToDevice));
+        S->tree.sys_start_h = S->sys_start_h;
         S->dt_sys.ensure(nsys);
         S->time_sys.ensure(nsys);
         SPH_HIP(hipMemsetAsync(S->time_sys.get(), 0, sizeof(double) * nsys, s));

@@ -212,6 +212,8 @@ struct TreeParams {
 struct Tree {
     // per-system
     DevBuf<int32_t> sys_start;  // nsys+1 (host-known, fixed)
+    std::vector<int32_t> sys_start_h;  // the same offsets on the host
+    PinnedBuf<int32_t> hcnt;    // level / group counts read back per build
     DevBuf<double> sys_lo, sys_scale;  // 3*nsys, nsys
     DevBuf<double> part;           // bbox partials
     DevBuf<int32_t> sys_root;      // nsys
@@ -246,6 +248,7 @@ struct Lists {
     DevBuf<int32_t> frontier;
     int64_t fcap = 0;        // per-workgroup BFS frontier capacity
     DevBuf<int32_t> err, flag;
+    PinnedBuf<int32_t> hflag;  // host side of the overflow / validity flags
     DevBuf<double> sbox;     // gather lists: per-group search box (lo[3], hi[3])
     int kind = -1;           // kind of the lists held (-1 none, 0 gather, 1 symmetric, 2 gravity)
 };

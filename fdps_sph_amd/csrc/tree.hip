@@ -629,8 +629,8 @@ inline int nblk(int64_t n, int b) { return (int) ((n + b - 1) / b); }
 // ------------------------------------------------------------------ host side
 static void compute_sys_boxes(Tree &t, const TreeParams &tp, int64_t n, const PArrays &P, hipStream_t s) {
     (void) n;
-    std::vector<int32_t> ss(tp.nsys + 1);
-    SPH_HIP(hipMemcpy(ss.data(), t.sys_start.get(), sizeof(int32_t) * (tp.nsys + 1), hipMemcpyDeviceToHost));
+    const std::vector<int32_t> &ss = t.sys_start_h;  // system offsets never change after init
+    if ((int) ss.size() != tp.nsys + 1) throw Error(-2, "tree: system offsets not set");
     int maxn = 0;
     for (int q = 0; q < tp.nsys; ++q) maxn = std::max(maxn, ss[q + 1] - ss[q]);
     const int nchunk = std::max(1, (maxn + kChunk - 1) / kChunk);
@@ -673,8 +673,8 @@ static void build_levels(Tree &t, const TreeParams &tp, int64_t n, const PArrays
                                               t.counts.get() + l, (int) n, s));
         if (l < L - 1) T *= 8;
     }
-    std::vector<int32_t> cnts(L + 1);
-    SPH_HIP(hipMemcpyAsync(cnts.data(), t.counts.get(), sizeof(int32_t) * L, hipMemcpyDeviceToHost, s));
+    int32_t *cnts = t.hcnt.ensure(L + 1);  // pinned: a direct copy, no staging
+    SPH_HIP(hipMemcpyAsync(cnts, t.counts.get(), sizeof(int32_t) * L, hipMemcpyDeviceToHost, s));
     SPH_HIP(hipStreamSynchronize(s));
     t.nlevels = L;
     t.level_off[0] = 0;
@@ -725,10 +725,10 @@ static void build_levels(Tree &t, const TreeParams &tp, int64_t n, const PArrays
     size_t bytes = t.sel_tmp_bytes;
     SPH_HIP(hipcub::DeviceSelect::Flagged(t.sel_tmp.get(), bytes, it, t.flags.get(), t.sel_out.get() + (size_t) L * n,
                                           t.counts.get() + L, (int) n, s));
-    int32_t ng = 0;
-    SPH_HIP(hipMemcpyAsync(&ng, t.counts.get() + L, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    int32_t *hng = t.hcnt.p + L;
+    SPH_HIP(hipMemcpyAsync(hng, t.counts.get() + L, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     SPH_HIP(hipStreamSynchronize(s));
-    t.ngroups = ng;
+    t.ngroups = *hng;
     t.gstart.ensure(t.ngroups + 1);
     SPH_HIP(hipMemcpyAsync(t.gstart.get(), t.sel_out.get() + (size_t) L * n, sizeof(int32_t) * t.ngroups,
                            hipMemcpyDeviceToDevice, s));
@@ -853,9 +853,10 @@ static bool run_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int
     else if (kind == 1) hipLaunchKernelGGL(k_list_sph<1>, dim3(grid), dim3(kWave), 0, s, a);
     else hipLaunchKernelGGL(k_list_grav, dim3(grid), dim3(kWave), 0, s, a);
     SPH_LAUNCH_CHECK();
-    int32_t err = 0;
-    SPH_HIP(hipMemcpyAsync(&err, L.err.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    int32_t *herr = L.hflag.ensure(1);
+    SPH_HIP(hipMemcpyAsync(herr, L.err.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
     SPH_HIP(hipStreamSynchronize(s));
+    const int32_t err = *herr;
     bool again = false;
     if (err & 4) { L.fcap *= 2; again = true; }
     if (err & 1) { L.cap *= 2; again = true; }
@@ -880,10 +881,10 @@ bool gather_lists_valid(const Tree &t, const PArrays &P, Lists &L, hipStream_t s
     hipLaunchKernelGGL(k_list_box_check, dim3((unsigned) t.ngroups), dim3(kWave), 0, s, (int) t.ngroups,
                        t.gstart.get(), P.x, P.y, P.z, P.h, L.sbox.get(), L.flag.get());
     SPH_LAUNCH_CHECK();
-    int32_t flag = 0;
-    SPH_HIP(hipMemcpyAsync(&flag, L.flag.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    int32_t *hf = L.hflag.ensure(1);
+    SPH_HIP(hipMemcpyAsync(hf, L.flag.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
     SPH_HIP(hipStreamSynchronize(s));
-    return flag == 0;
+    return *hf == 0;
 }
 
 bool build_lists_grav(const Tree &t, const TreeParams &tp, const PArrays &P, Lists &L, hipStream_t s) {
