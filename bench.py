@@ -23,6 +23,10 @@ Workloads (--workload):
                U[0,75] deg, one launch per stage for all systems; n_group and
                n_leaf picked by fdps_sph_amd.autotune
   impact16m    12M iron/dunite target + 4M granite impactor, 20 steps
+  functor1m    the drop-in boundary: the density pass of impact1m through the
+               multiwalk dispatch / retrieve entry points (host EPI/EPJ arrays
+               in the reference's layout, 200 walks per call, two tags in
+               flight), PCIe included; reported beside the device-resident rate
 
 Mixed-precision error (`mixed.err`): per-particle relative error of density,
 acceleration (hydro, gravity, total) and du/dt against the FP64 path from the
@@ -67,7 +71,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
 DEFAULTS = {  # workload: (warmup, steps)
     "impact1m": (10, 90), "granite10k": (5, 45), "theta-sweep": (10, 90), "batch64": (5, 20),
-    "impact16m": (2, 18)}
+    "impact16m": (2, 18), "functor1m": (1, 3)}
 
 
 def log(*a):
@@ -78,7 +82,7 @@ def log(*a):
 def make_workload(name, n=None):
     from fdps_sph_amd import ic as IC
     t0 = time.perf_counter()
-    if name in ("impact1m", "theta-sweep"):
+    if name in ("impact1m", "theta-sweep", "functor1m"):
         n = n or 1_000_000
         nt = int(round(0.9 * n))
         ic = IC.iron_dunite_impact(nt, n - nt)
@@ -359,6 +363,75 @@ def copy_calibration(nbytes=1 << 30, reps=5):
     print(json.dumps({"copy_calibration": {"bytes_read": nbytes, "bytes_written": nbytes, "launches": reps}}))
 
 
+def functor_pass(precisions, tables, ic, warmup, steps, theta):
+    """Density pass through sphgpu_dens_dispatch / _retrieve, as FDPS's
+    calcForceAllAndWriteBackMultiWalk drives it (src/main.cpp:139, multiwalk
+    form): the walks are the device driver's own exact gather lists of the
+    initial state, copied into host arrays of the reference's EPI/EPJ layout
+    before the clock starts."""
+    import ctypes as C
+    from fdps_sph_amd import Context, Simulation
+    from fdps_sph_amd import structs as S
+    from fdps_sph_amd._lib import check, lib
+    out = {}
+    for prec in precisions:
+        ctx = Context(0, prec)
+        load_tables(ctx, tables)
+        sim = Simulation(ctx, ic, theta=theta)
+        pos, m, h = sim.get("POS"), sim.get("MASS"), sim.get("SMTH")
+        L = sim.lists(0)
+        gs, joff, jidx = L["group_start"], L["j_off"], L["j_idx"]
+        G = len(gs) - 1
+        # the library's groups and lists, as host arrays of the boundary layout
+        epi = np.zeros(len(m), S.EPI_DENS)
+        epi["pos"], epi["mass"], epi["smth"], epi["id"] = pos, m, h, np.arange(len(m))
+        epj = np.zeros(len(jidx), S.EPJ_DENS)
+        epj["mass"], epj["pos"], epj["smth"] = m[jidx], pos[jidx], h[jidx]
+        res = np.zeros(len(m), S.RES_DENS)
+        ni = np.diff(gs).astype(np.int32)
+        nj = np.diff(joff).astype(np.int32)
+        pi = (C.c_void_p * G)(*[epi.ctypes.data + int(gs[g]) * epi.itemsize for g in range(G)])
+        pj = (C.c_void_p * G)(*[epj.ctypes.data + int(joff[g]) * epj.itemsize for g in range(G)])
+        pr = (C.c_void_p * G)(*[res.ctypes.data + int(gs[g]) * res.itemsize for g in range(G)])
+        LB = lib()
+        W = 200  # walks per call (FDPS n_walk_limit)
+        batches = [(b, min(G, b + W)) for b in range(0, G, W)]
+        sz = C.sizeof(C.c_void_p)
+
+        def at(arr, k):  # pointer to element k of a pointer array
+            return C.cast(C.addressof(arr) + k * sz, C.POINTER(C.c_void_p))
+
+        def one_pass():
+            pend = None
+            for k, (b, e) in enumerate(batches):
+                tag = k & 1
+                check(LB.sphgpu_dens_dispatch(ctx.handle, tag, e - b, at(pi, b),
+                                              ni[b:e].ctypes.data_as(C.POINTER(C.c_int32)), at(pj, b),
+                                              nj[b:e].ctypes.data_as(C.POINTER(C.c_int32))))
+                if pend is not None:
+                    pt, pb, pe = pend
+                    check(LB.sphgpu_dens_retrieve(ctx.handle, pt, pe - pb, ni[pb:pe].ctypes.data_as(
+                        C.POINTER(C.c_int32)), at(pr, pb)))
+                pend = (tag, b, e)
+            pt, pb, pe = pend
+            check(LB.sphgpu_dens_retrieve(ctx.handle, pt, pe - pb, ni[pb:pe].ctypes.data_as(C.POINTER(C.c_int32)),
+                                          at(pr, pb)))
+        for _ in range(warmup):
+            one_pass()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            one_pass()
+        sec = (time.perf_counter() - t0) / steps
+        host_bytes = epi.nbytes + epj.nbytes + res.nbytes
+        log(f"[functor {prec}] density pass over {G:,} walks, {len(jidx):,} j entries: {1e3 * sec:.2f} ms "
+            f"({host_bytes / sec / 1e9:.1f} GB/s of boundary arrays)")
+        out[prec] = {"ms_per_pass": 1e3 * sec, "walks": G, "j_entries": int(len(jidx)), "boundary_bytes": host_bytes,
+                     "boundary_GBps": host_bytes / sec / 1e9, "walks_per_call": W,
+                     "results_finite": bool(np.all(np.isfinite(res["dens"])) and np.all(res["dens"] >= 5.0))}
+        ctx.close()
+    return out
+
+
 # ------------------------------------------------------------------ main
 def main():
     ap = argparse.ArgumentParser()
@@ -394,6 +467,17 @@ def main():
 
     from fdps_sph_amd import Context
     ic, desc, tables = make_workload(args.workload, args.n)
+    if args.workload == "functor1m":
+        precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+        fr = functor_pass(precs, tables, ic, warmup, steps, args.theta)
+        main_p = precs[0]
+        out = {"metric": METRIC, "value": None, "unit": "steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+               "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "f64" if main_p == "fp64" else "f32", "data": "synthetic (seeded lattice bodies)",
+               "config": {"workload": f"functor1m: density pass of {desc} through the multiwalk boundary"},
+               "roofline": None, "functor_density_pass": fr}
+        print(json.dumps(out), flush=True)
+        return
     n = len(ic["mass"])
     nsys = int(ic["sys"].max()) + 1 if "sys" in ic else 1
     precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
