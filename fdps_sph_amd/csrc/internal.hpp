@@ -134,10 +134,10 @@ struct DrvtArgs {
 
 // The j side of the hydro pass as one 128-byte record per particle (one
 // cache line per list entry instead of 13 gathers from separate arrays):
-// A = f P / rho^2, H = 2.5 h and the kernel normalisation are formed by the
-// same FP64 operations the kernel would apply to the separate fields.
+// A = f P / rho^2, H = 2.5 h, 1/H and the kernel normalisation are formed by
+// the same FP64 operations the kernel would apply to the separate fields.
 struct alignas(16) HydroRec {
-    double x, y, z, vx, vy, vz, m, rho, cs, bal, gs, A, h, H, norm, pad;
+    double x, y, z, vx, vy, vz, m, rho, cs, bal, gs, A, h, H, norm, invH;
 };
 static_assert(sizeof(HydroRec) == 128, "one cache line per record");
 

@@ -27,13 +27,24 @@ __device__ __forceinline__ double W_(double s, double norm) {
     return r;
 }
 
+// Reciprocal of a positive, finite x: the hardware estimate refined by two
+// Newton steps (five instructions against ten for the IEEE divide; the
+// reference's -ffast-math build also divides through reciprocals).
+__device__ __forceinline__ double rcp_(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
 // WendlandC6::gradW scalar factor: gradW = dr * f (src/kernel.h:22-34); the
 // vector division of FDPS is a multiplication by the reciprocal.
 __device__ __forceinline__ double gradW_factor_(double s, double r, double H, double h, double norm, double &inv) {
     double rv = pow7_(plus_(1.0 - s)) *
                 (plus_(1.0 - s) * (8.0 + s * (50.0 + s * (96.0))) - 8.0 * (1.0 + s * (8.0 + s * (25.0 + s * (32.0)))));
     rv *= norm;
-    inv = 1.0 / (r * H + 1.0e-6 * h);
+    inv = rcp_(r * H + 1.0e-6 * h);
     return rv;
 }
 
@@ -107,6 +118,7 @@ __global__ __launch_bounds__(kWave) void k_dens(DensArgs a) {
     const int ip = i0 + (act ? lane : 0);
     const double xi = a.xi[ip], yi = a.yi[ip], zi = a.zi[ip], hi = a.hi[ip], mi = a.mi[ip];
     const double H = kSupport * hi;
+    const double invH = 1.0 / H;
     const double norm = wnorm_(H);
     const Frame fr = group_frame(xi, yi, zi, act);
     const float oxi = (float) (xi - fr.cx), oyi = (float) (yi - fr.cy), ozi = (float) (zi - fr.cz);
@@ -138,7 +150,7 @@ __global__ __launch_bounds__(kWave) void k_dens(DensArgs a) {
             const double4 pj = sp[__builtin_ctzll(m)];
             const double dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
             const double r2 = dx * dx + dy * dy + dz * dz;
-            const double s = sqrt(r2) / H;
+            const double s = sqrt(r2) * invH;  // r / H with 1/H hoisted out of the j loop
             dens += pj.w * W_(s, norm);
             ngb += (plus_(1.0 - s) > 0) ? 1 : 0;
         }
@@ -165,6 +177,7 @@ __global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
     const double vxi = a.vxi[ip], vyi = a.vyi[ip], vzi = a.vzi[ip];
     const double hi = a.hi[ip], rhoi = a.rhoi[ip];
     const double H = kSupport * hi;
+    const double invH = 1.0 / H;
     const double norm = wnorm_(H);
     const Frame fr = group_frame(xi, yi, zi, act);
     const float oxi = (float) (xi - fr.cx), oyi = (float) (yi - fr.cy), ozi = (float) (zi - fr.cz);
@@ -209,7 +222,7 @@ __global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
             {
                 const double dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
                 const double r = sqrt(r2);
-                const double s = r / H;
+                const double s = r * invH;
                 double inv;
                 const double f = gradW_factor_(s, r, H, hi, norm, inv);
                 const double gwx = (drx * f) * inv, gwy = (dry * f) * inv, gwz = (drz * f) * inv;
@@ -263,7 +276,7 @@ __device__ __forceinline__ HydroRec make_hydro_rec(double x, double y, double z,
     r.h = h;
     r.H = kSupport * h;
     r.norm = wnorm_(r.H);
-    r.pad = 0.0;
+    r.invH = 1.0 / r.H;
     return r;
 }
 
@@ -295,6 +308,7 @@ __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
     const double hi = a.hi[ip], rhoi = a.rhoi[ip], pi = a.pi[ip], csi = a.csi[ip], gsi = a.gsi[ip],
                  bali = a.bali[ip];
     const double Hi = kSupport * hi;
+    const double invHi = 1.0 / Hi;
     const double normi = wnorm_(Hi);
     const double Ai = gsi * pi / (rhoi * rhoi);
     const int64_t lb = list_begin(a.L, g);
@@ -353,16 +367,16 @@ __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
             const double dvx = vxi - J.vx, dvy = vyi - J.vy, dvz = vzi - J.vz;
             const double dvdr = dvx * drx + dvy * dry + dvz * drz;
             const double r2 = drx * drx + dry * dry + drz * drz;
-            const double w = (dvdr < 0) ? dvdr / sqrt(r2) : 0;
+            const double r = sqrt(r2);
+            const double w = (dvdr < 0) ? dvdr * rcp_(r) : 0;
             const double v_sig = csi + J.cs - 3.0 * w;
             {
                 const double rhoj = J.rho;
-                const double AV = -kAV * 0.5 * v_sig * w / (0.5 * (rhoi + rhoj)) * 0.5 * (bali + J.bal);
-                const double r = sqrt(r2);
+                const double AV = -kAV * 0.5 * v_sig * w * rcp_(0.5 * (rhoi + rhoj)) * 0.5 * (bali + J.bal);
                 double invi, invj;
-                const double fi = gradW_factor_(r / Hi, r, Hi, hi, normi, invi);
+                const double fi = gradW_factor_(r * invHi, r, Hi, hi, normi, invi);
                 const double Hj = J.H;
-                const double fj = gradW_factor_(r / Hj, r, Hj, J.h, J.norm, invj);
+                const double fj = gradW_factor_(r * J.invH, r, Hj, J.h, J.norm, invj);
                 const double gix = (drx * fi) * invi, giy = (dry * fi) * invi, giz = (drz * fi) * invi;
                 const double gjx = (drx * fj) * invj, gjy = (dry * fj) * invj, gjz = (drz * fj) * invj;
                 const double gsj = J.gs;
