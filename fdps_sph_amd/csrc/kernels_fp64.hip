@@ -318,31 +318,45 @@ __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
     const float padi = pad_(oxi, oyi, ozi);
     const float cuti = (float) Hi * 1.0001f + padi;
     const float padi_max = wave_maxf_(act ? padi : 0.f);
+    // v_sig bound of pass 1: with v0 the centre of the group's velocity box,
+    // c_i + c_j - 3w <= c_i + c_j + 3|v_i - v_j| <= (c_i + 3|v_i - v0|) + (c_j + 3|v_j - v0|)
+    const double cvx = 0.5 * (wave_min_(act ? vxi : INFINITY) + wave_max_(act ? vxi : -INFINITY));
+    const double cvy = 0.5 * (wave_min_(act ? vyi : INFINITY) + wave_max_(act ? vyi : -INFINITY));
+    const double cvz = 0.5 * (wave_min_(act ? vzi : INFINITY) + wave_max_(act ? vzi : -INFINITY));
+    const double bi = csi + 3.0 * sqrt((vxi - cvx) * (vxi - cvx) + (vyi - cvy) * (vyi - cvy) + (vzi - cvz) * (vzi - cvz));
     double ax = 0.0, ay = 0.0, az = 0.0, du = 0.0, vmax = 0.0;
     __shared__ HydroJ sj[kWave];
     __shared__ float4 so[kWave];  // FP32 offsets, cut2 of j
     for (int t = 0; t < tl.T; ++t) {
         const int k = tl.entry(t, lane);
         float padj = 0.f;
+        double bj = 0.0;
         if (k < tl.nj) {
             const int q = a.L.idx[lb + k];
             const HydroJ r = a.jrec ? a.jrec[q]
                                     : make_hydro_rec(a.xj[q], a.yj[q], a.zj[q], a.vxj[q], a.vyj[q], a.vzj[q], a.mj[q],
                                                      a.rhoj[q], a.csj[q], a.balj[q], a.gsj[q], a.pj[q], a.hj[q]);
             sj[lane] = r;
+            const double ux = r.vx - cvx, uy = r.vy - cvy, uz = r.vz - cvz;
+            bj = r.cs + 3.0 * sqrt(ux * ux + uy * uy + uz * uz);
             const float ox = (float) (r.x - fr.cx), oy = (float) (r.y - fr.cy), oz = (float) (r.z - fr.cz);
             padj = pad_(ox, oy, oz);
             const float cutj = (float) r.H * 1.0001f + padj + padi_max;
             so[lane] = make_float4(ox, oy, oz, cutj * cutj);
         }
         const float cut = cuti + wave_maxf_(padj);
+        // the tile can raise no lane's maximum when the bound, with a margin far
+        // above the rounding of either side, stays below it: pass 1 is skipped
+        const double bmax = wave_max_(bj);
+        const bool raise = act && !(bi + bmax < vmax * (1.0 - 1.0e-10));
         __syncthreads();
         const int nt = tl.count(t);
+        const int n1 = __ballot(raise) ? nt : 0;
         // pass 1, every j: v_sig_max.  v_sig = (c_i + c_j) - 3w with |w| =
         // |dv.dr|/|dr| <= |dv|; when even that bound stays below the current
         // maximum (with a margin far above the rounding of either side) the
         // square root and division are skipped -- the maximum cannot change.
-        for (int jj = 0; jj < nt; ++jj) {
+        for (int jj = 0; jj < n1; ++jj) {
             const double drx = xi - sj[jj].x, dry = yi - sj[jj].y, drz = zi - sj[jj].z;
             const double dvx = vxi - sj[jj].vx, dvy = vyi - sj[jj].vy, dvz = vzi - sj[jj].vz;
             const double dvdr = dvx * drx + dvy * dry + dvz * drz;

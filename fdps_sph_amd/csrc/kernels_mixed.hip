@@ -46,6 +46,11 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 // centre of the box spanned by the active lanes' values
+__device__ __forceinline__ float wave_maxf(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
+    return v;
+}
 __device__ __forceinline__ double wave_centre(double v, bool act) {
     const double lo = wave_min(act ? v : INFINITY);
     const double hi = wave_max(act ? v : -INFINITY);
@@ -265,6 +270,8 @@ __global__ __launch_bounds__(kWave) void k_hydro_mixed(HydroArgs a) {
     const float rhoi = (float) rhoid, csi = (float) a.csi[ip], bali = (float) a.bali[ip];
     const float fAi = (float) (a.gsi[ip] * a.pi[ip] / (rhoid * rhoid));  // f_i P_i / rho_i^2
     const float gsi = (float) a.gsi[ip];
+    // v_sig bound: c_i + c_j - 3w <= (c_i + 3|v_i - v0|) + (c_j + 3|v_j - v0|), v0 the velocity centre
+    const float bi = csi + 3.f * sqrtf(vxi * vxi + vyi * vyi + vzi * vzi);
     Kahan ax, ay, az, du;
     float vmax = 0.f;
     const int64_t lb = list_begin(a.L, g);
@@ -275,6 +282,7 @@ __global__ __launch_bounds__(kWave) void k_hydro_mixed(HydroArgs a) {
     const Tiles tl(nj);
     for (int t = 0; t < tl.T; ++t) {
         const int k = tl.entry(t, lane);
+        float bnd = 0.f;
         if (k < nj) {
             const int q = a.L.idx[lb + k];
             double x, y, z, vx, vy, vz, m, rj, cj, bj, gj, Aj, hj, Hj, nj3;
@@ -295,6 +303,7 @@ __global__ __launch_bounds__(kWave) void k_hydro_mixed(HydroArgs a) {
             svx[lane] = (float) (vx - cvx);
             svy[lane] = (float) (vy - cvy);
             svz[lane] = (float) (vz - cvz);
+            bnd = (float) cj + 3.f * sqrtf(svx[lane] * svx[lane] + svy[lane] * svy[lane] + svz[lane] * svz[lane]);
             sm[lane] = (float) m;
             srho[lane] = (float) rj;
             scs[lane] = (float) cj;
@@ -307,19 +316,31 @@ __global__ __launch_bounds__(kWave) void k_hydro_mixed(HydroArgs a) {
             snorm[lane] = (float) nj3;
             scut[lane] = (float) (Hj * Hj) * 1.0001f;
         }
+        // the tile can raise no lane's v_sig maximum when the bound, with a
+        // margin far above the FP32 rounding, stays below it
+        const float bmax = wave_maxf(bnd);
+        const bool raise = act && !(bi + bmax < vmax * (1.f - 1.0e-5f));
         __syncthreads();
         const int nt = tl.count(t);
         float px = 0.f, py = 0.f, pz = 0.f, pu = 0.f;
         // pass 1, every j: v_sig_max and the mask of pairs inside either support
         uint64_t mask = 0;
-        for (int jj = 0; jj < nt; ++jj) {
-            const float drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
-            const float dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
-            const float dvdr = dvx * drx + dvy * dry + dvz * drz;
-            const float r2 = drx * drx + dry * dry + drz * drz;
-            const float w = (dvdr < 0.f) ? dvdr * rsq(r2 + 1e-30f) : 0.f;
-            vmax = fmaxf(vmax, csi + scs[jj] - 3.f * w);
-            mask |= (r2 < rcut2i || (act && r2 < scut[jj])) ? (1ULL << jj) : 0ULL;
+        if (__ballot(raise)) {
+            for (int jj = 0; jj < nt; ++jj) {
+                const float drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
+                const float dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
+                const float dvdr = dvx * drx + dvy * dry + dvz * drz;
+                const float r2 = drx * drx + dry * dry + drz * drz;
+                const float w = (dvdr < 0.f) ? dvdr * rsq(r2 + 1e-30f) : 0.f;
+                vmax = fmaxf(vmax, csi + scs[jj] - 3.f * w);
+                mask |= (r2 < rcut2i || (act && r2 < scut[jj])) ? (1ULL << jj) : 0ULL;
+            }
+        } else {
+            for (int jj = 0; jj < nt; ++jj) {
+                const float drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
+                const float r2 = drx * drx + dry * dry + drz * drz;
+                mask |= (r2 < rcut2i || (act && r2 < scut[jj])) ? (1ULL << jj) : 0ULL;
+            }
         }
         // pass 2: force terms of this lane's in-support pairs
         for (uint64_t msk = mask; msk; msk &= msk - 1) {
