@@ -134,10 +134,11 @@ struct DrvtArgs {
 
 // The j side of the hydro pass as one 128-byte record per particle (one
 // cache line per list entry instead of 13 gathers from separate arrays):
-// A = f P / rho^2, H = 2.5 h, 1/H and the kernel normalisation are formed by
-// the same FP64 operations the kernel would apply to the separate fields.
+// A = f P / rho^2, H = 2.5 h, 1/H, the kernel normalisation and the gradient's
+// regularisation reg = 1e-6 h are formed by the same FP64 operations the
+// kernel would apply to the separate fields.
 struct alignas(16) HydroRec {
-    double x, y, z, vx, vy, vz, m, rho, cs, bal, gs, A, h, H, norm, invH;
+    double x, y, z, vx, vy, vz, m, rho, cs, bal, gs, A, reg, H, norm, invH;
 };
 static_assert(sizeof(HydroRec) == 128, "one cache line per record");
 

@@ -3,29 +3,36 @@
 // One 64-lane wavefront per i-group: lane l owns i-particle gstart[g]+l, and
 // the group's j-list is streamed through LDS in tiles of 64 (each lane
 // gathers one j, then every lane reads the tile back as an LDS broadcast).
-// Every pair expression keeps the operand order of src/force.h /
-// src/kernel.h; this file is compiled with -ffp-contract=off so no
-// multiply-add is fused.  Each lane sums its pairs sequentially in tile order
-// (see Tiles).  Pairs outside the support are skipped: there W and gradW are
-// exactly zero, so the skip changes no sum.
+// The pair terms are those of src/force.h / src/kernel.h, evaluated in FP64
+// with the algebra rearranged for instruction count: the kernel normalisation
+// and constant factors come out of the sums, both gradients of a hydro pair are
+// multiples of dr, |dr| comes from a refined reciprocal square root, and
+// polynomials and dot products are fused multiply-adds (written out with
+// fma(): the file is compiled with -ffp-contract=off, so nothing else is
+// fused).  Every change moves results by a few units in the last place of the
+// terms.  Each lane sums its pairs sequentially in tile order (see Tiles).
+// Pairs outside the support are skipped: there W and gradW are exactly zero,
+// so the skip changes no sum.
 #include "internal.hpp"
 
 namespace sphgpu {
 namespace {
 
-__device__ __forceinline__ double plus_(double a) { return (a > 0) ? a : 0; }
-__device__ __forceinline__ double pow7_(double a) { const double a3 = a * a * a; return a3 * a3 * a; }
 __device__ __forceinline__ double pow8_(double a) { const double a2 = a * a; const double a4 = a2 * a2; return a4 * a4; }
 
 // WendlandC6 normalisation (1365/64)/(H^3 pi) (src/kernel.h:16)
 __device__ __forceinline__ double wnorm_(double H) { return (1365. / 64.) / (H * H * H * kPi); }
 
-// WendlandC6::W(r) with s = r/H (src/kernel.h:8-19)
-__device__ __forceinline__ double W_(double s, double norm) {
-    double r = (1.0 + s * (8.0 + s * (25.0 + s * (32.0)))) * pow8_(plus_(1.0 - s));
-    r *= norm;
-    return r;
-}
+// WendlandC6::W(r) with s = r/H (src/kernel.h:8-19) is norm * p(s) (1-s)_+^8,
+// p(s) = 1 + 8s + 25s^2 + 32s^3.
+__device__ __forceinline__ double wpoly_(double s) { return fma(fma(fma(32.0, s, 25.0), s, 8.0), s, 1.0); }
+
+// WendlandC6::gradW (src/kernel.h:22-34) is dr * norm * dW/ds / (r H + 1e-6 h)
+// with dW/ds = (1-s)^7 [(1-s)(8 + 50s + 96s^2) - 8 p(s)]; the bracket expands
+// to -22 s (1 + 7s + 16s^2) (its constant terms cancel exactly).  Returned
+// without the factor -22 norm, from o7 = (1-s)_+^7.
+__device__ __forceinline__ double dwds_(double s, double o7) { return (s * fma(fma(16.0, s, 7.0), s, 1.0)) * o7; }
+__device__ __forceinline__ double pow7f_(double o) { const double o2 = o * o, o4 = o2 * o2; return (o4 * o2) * o; }
 
 // Reciprocal of a positive, finite x: the hardware estimate refined by two
 // Newton steps (five instructions against ten for the IEEE divide; the
@@ -38,15 +45,25 @@ __device__ __forceinline__ double rcp_(double x) {
     return fma(y, e, y);
 }
 
-// WendlandC6::gradW scalar factor: gradW = dr * f (src/kernel.h:22-34); the
-// vector division of FDPS is a multiplication by the reciprocal.
-__device__ __forceinline__ double gradW_factor_(double s, double r, double H, double h, double norm, double &inv) {
-    double rv = pow7_(plus_(1.0 - s)) *
-                (plus_(1.0 - s) * (8.0 + s * (50.0 + s * (96.0))) - 8.0 * (1.0 + s * (8.0 + s * (25.0 + s * (32.0)))));
-    rv *= norm;
-    inv = rcp_(r * H + 1.0e-6 * h);
-    return rv;
+// 1/sqrt(x) of a positive, finite x: the hardware estimate y0 refined by one
+// step of the series (1 - e)^(-1/2) = 1 + e/2 + 3e^2/8 + O(e^3), e = 1 - x y0^2
+// (Householder's method of order 2): six instructions against eighteen for
+// the IEEE square root, which is followed by a divide besides.
+__device__ __forceinline__ double rsqrt_(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(y * e, fma(0.375, e, 0.5), y);
 }
+// |dr| and 1/|dr| from r2 = |dr|^2 >= 0.  The 1e-300, far below the rounding
+// of any r2 of a physical distance, keeps r2 = 0 finite: |dr| = 0 exactly.
+struct Dist {
+    double r, rinv;
+};
+__device__ __forceinline__ Dist dist_(double r2) {
+    const double rinv = rsqrt_(r2 + 1.0e-300);
+    return {r2 * rinv, rinv};
+}
+__device__ __forceinline__ double norm2_(double x, double y, double z) { return fma(z, z, fma(y, y, x * x)); }
 
 __device__ __forceinline__ double wave_min_(double v) {
 #pragma unroll
@@ -149,15 +166,15 @@ __global__ __launch_bounds__(kWave) void k_dens(DensArgs a) {
         for (uint64_t m = mask; m; m &= m - 1) {
             const double4 pj = sp[__builtin_ctzll(m)];
             const double dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            const double s = sqrt(r2) * invH;  // r / H with 1/H hoisted out of the j loop
-            dens += pj.w * W_(s, norm);
-            ngb += (plus_(1.0 - s) > 0) ? 1 : 0;
+            const double s = dist_(norm2_(dx, dy, dz)).r * invH;  // r / H with 1/H hoisted out of the j loop
+            const double o = fmax(1.0 - s, 0.0);
+            dens = fma(pj.w, wpoly_(s) * pow8_(o), dens);  // sum of m_j W_ij / norm
+            ngb += (o > 0.0) ? 1 : 0;
         }
         __syncthreads();
     }
     if (act) {
-        dens = dmax(5.0, dens);  // FLAG_GI floor (src/force.h:18-20)
+        dens = dmax(5.0, dens * norm);  // FLAG_GI floor (src/force.h:18-20)
         a.rho_out[ip] = dens;
         a.h_out[ip] = kSMTH * pow(mi / dens, 1.0 / 3.0);
         if (a.ngb_out) a.ngb_out[ip] = ngb;
@@ -184,7 +201,9 @@ __global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
     const float Hc = (float) H * 1.0001f;
     const float wi = fmaf(oxi, oxi, fmaf(oyi, oyi, ozi * ozi)), n1i = norm1_(oxi, oyi, ozi);
     const float tx = -2.f * oxi, ty = -2.f * oyi, tz = -2.f * ozi;
-    double divv = 0.0, rx = 0.0, ry = 0.0, rz = 0.0, gs = 0.0;
+    const double reg = 1.0e-6 * hi;
+    // the sums leave out the factor K = -22 norm of gradW
+    double divv = 0.0, rx = 0.0, ry = 0.0, rz = 0.0, gsw = 0.0, gsf = 0.0;
     const int64_t lb = list_begin(a.L, g);
     const Tiles tl(a.L.cnt[g]);
     __shared__ double4 sp[kWave];  // x, y, z, m
@@ -218,28 +237,32 @@ __global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
             const double4 pj = sp[jj];
             const double4 vj = sv[jj];
             const double drx = xi - pj.x, dry = yi - pj.y, drz = zi - pj.z;
-            const double r2 = drx * drx + dry * dry + drz * drz;
-            {
-                const double dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
-                const double r = sqrt(r2);
-                const double s = r * invH;
-                double inv;
-                const double f = gradW_factor_(s, r, H, hi, norm, inv);
-                const double gwx = (drx * f) * inv, gwy = (dry * f) * inv, gwz = (drz * f) * inv;
-                const double w = W_(s, norm);
-                const double mj = pj.w;
-                const double mneg = -mj;
-                const double ax = mneg * dvx, ay = mneg * dvy, az = mneg * dvz;
-                divv += ax * gwx + ay * gwy + az * gwz;
-                rx += ay * gwz - az * gwy;
-                ry += az * gwx - ax * gwz;
-                rz += ax * gwy - ay * gwx;
-                gs -= mj / hi * (3.0 * w + (drx * gwx + dry * gwy + drz * gwz));
-            }
+            const double dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
+            const double r2 = norm2_(drx, dry, drz);
+            const double r = dist_(r2).r;
+            const double s = r * invH;
+            const double o = fmax(1.0 - s, 0.0);
+            const double o7 = pow7f_(o);
+            const double mf = pj.w * (dwds_(s, o7) * rcp_(fma(r, H, reg)));  // m_j gradW / (K dr)
+            const double dvdr = fma(dvz, drz, fma(dvy, dry, dvx * drx));
+            divv = fma(mf, dvdr, divv);                     // -divv / K: sum of m_j dv . gradW / K
+            rx = fma(mf, fma(dvy, drz, -(dvz * dry)), rx);  // -rot / K: sum of m_j dv x gradW / K
+            ry = fma(mf, fma(dvz, drx, -(dvx * drz)), ry);
+            rz = fma(mf, fma(dvx, dry, -(dvy * drx)), rz);
+            gsw = fma(pj.w, wpoly_(s) * (o7 * o), gsw);  // sum of m_j W / norm
+            gsf = fma(mf, r2, gsf);                       // sum of m_j dr . gradW / K
         }
         __syncthreads();
     }
     if (act) {
+        // divv = -sum m_j dv.gradW, rot = -sum m_j dv x gradW,
+        // gs = -sum m_j / h (3 W + dr.gradW)  (src/force.h:95-101)
+        const double K = -22.0 * norm;
+        divv *= -K;
+        rx *= -K;
+        ry *= -K;
+        rz *= -K;
+        double gs = -(3.0 * norm * gsw + K * gsf) / hi;
         gs = 1.0 / (1.0 + hi * gs / (3.0 * rhoi));
         divv /= rhoi;
         const double inv = 1.0 / rhoi;
@@ -273,7 +296,7 @@ __device__ __forceinline__ HydroRec make_hydro_rec(double x, double y, double z,
     r.bal = bal;
     r.gs = gs;
     r.A = gs * p / (rho * rho);
-    r.h = h;
+    r.reg = 1.0e-6 * h;
     r.H = kSupport * h;
     r.norm = wnorm_(r.H);
     r.invH = 1.0 / r.H;
@@ -310,6 +333,7 @@ __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
     const double Hi = kSupport * hi;
     const double invHi = 1.0 / Hi;
     const double normi = wnorm_(Hi);
+    const double regi = 1.0e-6 * hi;
     const double Ai = gsi * pi / (rhoi * rhoi);
     const int64_t lb = list_begin(a.L, g);
     const Tiles tl(a.L.cnt[g]);
@@ -365,53 +389,50 @@ __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
                 const double room = vmax - cij;
                 const double dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
                 if (!(room > 1.0e-4 * vmax && 9.0 * dv2 < (room * room) * (1.0 - 1.0e-10))) {
-                    const double r2 = drx * drx + dry * dry + drz * drz;
-                    const double w = dvdr / sqrt(r2);
-                    vmax = dmax(vmax, cij - 3.0 * w);
+                    const double w = dvdr * rsqrt_(norm2_(drx, dry, drz));  // dv.dr < 0: r2 > 0
+                    vmax = dmax(vmax, fma(-3.0, w, cij));
                 }
             } else {
                 vmax = dmax(vmax, cij - 3.0 * 0.0);
             }
         }
-        // pass 2: the force terms of the pairs inside either support
+        // pass 2: the force terms of the pairs inside either support.  Both
+        // gradients are multiples of dr: gradW_i = K F_i dr with K = -22 and
+        // F_i = norm_i s(1 + 7s + 16s^2)(1-s)^7 / (r H_i + 1e-6 h_i); the two
+        // reciprocals come from one, 1/(a_i a_j).  AVh is half the viscosity
+        // AV of the reference, and the sums leave out K.
         const uint64_t mask = support_mask_sym_(so, nt, oxi, oyi, ozi, act ? cut * cut : -1.f, act);
         for (uint64_t msk = mask; msk; msk &= msk - 1) {
             const HydroJ &J = sj[__builtin_ctzll(msk)];
             const double drx = xi - J.x, dry = yi - J.y, drz = zi - J.z;
             const double dvx = vxi - J.vx, dvy = vyi - J.vy, dvz = vzi - J.vz;
-            const double dvdr = dvx * drx + dvy * dry + dvz * drz;
-            const double r2 = drx * drx + dry * dry + drz * drz;
-            const double r = sqrt(r2);
-            const double w = (dvdr < 0) ? dvdr * rcp_(r) : 0;
-            const double v_sig = csi + J.cs - 3.0 * w;
-            {
-                const double rhoj = J.rho;
-                const double AV = -kAV * 0.5 * v_sig * w * rcp_(0.5 * (rhoi + rhoj)) * 0.5 * (bali + J.bal);
-                double invi, invj;
-                const double fi = gradW_factor_(r * invHi, r, Hi, hi, normi, invi);
-                const double Hj = J.H;
-                const double fj = gradW_factor_(r * J.invH, r, Hj, J.h, J.norm, invj);
-                const double gix = (drx * fi) * invi, giy = (dry * fi) * invi, giz = (drz * fi) * invi;
-                const double gjx = (drx * fj) * invj, gjy = (dry * fj) * invj, gjz = (drz * fj) * invj;
-                const double gsj = J.gs;
-                const double gx = 0.5 * (gix * gsi + gjx * gsj);
-                const double gy = 0.5 * (giy * gsi + gjy * gsj);
-                const double gz = 0.5 * (giz * gsi + gjz * gsj);
-                const double mj = J.m, Aj = J.A;
-                ax -= mj * (Ai * gix + Aj * gjx + AV * gx);
-                ay -= mj * (Ai * giy + Aj * gjy + AV * gy);
-                az -= mj * (Ai * giz + Aj * gjz + AV * gz);
-                const double sc = mj * (Ai + 0.5 * AV);
-                du += (sc * dvx) * gx + (sc * dvy) * gy + (sc * dvz) * gz;
-            }
+            const double dvdr = fma(dvz, drz, fma(dvy, dry, dvx * drx));
+            const Dist d = dist_(norm2_(drx, dry, drz));
+            const double w = fmin(dvdr, 0.0) * d.rinv;  // dv.dr / r when negative, else 0
+            const double v_sig = fma(-3.0, w, csi + J.cs);
+            const double AVh = ((-0.25 * kAV) * (v_sig * w)) * ((bali + J.bal) * rcp_(rhoi + J.rho));
+            const double si = d.r * invHi, sj = d.r * J.invH;
+            const double ai = fma(d.r, Hi, regi), aj = fma(d.r, J.H, J.reg);
+            const double rab = rcp_(ai * aj);
+            const double Fi = (normi * dwds_(si, pow7f_(fmax(1.0 - si, 0.0)))) * (aj * rab);
+            const double Fj = (J.norm * dwds_(sj, pow7f_(fmax(1.0 - sj, 0.0)))) * (ai * rab);
+            const double Fm = fma(Fj, J.gs, Fi * gsi);  // 2 (gradW_i gs_i + gradW_j gs_j) / 2 / (K dr)
+            const double mj = J.m;
+            const double C = mj * fma(AVh, Fm, fma(J.A, Fj, Ai * Fi));
+            ax = fma(C, drx, ax);
+            ay = fma(C, dry, ay);
+            az = fma(C, drz, az);
+            du = fma((mj * (Ai + AVh)) * Fm, dvdr, du);
         }
         __syncthreads();
     }
     if (act) {
-        a.ax[ip] = ax;
-        a.ay[ip] = ay;
-        a.az[ip] = az;
-        a.du[ip] = du;
+        // acc = -sum m_j (A_i gradW_i + A_j gradW_j + AV gradW_m) = 22 sum C dr,
+        // du = sum m_j (A_i + AV/2) dv . gradW_m = -11 sum ...  (src/force.h:126-145)
+        a.ax[ip] = 22.0 * ax;
+        a.ay[ip] = 22.0 * ay;
+        a.az[ip] = 22.0 * az;
+        a.du[ip] = -11.0 * du;
         a.dt[ip] = kCFL * 2.0 * hi / vmax;
     }
 }
@@ -419,25 +440,22 @@ __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
 // ---------------------------------------------------------------- gravity
 // CalcGravityForce<EPJ::Grav> then CalcGravityForce<SPJMonopole>
 // (src/force.h:154-168): the particle list first, then the monopoles.  The
-// reference's 1.0 / sqrt(r^2 + eps^2) is the FP64 rsqrt of the math library
-// (hardware estimate refined in FP64, within an ulp or so of the quotient of
-// the correctly rounded square root: a third of the instructions of the
-// IEEE divide and square root), and the accumulations are fused
-// multiply-adds.  Both change results only at the rounding level.
+// reference's 1.0 / sqrt(r^2 + eps^2) is rsqrt_ (within an ulp or so of the
+// quotient of the correctly rounded square root), r^2 + eps^2 and the
+// accumulations are fused multiply-adds, and G m_j is formed once per j when
+// the tile is staged.  Each changes results at the rounding level only.
 __device__ __forceinline__ void grav_tile_(const double4 *sp, int nt, double xi, double yi, double zi, double eps2,
                                            double &ax, double &ay, double &az, double &pot) {
 #pragma unroll 2
     for (int jj = 0; jj < nt; ++jj) {
-        const double4 pj = sp[jj];  // x, y, z, m
+        const double4 pj = sp[jj];  // x, y, z, G m
         const double drx = xi - pj.x, dry = yi - pj.y, drz = zi - pj.z;
-        const double dr2 = drx * drx + dry * dry + drz * drz;
-        const double dr_inv = rsqrt(dr2 + eps2);
-        const double m_dr3_inv = pj.w * (dr_inv * dr_inv * dr_inv);
-        const double g3 = kG * m_dr3_inv;
+        const double dr_inv = rsqrt_(fma(drz, drz, fma(dry, dry, fma(drx, drx, eps2))));
+        const double g3 = pj.w * (dr_inv * dr_inv * dr_inv);
         ax = fma(-g3, drx, ax);
         ay = fma(-g3, dry, ay);
         az = fma(-g3, drz, az);
-        pot = fma(-(kG * pj.w), dr_inv, pot);
+        pot = fma(-pj.w, dr_inv, pot);
     }
 }
 
@@ -464,7 +482,7 @@ __global__ __launch_bounds__(kWave) void k_grav(GravArgs a) {
             const int k = t + lane;
             if (k < nj) {
                 const int q = L.idx[lb + k];
-                sp[lane] = make_double4(X[q], Y[q], Z[q], Mm[q]);
+                sp[lane] = make_double4(X[q], Y[q], Z[q], kG * Mm[q]);
             }
             __syncthreads();
             grav_tile_(sp, min(kWave, nj - t), xi, yi, zi, eps2, ax, ay, az, pot);

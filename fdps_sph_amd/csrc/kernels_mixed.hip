@@ -285,14 +285,16 @@ __global__ __launch_bounds__(kWave) void k_hydro_mixed(HydroArgs a) {
         float bnd = 0.f;
         if (k < nj) {
             const int q = a.L.idx[lb + k];
-            double x, y, z, vx, vy, vz, m, rj, cj, bj, gj, Aj, hj, Hj, nj3;
+            double x, y, z, vx, vy, vz, m, rj, cj, bj, gj, Aj, regj, Hj, nj3;
             if (a.jrec) {  // one 128-byte record per entry
                 const HydroRec r = a.jrec[q];
                 x = r.x; y = r.y; z = r.z; vx = r.vx; vy = r.vy; vz = r.vz; m = r.m;
-                rj = r.rho; cj = r.cs; bj = r.bal; gj = r.gs; Aj = r.A; hj = r.h; Hj = r.H; nj3 = r.norm;
+                rj = r.rho; cj = r.cs; bj = r.bal; gj = r.gs; Aj = r.A; regj = r.reg; Hj = r.H; nj3 = r.norm;
             } else {
                 x = a.xj[q]; y = a.yj[q]; z = a.zj[q]; vx = a.vxj[q]; vy = a.vyj[q]; vz = a.vzj[q]; m = a.mj[q];
-                rj = a.rhoj[q]; cj = a.csj[q]; bj = a.balj[q]; gj = a.gsj[q]; hj = a.hj[q];
+                rj = a.rhoj[q]; cj = a.csj[q]; bj = a.balj[q]; gj = a.gsj[q];
+                const double hj = a.hj[q];
+                regj = 1.0e-6 * hj;
                 Aj = gj * a.pj[q] / (rj * rj);
                 Hj = kSupport * hj;
                 nj3 = kNormD / (Hj * Hj * Hj);
@@ -312,7 +314,7 @@ __global__ __launch_bounds__(kWave) void k_hydro_mixed(HydroArgs a) {
             sA[lane] = (float) Aj;
             sH[lane] = (float) Hj;
             sinvH[lane] = (float) (1.0 / Hj);
-            sreg[lane] = (float) (1.0e-6 * hj);
+            sreg[lane] = (float) regj;
             snorm[lane] = (float) nj3;
             scut[lane] = (float) (Hj * Hj) * 1.0001f;
         }
