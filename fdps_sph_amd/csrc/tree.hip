@@ -467,13 +467,50 @@ __device__ __forceinline__ int expand_level(const ListArgs &a, int32_t *fa, int 
     return nn;
 }
 
+// emit the particles of one chunk of leaves (one per lane, sz particles each,
+// sz = 0 for none) that pass part_ok, in leaf order then index order
+template <class PartOk>
+__device__ __forceinline__ int emit_chunk(const ListArgs &a, int leaf, int sz, int32_t *out, int64_t cap, int count,
+                                          PartOk part_ok, int *s_pref, int *s_leaf) {
+    const int lane = threadIdx.x;
+    const uint64_t lt = (lane == 0) ? 0ULL : ((~0ULL) >> (64 - lane));
+    const int incl = wave_incl_scan(sz);
+    const int tot = __shfl(incl, kWave - 1, kWave);
+    s_pref[lane] = incl - sz;
+    s_leaf[lane] = leaf;
+    if (lane == kWave - 1) s_pref[kWave] = incl;
+    __syncthreads();
+    for (int s0 = 0; s0 < tot; s0 += kWave) {
+        const int slot = s0 + lane;
+        bool pass = false;
+        int p = 0;
+        if (slot < tot) {
+            int lo = 0, hi = kWave;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_pref[mid] <= slot) lo = mid;
+                else hi = mid;
+            }
+            p = a.nstart[s_leaf[lo]] + (slot - s_pref[lo]);
+            pass = part_ok(p);
+        }
+        const uint64_t m = __ballot(pass);
+        if (pass) {
+            const int pos = count + __popcll(m & lt);
+            if (pos < cap) out[pos] = p;
+        }
+        count += __popcll(m);
+    }
+    __syncthreads();
+    return count;
+}
+
 // emit the particles of the leaves fa[0..nf) that pass leaf_ok, filtered by
 // part_ok, in leaf order then index order
 template <class LeafOk, class PartOk>
 __device__ __forceinline__ int emit_particles(const ListArgs &a, const int32_t *fa, int nf, int32_t *out, int64_t cap,
                                               int count, LeafOk leaf_ok, PartOk part_ok, int *s_pref, int *s_leaf) {
     const int lane = threadIdx.x;
-    const uint64_t lt = (lane == 0) ? 0ULL : ((~0ULL) >> (64 - lane));
     for (int base = 0; base < nf; base += kWave) {
         const int idx = base + lane;
         int sz = 0, leaf = 0;
@@ -481,34 +518,7 @@ __device__ __forceinline__ int emit_particles(const ListArgs &a, const int32_t *
             leaf = fa[idx];
             if (leaf_ok(leaf)) sz = a.nend[leaf] - a.nstart[leaf];
         }
-        const int incl = wave_incl_scan(sz);
-        const int tot = __shfl(incl, kWave - 1, kWave);
-        s_pref[lane] = incl - sz;
-        s_leaf[lane] = leaf;
-        if (lane == kWave - 1) s_pref[kWave] = incl;
-        __syncthreads();
-        for (int s0 = 0; s0 < tot; s0 += kWave) {
-            const int slot = s0 + lane;
-            bool pass = false;
-            int p = 0;
-            if (slot < tot) {
-                int lo = 0, hi = kWave;
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (s_pref[mid] <= slot) lo = mid;
-                    else hi = mid;
-                }
-                p = a.nstart[s_leaf[lo]] + (slot - s_pref[lo]);
-                pass = part_ok(p);
-            }
-            const uint64_t m = __ballot(pass);
-            if (pass) {
-                const int pos = count + __popcll(m & lt);
-                if (pos < cap) out[pos] = p;
-            }
-            count += __popcll(m);
-        }
-        __syncthreads();
+        count = emit_chunk(a, leaf, sz, out, cap, count, part_ok, s_pref, s_leaf);
     }
     return count;
 }
@@ -564,25 +574,45 @@ __global__ __launch_bounds__(kWave) void k_list_grav(ListArgs a) {
         int nf = 1, cj = 0, cs = 0;
         int32_t *jout = a.out + (int64_t) g * a.cap;
         int32_t *sout = a.sout + (int64_t) g * a.scap;
+        auto all = [&](int) { return true; };
         for (int l = a.nlevels - 1; l >= 0; --l) {
-            // monopoles accepted at this level, in frontier order
+            // one pass over the frontier: the accepted nodes become monopoles
+            // (in frontier order), the others are opened -- their children
+            // form the next frontier, or at the leaf level their particles
+            // are emitted
+            int nn = 0;
             for (int base = 0; base < nf; base += kWave) {
                 const int idx = base + lane;
-                const bool acc = (idx < nf) && accept_node(a, fa[idx], b);
+                const int node = (idx < nf) ? fa[idx] : -1;
+                const bool acc = (node >= 0) && accept_node(a, node, b);
                 const uint64_t m = __ballot(acc);
                 if (acc) {
                     const int pos = cs + __popcll(m & lt);
-                    if (pos < a.scap) sout[pos] = fa[idx];
+                    if (pos < a.scap) sout[pos] = node;
                 }
                 cs += __popcll(m);
+                const bool open = (node >= 0) && !acc;
+                if (l >= 1) {
+                    int nc = 0, cfirst = 0;
+                    if (open) { cfirst = a.c0[node]; nc = a.c1[node] - cfirst; }
+                    const int incl = wave_incl_scan(nc);
+                    const int pos0 = nn + incl - nc;
+                    for (int c = 0; c < nc; ++c)
+                        if (pos0 + c < a.fcap) fb[pos0 + c] = cfirst + c;
+                    nn += __shfl(incl, kWave - 1, kWave);
+                } else {
+                    const int sz = open ? a.nend[node] - a.nstart[node] : 0;
+                    cj = emit_chunk(a, open ? node : 0, sz, jout, a.cap, cj, all, s_pref, s_leaf);
+                }
             }
-            auto open = [&](int node) { return !accept_node(a, node, b); };
             if (l >= 1) {
-                nf = expand_level(a, fa, nf, fb, open);
+                if (nn > a.fcap) {
+                    if (lane == 0) atomicOr(a.err, 4);
+                    nn = (int) a.fcap;
+                }
+                __syncthreads();
+                nf = nn;
                 int32_t *t = fa; fa = fb; fb = t;
-            } else {
-                auto all = [&](int) { return true; };
-                cj = emit_particles(a, fa, nf, jout, a.cap, cj, open, all, s_pref, s_leaf);
             }
         }
         if (lane == 0) {
