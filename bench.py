@@ -188,7 +188,9 @@ def measured_traffic(kernel, precision, workload):
 
 # ------------------------------------------------------------------ runs
 def run(ctx, ic, warmup, steps, label, **params):
-    """Simulation from `ic`: warm-up steps, then `steps` steps timed one by one."""
+    """Simulation from `ic`: warm-up steps, then `steps` steps timed as one call
+    (the library's stream drains at its end), then a diagnostics pass of a few
+    more steps timed one by one with the per-kernel statistics on."""
     from fdps_sph_amd import Simulation
     t0 = time.perf_counter()
     sim = Simulation(ctx, ic, **params)
@@ -200,24 +202,30 @@ def run(ctx, ic, warmup, steps, label, **params):
     for _ in range(warmup):
         sim.step(1)
         dts.append(float(np.min(sim.time()[0])))
-    sim.set_stats(True)
-    ms = []
     t_all = time.perf_counter()
-    for _ in range(steps):
-        t0 = time.perf_counter()
-        sim.step(1)  # returns after the context's stream has drained
-        ms.append(1e3 * (time.perf_counter() - t0))
+    sim.step(steps)  # returns after the context's stream has drained
     wall = time.perf_counter() - t_all
-    stats = sim.kernel_stats()
-    stages = sim.stage_ms()
     parts1 = energy_parts(sim)
     e1 = sum(parts1.values())
     dt, t = sim.time()
     gb, fe = sim.list_builds()
+    state = snapshot(sim)
+    # diagnostics pass: per-step wall times and per-kernel HIP-event statistics
+    nstat = min(steps, 20)
+    sim.set_stats(True)
+    ms = []
+    for _ in range(nstat):
+        t0 = time.perf_counter()
+        sim.step(1)
+        ms.append(1e3 * (time.perf_counter() - t0))
+    stats = sim.kernel_stats()
+    stages = sim.stage_ms()
     ms = np.array(ms)
-    log(f"[{label}] init {t_init:.2f} s; {steps} steps in {wall:.3f} s: mean {ms.mean():.2f} ms/step, "
-        f"median {np.median(ms):.2f}, min {ms.min():.2f}; dt {float(np.min(dt)):.4g} s, t {float(np.max(t)):.4g} s")
-    return {"sim": sim, "wall": wall, "ms_per_step": 1e3 * wall / steps, "median_ms": float(np.median(ms)),
+    log(f"[{label}] init {t_init:.2f} s; {steps} steps in {wall:.3f} s: {1e3 * wall / steps:.2f} ms/step; "
+        f"{nstat} more steps one by one with kernel statistics: median {np.median(ms):.2f}, min {ms.min():.2f}; "
+        f"dt {float(np.min(dt)):.4g} s, t {float(np.max(t)):.4g} s")
+    return {"sim": sim, "state": state, "wall": wall, "ms_per_step": 1e3 * wall / steps,
+            "median_ms": float(np.median(ms)),
             "steps_per_s": steps / wall, "stats": stats, "stages": stages, "init": init, "E0": e0, "E1": e1,
             "parts0": parts0, "parts1": parts1, "warmup_dts": dts, "gather_builds_per_force_eval": gb / max(fe, 1),
             "t": float(np.max(t)), "steps_total": warmup + steps}
@@ -269,7 +277,7 @@ def compare_precisions(ctx64, ctx32, ic, warmup, steps, **params):
     M = run(ctx32, ic, warmup, steps, "mixed", **params)
     err0 = force_errors(M["init"], F["init"])
     # one force evaluation of each precision from the FP64 state after the last step
-    state = snapshot(F["sim"])
+    state = F["state"]
     from fdps_sph_amd import Simulation
     fe = force_fields(Simulation(ctx64, state, **params))
     me = force_errors(force_fields(Simulation(ctx32, state, **params)), fe)
@@ -288,7 +296,7 @@ def precision_block(R, mixed, workload):
     roof = kernel_roofline(R["stats"], mixed)
     for k, v in roof.items():
         v["traffic"] = measured_traffic(k, "mixed" if mixed else "fp64", workload)
-    return {"steps_per_s": R["steps_per_s"], "ms_per_step": R["ms_per_step"], "median_ms_per_step": R["median_ms"],
+    return {"steps_per_s": R["steps_per_s"], "ms_per_step": R["ms_per_step"], "median_ms_per_step_stats_on": R["median_ms"],
             "gather_builds_per_force_eval": R["gather_builds_per_force_eval"],
             "kernels": {k: {kk: vv for kk, vv in v.items() if kk != "per_launch"} for k, v in roof.items()},
             "per_launch": {k: v["per_launch"] for k, v in roof.items()},

@@ -153,6 +153,8 @@ struct SimState {
     double kms[kStatKinds] = {0};
     int64_t klaunch[kStatKinds] = {0};
     DevBuf<long long> kcount;  // [kStatKinds][4]: n_i, n_j, n_s, pairs
+    DevBuf<long long> kcount_try;  // the same for the force evaluation in progress
+    DevBuf<double> hsave;      // h at the start of a force evaluation (for its redo)
     DevBuf<HydroRec> hrec;     // j records of the hydro pass
     DevBuf<double4> kpos, kvel; // j records of the density and derivative passes
     int64_t gather_builds = 0; // gather-list constructions since init
@@ -478,6 +480,15 @@ void retrieve(sphgpu_ctx *c, int kind, int tag, int n_walk, const int32_t n_epi[
 ListRef list_ref(const int32_t *idx, const int64_t *off, const int32_t *cnt) { return ListRef{idx, off, cnt, 0}; }
 
 // ------------------------------------------------------------------ simulation helpers
+__global__ void k_fill_ll(int64_t n, long long *a) {
+    const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] = 0;
+}
+__global__ void k_add_ll(int64_t n, const long long *src, long long *dst) {
+    const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] += src[k];
+}
+
 void sim_bind(SimState &S) {
     auto bind = [&](int k, PArrays &P) {
         double **fp[31] = {&P.x, &P.y, &P.z, &P.vx, &P.vy, &P.vz, &P.ax, &P.ay, &P.az, &P.gx, &P.gy,
@@ -493,6 +504,13 @@ void sim_bind(SimState &S) {
     bind(1, S.Palt);
 }
 
+// One force evaluation (src/main.cpp:125-146).  The list constructions are
+// enqueued without waiting for them: each ORs capacity overflows into its
+// Lists' error bits, and whether a gather list must be rebuilt is decided on
+// the device (enqueue_gather_check / the `need` flag of enqueue_lists).  The
+// host waits once, at the end; when some list overflowed, the capacities grow
+// and the evaluation runs again from the saved h -- the only input it
+// overwrites (the density passes update h).
 void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
     hipStream_t s = c->stream;
     const int64_t n = S.n;
@@ -507,93 +525,116 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
     const int32_t *gs = S.tree.gstart.get();
     double4 *kpos = S.kpos.ensure(n), *kvel = S.kvel.ensure(n);  // j records of the density / derivative passes
     launch_pack_kin(n, P.x, P.y, P.z, P.m, P.vx, P.vy, P.vz, kpos, kvel, s);
-    int kslot = 0;
-    auto kbegin = [&](const Lists *Lp, bool spj, int kind) {
-        if (!S.stats) return;
-        S.kkind[kslot] = kind;
-        hipLaunchKernelGGL(k_list_counts, dim3(nblk(G, 256)), dim3(256), 0, s, G, gs, Lp->cnt.get(),
-                           spj ? Lp->scnt.get() : nullptr, S.kcount.get() + 4 * kind);
-        SPH_LAUNCH_CHECK();
-        SPH_HIP(hipEventRecord(S.kev[kslot][0], s));
-    };
-    auto kend = [&]() {
-        if (!S.stats) return;
-        SPH_HIP(hipEventRecord(S.kev[kslot][1], s));
-        ++kslot;
-    };
-    Lists &L = S.lsph;
+    double *hsave = S.hsave.ensure(n);
+    SPH_HIP(hipMemcpyAsync(hsave, P.h, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    Lists &L = S.lsph, &Lg = S.lgrav;
     ++S.force_evals;
     auto lref = [&]() { return ListRef{L.idx.get(), nullptr, L.cnt.get(), L.cap}; };
-    // 4 density passes, each over lists that cover the current h
-    // (src/main.cpp:138-140).  One gather list, built with a margin on h, serves
-    // every pass (and the derivative pass) while each group's search box still
-    // fits inside the box it was built for; see gather_lists_valid.
-    for (int loop = 0; loop <= 3; ++loop) {
-        if (loop == 0 || !gather_lists_valid(S.tree, P, L, s)) {
-            build_lists_sph(S.tree, S.tp, P, 0, L, s, kGatherMargin);
-            ++S.gather_builds;
+    for (int attempt = 0;; ++attempt) {
+        if (attempt >= 12) throw Error(-2, "interaction list capacity could not be grown enough");
+        if (attempt > 0) SPH_HIP(hipMemcpyAsync(P.h, hsave, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        int kslot = 0;
+        if (S.stats) {
+            S.kcount_try.ensure(4 * kStatKinds);
+            hipLaunchKernelGGL(k_fill_ll, dim3(1), dim3(64), 0, s, (int64_t) 4 * kStatKinds, S.kcount_try.get());
+            SPH_LAUNCH_CHECK();
         }
-        DensArgs a{G, gs, P.x, P.y, P.z, P.m, P.h, P.m, P.x, P.y, P.z, lref(), P.rho, P.h, P.ngb, kpos};
-        // the density of the last pass feeds the EoS; the precision form keeps
-        // it FP64 unless asked otherwise (the earlier passes only steer h)
-        const bool d64 = !mixed || (loop == 3 && S.prm.eos_density_fp64);
-        kbegin(&L, false, (mixed && d64) ? 4 : 0);
-        if (d64) launch_dens_fp64(a, s);
-        else launch_dens_mixed(a, s);
-        kend();
-    }
-    rec_ev(3);
-    launch_eos_pressure(n, P.mat, P.rho, P.u, c->d_mats.get(), SPHGPU_MAX_MATERIALS, P.p, P.cs, s);
-    rec_ev(4);
-    if (!gather_lists_valid(S.tree, P, L, s)) {
-        build_lists_sph(S.tree, S.tp, P, 0, L, s, kGatherMargin);
-        ++S.gather_builds;
-    }
-    {
-        DrvtArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.cs, P.m, P.x, P.y, P.z, P.vx, P.vy, P.vz,
-                   lref(), P.divv, P.rotx, P.roty, P.rotz, P.gs, P.bal, kpos, kvel};
-        kbegin(&L, false, 1);
-        if (mixed) launch_drvt_mixed(a, s);
-        else launch_drvt_fp64(a, s);
-        kend();
-    }
-    rec_ev(5);
-    tree_update_maxH(S.tree, P.h, s);
-    build_lists_sph(S.tree, S.tp, P, 1, L, s);
-    {
-        HydroRec *rec = S.hrec.ensure(n);
-        launch_pack_hydro(n, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.m, P.rho, P.cs, P.bal, P.gs, P.p, P.h, rec, s);
-        HydroArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal,
-                    P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal, P.m,
-                    lref(), P.ax, P.ay, P.az, P.du, P.dti, rec};
-        kbegin(&L, false, 2);
-        if (mixed) launch_hydro_mixed(a, s);
-        else launch_hydro_fp64(a, s);
-        kend();
-    }
-    rec_ev(6);
-    build_lists_grav(S.tree, S.tp, P, S.lgrav, s);
-    {
-        const Tree &t = S.tree;
-        GravArgs a{G, gs, P.x, P.y, P.z, nullptr, P.h, P.x, P.y, P.z, P.m,
-                   ListRef{S.lgrav.idx.get(), nullptr, S.lgrav.cnt.get(), S.lgrav.cap},
-                   t.comx.get(), t.comy.get(), t.comz.get(), t.mass.get(),
-                   ListRef{S.lgrav.sidx.get(), nullptr, S.lgrav.scnt.get(), S.lgrav.scap},
-                   P.gx, P.gy, P.gz, P.pot};
-        kbegin(&S.lgrav, true, 3);
-        if (mixed) launch_grav_mixed(a, s);
-        else launch_grav_fp64(a, s);
-        kend();
-    }
-    rec_ev(7);
-    if (S.stats) {
-        SPH_HIP(hipStreamSynchronize(s));
-        for (int k = 0; k < kslot; ++k) {
-            float ms = 0;
-            SPH_HIP(hipEventElapsedTime(&ms, S.kev[k][0], S.kev[k][1]));
-            S.kms[S.kkind[k]] += ms;
-            S.klaunch[S.kkind[k]] += 1;
+        auto kbegin = [&](const Lists *Lp, bool spj, int kind) {
+            if (!S.stats) return;
+            S.kkind[kslot] = kind;
+            hipLaunchKernelGGL(k_list_counts, dim3(nblk(G, 256)), dim3(256), 0, s, G, gs, Lp->cnt.get(),
+                               spj ? Lp->scnt.get() : nullptr, S.kcount_try.get() + 4 * kind);
+            SPH_LAUNCH_CHECK();
+            SPH_HIP(hipEventRecord(S.kev[kslot][0], s));
+        };
+        auto kend = [&]() {
+            if (!S.stats) return;
+            SPH_HIP(hipEventRecord(S.kev[kslot][1], s));
+            ++kslot;
+        };
+        lists_reset(L, s);
+        lists_reset(Lg, s);
+        // 4 density passes, each over lists that cover the current h
+        // (src/main.cpp:138-140).  One gather list, built with a margin on h,
+        // serves every pass (and the derivative pass) while each group's
+        // search box still fits inside the box it was built for.
+        for (int loop = 0; loop <= 3; ++loop) {
+            if (loop == 0) {
+                enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s);
+            } else {
+                enqueue_gather_check(S.tree, P, L, s);
+                enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s, L.flag.get());
+            }
+            DensArgs a{G, gs, P.x, P.y, P.z, P.m, P.h, P.m, P.x, P.y, P.z, lref(), P.rho, P.h, P.ngb, kpos};
+            // the density of the last pass feeds the EoS; the precision form
+            // keeps it FP64 unless asked otherwise (the earlier passes only steer h)
+            const bool d64 = !mixed || (loop == 3 && S.prm.eos_density_fp64);
+            kbegin(&L, false, (mixed && d64) ? 4 : 0);
+            if (d64) launch_dens_fp64(a, s);
+            else launch_dens_mixed(a, s);
+            kend();
         }
+        rec_ev(3);
+        launch_eos_pressure(n, P.mat, P.rho, P.u, c->d_mats.get(), SPHGPU_MAX_MATERIALS, P.p, P.cs, s);
+        rec_ev(4);
+        enqueue_gather_check(S.tree, P, L, s);
+        enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s, L.flag.get());
+        {
+            DrvtArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.cs, P.m, P.x, P.y, P.z, P.vx, P.vy, P.vz,
+                       lref(), P.divv, P.rotx, P.roty, P.rotz, P.gs, P.bal, kpos, kvel};
+            kbegin(&L, false, 1);
+            if (mixed) launch_drvt_mixed(a, s);
+            else launch_drvt_fp64(a, s);
+            kend();
+        }
+        rec_ev(5);
+        tree_update_maxH(S.tree, P.h, s);
+        lists_fetch(L, s, 1);  // gather-list bits and construction counter, before the symmetric lists reuse L
+        enqueue_lists(S.tree, S.tp, P, 1, L, 1.0, s);
+        {
+            HydroRec *rec = S.hrec.ensure(n);
+            launch_pack_hydro(n, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.m, P.rho, P.cs, P.bal, P.gs, P.p, P.h, rec, s);
+            HydroArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal,
+                        P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal, P.m,
+                        lref(), P.ax, P.ay, P.az, P.du, P.dti, rec};
+            kbegin(&L, false, 2);
+            if (mixed) launch_hydro_mixed(a, s);
+            else launch_hydro_fp64(a, s);
+            kend();
+        }
+        rec_ev(6);
+        enqueue_lists(S.tree, S.tp, P, 2, Lg, 1.0, s);
+        {
+            const Tree &t = S.tree;
+            GravArgs a{G, gs, P.x, P.y, P.z, nullptr, P.h, P.x, P.y, P.z, P.m,
+                       ListRef{Lg.idx.get(), nullptr, Lg.cnt.get(), Lg.cap},
+                       t.comx.get(), t.comy.get(), t.comz.get(), t.mass.get(),
+                       ListRef{Lg.sidx.get(), nullptr, Lg.scnt.get(), Lg.scap},
+                       P.gx, P.gy, P.gz, P.pot};
+            kbegin(&Lg, true, 3);
+            if (mixed) launch_grav_mixed(a, s);
+            else launch_grav_fp64(a, s);
+            kend();
+        }
+        rec_ev(7);
+        lists_fetch(L, s, 0);
+        lists_fetch(Lg, s, 0);
+        SPH_HIP(hipStreamSynchronize(s));  // the one wait of the evaluation
+        const bool ok_s = lists_grow(L), ok_g = lists_grow(Lg);
+        if (!(ok_s && ok_g)) continue;
+        S.gather_builds += L.hflag.p[3];
+        if (S.stats) {
+            hipLaunchKernelGGL(k_add_ll, dim3(1), dim3(64), 0, s, (int64_t) 4 * kStatKinds, S.kcount_try.get(),
+                               S.kcount.get());
+            SPH_LAUNCH_CHECK();
+            for (int k = 0; k < kslot; ++k) {
+                float ms = 0;
+                SPH_HIP(hipEventElapsedTime(&ms, S.kev[k][0], S.kev[k][1]));
+                S.kms[S.kkind[k]] += ms;
+                S.klaunch[S.kkind[k]] += 1;
+            }
+        }
+        break;
     }
 }
 

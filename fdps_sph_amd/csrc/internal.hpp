@@ -248,22 +248,37 @@ struct Lists {
     int64_t cap = 0, scap = 0;
     DevBuf<int32_t> frontier;
     int64_t fcap = 0;        // per-workgroup BFS frontier capacity
-    DevBuf<int32_t> err, flag;
-    PinnedBuf<int32_t> hflag;  // host side of the overflow / validity flags
+    DevBuf<int32_t> err;       // overflow bits, construction counter, per-XCD group counters
+    DevBuf<int32_t> flag;      // gather-list validity: 1 when some group outgrew its list
+    PinnedBuf<int32_t> hflag;  // host copy of the overflow bits and the construction counter
     DevBuf<double> sbox;     // gather lists: per-group search box (lo[3], hi[3])
     int kind = -1;           // kind of the lists held (-1 none, 0 gather, 1 symmetric, 2 gravity)
 };
 // Search-radius factor of the gather lists that the density passes reuse:
 // 3 % on h covers the change of h over the density iterations of a step.
 constexpr double kGatherMargin = 1.03;
-// kind 0 gather, 1 symmetric.  Gather lists use the search radius 2.5 h hfac
-// (hfac >= 1: a superset that stays usable while h grows); symmetric lists are
-// always exact.  Capacities grow until the lists fit.
+constexpr int kListWords = 10;  // Lists::err: overflow bits, construction counter, 8 group counters
+// kind 0 gather, 1 symmetric, 2 gravity.  Gather lists use the search radius
+// 2.5 h hfac (hfac >= 1: a superset that stays usable while h grows);
+// symmetric lists are always exact.
+//
+// Asynchronous form: lists_reset clears the overflow bits and the construction
+// counter; enqueue_lists builds without waiting -- when `need` is given, only
+// if that device flag is nonzero -- and ORs overflows into the bits (the lists
+// are then incomplete); lists_fetch copies bits and counter to slot 0 or 1 of
+// L.hflag (4 ints), and after the stream has drained lists_grow enlarges the
+// capacities the bits of either slot name and returns true when there were none.
+void lists_reset(Lists &L, hipStream_t s);
+void enqueue_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, double hfac,
+                   hipStream_t s, const int32_t *need = nullptr);
+void lists_fetch(Lists &L, hipStream_t s, int slot = 0);
+bool lists_grow(Lists &L);
+// Sets L.flag (device) to 1 when some group's current search box (same tree,
+// positions unchanged, h possibly changed) has outgrown its gather list.
+void enqueue_gather_check(const Tree &t, const PArrays &P, Lists &L, hipStream_t s);
+// Synchronous forms: capacities grow until the lists fit.
 bool build_lists_sph(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, hipStream_t s,
                      double hfac = 1.0);
-// True when the gather lists in L still cover every particle's current support
-// (same tree, positions unchanged, h possibly changed).  Synchronises s.
-bool gather_lists_valid(const Tree &t, const PArrays &P, Lists &L, hipStream_t s);
 bool build_lists_grav(const Tree &t, const TreeParams &tp, const PArrays &P, Lists &L, hipStream_t s);
 int64_t lists_max_count(const Lists &L, int64_t ngroups, bool spj, hipStream_t s);
 

@@ -326,8 +326,18 @@ struct ListArgs {
     double hfac;    // gather search radius = 2.5 h hfac (hfac > 1 makes a reusable superset list)
     int32_t *queue; // 8 per-XCD group counters, zero at launch
     double *sbox;   // optional, per group: the search box the list was built for (lo[3], hi[3])
-    int32_t *err;
+    int32_t *err;   // overflow bits: 1 list, 2 monopole list, 4 frontier
+    const int32_t *need;  // optional device flag: build only when it is nonzero
+    int32_t *builds;      // optional device counter of the constructions done
 };
+
+// Decides on the device whether this launch builds at all (the flag is read by
+// every wave, so no host round trip decides it); counts the constructions.
+__device__ __forceinline__ bool list_launch_wanted(const ListArgs &a) {
+    if (a.need && *a.need == 0) return false;
+    if (a.builds && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.builds, 1);
+    return true;
+}
 
 __device__ __forceinline__ int wave_incl_scan(int v) {
     const int lane = threadIdx.x;
@@ -525,6 +535,7 @@ __device__ __forceinline__ int emit_particles(const ListArgs &a, const int32_t *
 
 template <int KIND>
 __global__ __launch_bounds__(kWave) void k_list_sph(ListArgs a) {
+    if (!list_launch_wanted(a)) return;
     const int lane = threadIdx.x;
     int32_t *fa0 = a.frontier + (int64_t) blockIdx.x * 2 * a.fcap;
     int32_t *fb0 = fa0 + a.fcap;
@@ -549,7 +560,7 @@ __global__ __launch_bounds__(kWave) void k_list_sph(ListArgs a) {
         auto ptest = [&](int p) { return part_test<KIND>(a, p, b); };
         const int count = emit_particles(a, fa, nf, out, a.cap, 0, test, ptest, s_pref, s_leaf);
         if (lane == 0) {
-            a.cnt[g] = count;
+            a.cnt[g] = (int32_t) min((int64_t) count, a.cap);  // an overflowed list stays an in-bounds prefix
             if (count > a.cap) atomicOr(a.err, 1);
         }
         __syncthreads();
@@ -557,6 +568,7 @@ __global__ __launch_bounds__(kWave) void k_list_sph(ListArgs a) {
 }
 
 __global__ __launch_bounds__(kWave) void k_list_grav(ListArgs a) {
+    if (!list_launch_wanted(a)) return;
     const int lane = threadIdx.x;
     const uint64_t lt = (lane == 0) ? 0ULL : ((~0ULL) >> (64 - lane));
     int32_t *fa0 = a.frontier + (int64_t) blockIdx.x * 2 * a.fcap;
@@ -616,8 +628,8 @@ __global__ __launch_bounds__(kWave) void k_list_grav(ListArgs a) {
             }
         }
         if (lane == 0) {
-            a.cnt[g] = cj;
-            a.scnt[g] = cs;
+            a.cnt[g] = (int32_t) min((int64_t) cj, a.cap);  // overflowed lists stay in-bounds prefixes
+            a.scnt[g] = (int32_t) min((int64_t) cs, a.scap);
             if (cj > a.cap) atomicOr(a.err, 1);
             if (cs > a.scap) atomicOr(a.err, 2);
         }
@@ -849,7 +861,9 @@ static ListArgs list_args(const Tree &t, const TreeParams &tp, const PArrays &P,
     a.hfac = 1.0;
     a.sbox = nullptr;
     a.err = L.err.get();
-    a.queue = L.err.get() + 1;
+    a.builds = L.err.get() + 1;
+    a.queue = L.err.get() + 2;
+    a.need = nullptr;
     return a;
 }
 
@@ -859,9 +873,18 @@ static int list_grid(int64_t ngroups) {
     return (int) std::max<int64_t>(8, b);
 }
 
-static bool run_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, double hfac,
-                      hipStream_t s) {
+void lists_reset(Lists &L, hipStream_t s) {
+    L.err.ensure(kListWords);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int64_t) 2, L.err.get(), 0);
+    SPH_LAUNCH_CHECK();
+    int32_t *h = L.hflag.ensure(4);
+    for (int k = 0; k < 4; ++k) h[k] = 0;
+}
+
+void enqueue_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, double hfac,
+                   hipStream_t s, const int32_t *need) {
     const int64_t G = t.ngroups;
+    if (kind != 0) hfac = 1.0;  // symmetric lists are exact: v_sig_max runs over the whole list
     L.idx.ensure((size_t) G * L.cap);
     L.cnt.ensure(G);
     if (kind == 2) {
@@ -871,9 +894,12 @@ static bool run_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int
     if (L.fcap <= 0) L.fcap = kFrontier0;
     const int grid = list_grid(G);
     L.frontier.ensure((size_t) grid * 2 * L.fcap);
-    L.err.ensure(1 + 8);  // error flags, then the 8 per-XCD group counters
-    SPH_HIP(hipMemsetAsync(L.err.get(), 0, sizeof(int32_t) * (1 + 8), s));
+    L.err.ensure(kListWords);  // error bits, construction counter, the 8 per-XCD group counters
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int64_t) 8, L.err.get() + 2, 0);
+    SPH_LAUNCH_CHECK();
     ListArgs a = list_args(t, tp, P, L);
+    a.need = need;
+    if (kind != 0) a.builds = nullptr;  // the counter counts gather-list constructions
     if (kind == 0) {  // gather lists remember their search boxes (reuse check)
         L.sbox.ensure((size_t) G * 6);
         a.sbox = L.sbox.get();
@@ -883,44 +909,53 @@ static bool run_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int
     else if (kind == 1) hipLaunchKernelGGL(k_list_sph<1>, dim3(grid), dim3(kWave), 0, s, a);
     else hipLaunchKernelGGL(k_list_grav, dim3(grid), dim3(kWave), 0, s, a);
     SPH_LAUNCH_CHECK();
-    int32_t *herr = L.hflag.ensure(1);
-    SPH_HIP(hipMemcpyAsync(herr, L.err.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    SPH_HIP(hipStreamSynchronize(s));
-    const int32_t err = *herr;
-    bool again = false;
-    if (err & 4) { L.fcap *= 2; again = true; }
-    if (err & 1) { L.cap *= 2; again = true; }
-    if (err & 2) { L.scap *= 2; again = true; }
-    L.kind = again ? -1 : kind;
-    return !again;
+    L.kind = kind;
+}
+
+void lists_fetch(Lists &L, hipStream_t s, int slot) {
+    int32_t *h = L.hflag.ensure(4);
+    SPH_HIP(hipMemcpyAsync(h + 2 * slot, L.err.get(), sizeof(int32_t) * 2, hipMemcpyDeviceToHost, s));
+}
+
+bool lists_grow(Lists &L) {
+    const int32_t err = L.hflag.p[0] | L.hflag.p[2];
+    if (err & 4) L.fcap *= 2;
+    if (err & 1) L.cap *= 2;
+    if (err & 2) L.scap *= 2;
+    if (err) L.kind = -1;
+    return err == 0;
+}
+
+void enqueue_gather_check(const Tree &t, const PArrays &P, Lists &L, hipStream_t s) {
+    L.flag.ensure(1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int64_t) 1, L.flag.get(), 0);
+    SPH_LAUNCH_CHECK();
+    if (t.ngroups <= 0) return;
+    hipLaunchKernelGGL(k_list_box_check, dim3((unsigned) t.ngroups), dim3(kWave), 0, s, (int) t.ngroups,
+                       t.gstart.get(), P.x, P.y, P.z, P.h, L.sbox.get(), L.flag.get());
+    SPH_LAUNCH_CHECK();
+}
+
+// synchronous forms: build, wait, grow the capacities and build again until the lists fit
+static bool build_lists_sync(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, double hfac,
+                             hipStream_t s) {
+    for (int attempt = 0; attempt < 12; ++attempt) {
+        lists_reset(L, s);
+        enqueue_lists(t, tp, P, kind, L, hfac, s);
+        lists_fetch(L, s);
+        SPH_HIP(hipStreamSynchronize(s));
+        if (lists_grow(L)) return true;
+    }
+    throw Error(-2, "interaction list capacity could not be grown enough");
 }
 
 bool build_lists_sph(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, hipStream_t s,
                      double hfac) {
-    if (kind != 0) hfac = 1.0;  // symmetric lists are exact: v_sig_max runs over the whole list
-    for (int attempt = 0; attempt < 12; ++attempt)
-        if (run_lists(t, tp, P, kind, L, hfac, s)) return true;
-    throw Error(-2, "interaction list capacity could not be grown enough");
-}
-
-bool gather_lists_valid(const Tree &t, const PArrays &P, Lists &L, hipStream_t s) {
-    if (L.kind != 0 || t.ngroups <= 0) return false;
-    L.flag.ensure(1);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s, (int64_t) 1, L.flag.get(), 0);
-    SPH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_list_box_check, dim3((unsigned) t.ngroups), dim3(kWave), 0, s, (int) t.ngroups,
-                       t.gstart.get(), P.x, P.y, P.z, P.h, L.sbox.get(), L.flag.get());
-    SPH_LAUNCH_CHECK();
-    int32_t *hf = L.hflag.ensure(1);
-    SPH_HIP(hipMemcpyAsync(hf, L.flag.get(), sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    SPH_HIP(hipStreamSynchronize(s));
-    return *hf == 0;
+    return build_lists_sync(t, tp, P, kind, L, hfac, s);
 }
 
 bool build_lists_grav(const Tree &t, const TreeParams &tp, const PArrays &P, Lists &L, hipStream_t s) {
-    for (int attempt = 0; attempt < 12; ++attempt)
-        if (run_lists(t, tp, P, 2, L, 1.0, s)) return true;
-    throw Error(-2, "gravity list capacity could not be grown enough");
+    return build_lists_sync(t, tp, P, 2, L, 1.0, s);
 }
 
 }  // namespace sphgpu

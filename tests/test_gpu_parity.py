@@ -307,3 +307,19 @@ def test_gather_list_reuse(ctx64):
     _print(f"[gather lists] {builds} builds for {evals} force evaluations")
     assert evals == 4
     assert evals <= builds <= 2 * evals
+
+
+def test_list_overflow_redo(ctx64):
+    """List capacities far too small for the lists: every force evaluation
+    overflows, grows its capacities and runs again from the saved h.  The
+    results equal those of a run whose capacities fit from the start, bit for
+    bit, and the gather-list constructions are counted once."""
+    ic = IC.granite_impact(9000, 1000, seed=1001)
+    big = Simulation(ctx64, ic)
+    small = Simulation(ctx64, ic, list_cap_sph=16, list_cap_epj=16, list_cap_spj=16)
+    big.step(2)
+    small.step(2)
+    _print(f"[list overflow redo] builds {big.list_builds()} vs {small.list_builds()}")
+    assert big.list_builds() == small.list_builds()
+    for f in ("POS", "VEL", "DENS", "SMTH", "ENG", "ACC", "GRAV", "ENGDOT", "DT", "ID"):
+        assert np.array_equal(big.get(f), small.get(f)), f
