@@ -42,10 +42,14 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
+    // The first allocation is exact; a buffer that has to grow takes a quarter
+    // more than asked, so that sizes which creep up from step to step (node,
+    // group and list counts) reallocate rarely -- hipFree waits for the device.
     T *ensure(size_t count) {
         if (count <= n && p) return p;
-        release();
         size_t c = count ? count : 1;
+        if (p) c = std::max(c, n + n / 4);
+        release();
         SPH_HIP(hipMalloc(&p, c * sizeof(T)));
         n = c;
         return p;
@@ -226,6 +230,7 @@ struct Tree {
     DevBuf<int64_t> cnt;
     DevBuf<uint8_t> flags;
     DevBuf<int32_t> sel_out, counts;
+    DevBuf<int32_t> sel_blk;  // per-block node-start counts of every level, then their offsets
     // nodes (levels flattened, level 0 = leaves)
     int nlevels = 0;
     int64_t level_off[kMaxLevels + 1] = {0};

@@ -176,10 +176,100 @@ __global__ void k_lca(int64_t n, const uint64_t *key, int bits, int sysbits, int
     cnt[k] = ub_u64(key, n, chi) - lb_u64(key, n, clo);
 }
 
-__global__ void k_flags(int64_t n, const int64_t *cnt, int64_t T, uint8_t *flags) {
-    const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    flags[k] = (cnt[k] > T) ? 1 : 0;
+// Node starts of every level in one selection.  Particle k starts a node of
+// level l when cnt[k] > T_l, T_l = n_leaf 8^l below the top level and
+// INT64_MAX - 1 at the top level L-1 (only the starts of the systems pass).
+// The thresholds grow with l, so the levels are nested: k starts a node of
+// every level up to its start level, and of none when that is -1.
+__device__ __forceinline__ int start_level(int64_t c, int L, int64_t n_leaf) {
+    if (c == INT64_MAX) return L - 1;
+    int lv = -1;
+    int64_t T = n_leaf;
+    for (int l = 0; l < L - 1; ++l) {
+        if (c > T) lv = l;
+        T *= 8;
+    }
+    return lv;
+}
+constexpr int kSelThreads = 256, kSelItems = 8, kSelTile = kSelThreads * kSelItems;
+
+// per block of kSelTile particles: the node starts of each level
+__global__ __launch_bounds__(kSelThreads) void k_sel_count(int64_t n, const int64_t *cnt, int L, int64_t n_leaf,
+                                                           int32_t *bcount) {
+    __shared__ int hist[kMaxLevels];
+    const int tid = threadIdx.x;
+    if (tid < kMaxLevels) hist[tid] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t) blockIdx.x * kSelTile + (int64_t) tid * kSelItems;
+    for (int i = 0; i < kSelItems; ++i) {
+        const int64_t k = base + i;
+        if (k < n) {
+            const int lv = start_level(cnt[k], L, n_leaf);
+            if (lv >= 0) atomicAdd(&hist[lv], 1);
+        }
+    }
+    __syncthreads();
+    if (tid < L) {
+        int c = 0;
+        for (int m = tid; m < L; ++m) c += hist[m];  // starts of level tid: start level >= tid
+        bcount[(int64_t) blockIdx.x * kMaxLevels + tid] = c;
+    }
+}
+
+// exclusive scan of the block counts of every level (one workgroup); totals to counts[0..L)
+__global__ __launch_bounds__(kSelThreads) void k_sel_scan(int nb, int L, int32_t *bcount, int32_t *counts) {
+    __shared__ int buf[kSelThreads];
+    const int tid = threadIdx.x;
+    for (int l = 0; l < L; ++l) {
+        int carry = 0;
+        for (int c0 = 0; c0 < nb; c0 += kSelThreads) {
+            const int b = c0 + tid;
+            const int v = (b < nb) ? bcount[(int64_t) b * kMaxLevels + l] : 0;
+            buf[tid] = v;
+            __syncthreads();
+            for (int o = 1; o < kSelThreads; o <<= 1) {
+                const int t = (tid >= o) ? buf[tid - o] : 0;
+                __syncthreads();
+                buf[tid] += t;
+                __syncthreads();
+            }
+            if (b < nb) bcount[(int64_t) b * kMaxLevels + l] = carry + buf[tid] - v;
+            carry += buf[kSelThreads - 1];
+            __syncthreads();
+        }
+        if (tid == 0) counts[l] = carry;
+    }
+}
+
+// writes the node starts of level l, in particle order, to out[l n + ...]
+__global__ __launch_bounds__(kSelThreads) void k_sel_scatter(int64_t n, const int64_t *cnt, int L, int64_t n_leaf,
+                                                             const int32_t *boff, int32_t *out) {
+    __shared__ int wsum[kSelThreads / kWave];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+    const int64_t base = (int64_t) blockIdx.x * kSelTile + (int64_t) tid * kSelItems;
+    int lv[kSelItems];
+#pragma unroll
+    for (int i = 0; i < kSelItems; ++i) lv[i] = (base + i < n) ? start_level(cnt[base + i], L, n_leaf) : -1;
+    for (int l = 0; l < L; ++l) {
+        int c = 0;
+#pragma unroll
+        for (int i = 0; i < kSelItems; ++i) c += (lv[i] >= l) ? 1 : 0;
+        int incl = c;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int t = __shfl_up(incl, o, kWave);
+            if (lane >= o) incl += t;
+        }
+        if (lane == kWave - 1) wsum[w] = incl;
+        __syncthreads();
+        int pos = boff[(int64_t) blockIdx.x * kMaxLevels + l] + incl - c;
+        for (int v = 0; v < w; ++v) pos += wsum[v];
+        int32_t *o = out + (int64_t) l * n;
+#pragma unroll
+        for (int i = 0; i < kSelItems; ++i)
+            if (lv[i] >= l) o[pos++] = (int32_t) (base + i);
+        __syncthreads();
+    }
 }
 
 // node ranges and children of one level
@@ -703,17 +793,21 @@ static void build_levels(Tree &t, const TreeParams &tp, int64_t n, const PArrays
         t.sel_tmp.ensure(need);
         t.sel_tmp_bytes = need;
     }
-    int64_t T = tp.n_leaf;
-    for (int l = 0; l < L; ++l) {
-        int64_t thr;
-        if (l == L - 1) thr = INT64_MAX - 1;
-        else thr = T;
-        hipLaunchKernelGGL(k_flags, dim3(nblk(n, 256)), dim3(256), 0, s, n, t.cnt.get(), thr, t.flags.get());
+    {  // node starts of all levels: per-block counts, their scan, the scatter
+        const int nb = nblk(n, kSelTile);
+        int32_t *blk = t.sel_blk.ensure((size_t) nb * kMaxLevels);
+        if (nb > 0) {
+            hipLaunchKernelGGL(k_sel_count, dim3(nb), dim3(kSelThreads), 0, s, n, t.cnt.get(), L,
+                               (int64_t) tp.n_leaf, blk);
+            SPH_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(k_sel_scan, dim3(1), dim3(kSelThreads), 0, s, nb, L, blk, t.counts.get());
         SPH_LAUNCH_CHECK();
-        size_t bytes = t.sel_tmp_bytes;
-        SPH_HIP(hipcub::DeviceSelect::Flagged(t.sel_tmp.get(), bytes, it, t.flags.get(), t.sel_out.get() + (size_t) l * n,
-                                              t.counts.get() + l, (int) n, s));
-        if (l < L - 1) T *= 8;
+        if (nb > 0) {
+            hipLaunchKernelGGL(k_sel_scatter, dim3(nb), dim3(kSelThreads), 0, s, n, t.cnt.get(), L,
+                               (int64_t) tp.n_leaf, blk, t.sel_out.get());
+            SPH_LAUNCH_CHECK();
+        }
     }
     int32_t *cnts = t.hcnt.ensure(L + 1);  // pinned: a direct copy, no staging
     SPH_HIP(hipMemcpyAsync(cnts, t.counts.get(), sizeof(int32_t) * L, hipMemcpyDeviceToHost, s));
