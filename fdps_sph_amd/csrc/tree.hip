@@ -550,7 +550,8 @@ __device__ __forceinline__ int expand_level(const ListArgs &a, int32_t *fa, int 
         int nc = 0, cfirst = 0;
         if (idx < nf) {
             const int node = fa[idx];
-            if (test(node)) { cfirst = a.c0[node]; nc = a.c1[node] - cfirst; }
+            const int32_t c0 = a.c0[node], c1 = a.c1[node];  // loaded beside the node's box, not after its test
+            if (test(node)) { cfirst = c0; nc = c1 - c0; }
         }
         const int incl = wave_incl_scan(nc);
         const int tot = __shfl(incl, kWave - 1, kWave);
@@ -567,39 +568,52 @@ __device__ __forceinline__ int expand_level(const ListArgs &a, int32_t *fa, int 
     return nn;
 }
 
-// emit the particles of one chunk of leaves (one per lane, sz particles each,
-// sz = 0 for none) that pass part_ok, in leaf order then index order
+constexpr int kEmitUnroll = 4;
+
+// emit the particles of one chunk of leaves (one per lane: particles first ..
+// first + sz - 1, sz = 0 for none) that pass part_ok, in leaf order then
+// index order
 template <class PartOk>
-__device__ __forceinline__ int emit_chunk(const ListArgs &a, int leaf, int sz, int32_t *out, int64_t cap, int count,
-                                          PartOk part_ok, int *s_pref, int *s_leaf) {
+__device__ __forceinline__ int emit_chunk(const ListArgs &a, int first, int sz, int32_t *out, int64_t cap, int count,
+                                          PartOk part_ok, int *s_pref, int *s_first) {
     const int lane = threadIdx.x;
     const uint64_t lt = (lane == 0) ? 0ULL : ((~0ULL) >> (64 - lane));
     const int incl = wave_incl_scan(sz);
     const int tot = __shfl(incl, kWave - 1, kWave);
     s_pref[lane] = incl - sz;
-    s_leaf[lane] = leaf;
+    s_first[lane] = first;
     if (lane == kWave - 1) s_pref[kWave] = incl;
     __syncthreads();
-    for (int s0 = 0; s0 < tot; s0 += kWave) {
-        const int slot = s0 + lane;
-        bool pass = false;
-        int p = 0;
-        if (slot < tot) {
-            int lo = 0, hi = kWave;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_pref[mid] <= slot) lo = mid;
-                else hi = mid;
+    // kEmitUnroll rounds of 64 slots at a time: their particle tests' loads
+    // are in flight together
+    for (int s0 = 0; s0 < tot; s0 += kEmitUnroll * kWave) {
+        int p[kEmitUnroll];
+        bool pass[kEmitUnroll];
+#pragma unroll
+        for (int u = 0; u < kEmitUnroll; ++u) {
+            const int slot = s0 + u * kWave + lane;
+            p[u] = -1;
+            if (slot < tot) {
+                int lo = 0, hi = kWave;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_pref[mid] <= slot) lo = mid;
+                    else hi = mid;
+                }
+                p[u] = s_first[lo] + (slot - s_pref[lo]);
             }
-            p = a.nstart[s_leaf[lo]] + (slot - s_pref[lo]);
-            pass = part_ok(p);
         }
-        const uint64_t m = __ballot(pass);
-        if (pass) {
-            const int pos = count + __popcll(m & lt);
-            if (pos < cap) out[pos] = p;
+#pragma unroll
+        for (int u = 0; u < kEmitUnroll; ++u) pass[u] = (p[u] >= 0) && part_ok(p[u]);
+#pragma unroll
+        for (int u = 0; u < kEmitUnroll; ++u) {
+            const uint64_t m = __ballot(pass[u]);
+            if (pass[u]) {
+                const int pos = count + __popcll(m & lt);
+                if (pos < cap) out[pos] = p[u];
+            }
+            count += __popcll(m);
         }
-        count += __popcll(m);
     }
     __syncthreads();
     return count;
@@ -609,16 +623,17 @@ __device__ __forceinline__ int emit_chunk(const ListArgs &a, int leaf, int sz, i
 // part_ok, in leaf order then index order
 template <class LeafOk, class PartOk>
 __device__ __forceinline__ int emit_particles(const ListArgs &a, const int32_t *fa, int nf, int32_t *out, int64_t cap,
-                                              int count, LeafOk leaf_ok, PartOk part_ok, int *s_pref, int *s_leaf) {
+                                              int count, LeafOk leaf_ok, PartOk part_ok, int *s_pref, int *s_first) {
     const int lane = threadIdx.x;
     for (int base = 0; base < nf; base += kWave) {
         const int idx = base + lane;
-        int sz = 0, leaf = 0;
+        int sz = 0, first = 0;
         if (idx < nf) {
-            leaf = fa[idx];
-            if (leaf_ok(leaf)) sz = a.nend[leaf] - a.nstart[leaf];
+            const int leaf = fa[idx];
+            const int32_t st = a.nstart[leaf], en = a.nend[leaf];  // loaded beside the leaf's box
+            if (leaf_ok(leaf)) { first = st; sz = en - st; }
         }
-        count = emit_chunk(a, leaf, sz, out, cap, count, part_ok, s_pref, s_leaf);
+        count = emit_chunk(a, first, sz, out, cap, count, part_ok, s_pref, s_first);
     }
     return count;
 }
@@ -630,7 +645,7 @@ __global__ __launch_bounds__(kWave) void k_list_sph(ListArgs a) {
     int32_t *fa0 = a.frontier + (int64_t) blockIdx.x * 2 * a.fcap;
     int32_t *fb0 = fa0 + a.fcap;
     __shared__ int s_pref[kWave + 1];
-    __shared__ int s_leaf[kWave];
+    __shared__ int s_first[kWave];
     const GroupQueue Q = xcd_queue(a.ngroups, a.queue);
     for (int g = Q.next(); g < Q.end; g = Q.next()) {
         const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
@@ -638,7 +653,11 @@ __global__ __launch_bounds__(kWave) void k_list_sph(ListArgs a) {
         group_box(a, i0, ni, b);
         int32_t *fa = fa0, *fb = fb0;
         if (lane == 0) fa[0] = a.sys_root[a.sys[i0]];
-        if (a.sbox && lane < 6) a.sbox[(int64_t) g * 6 + lane] = (lane < 3) ? b.slo[lane] : b.shi[lane - 3];
+        if (a.sbox && lane < 6) {  // selects, not an index by lane: the box stays in registers
+            const double v = (lane == 0) ? b.slo[0] : (lane == 1) ? b.slo[1] : (lane == 2) ? b.slo[2]
+                           : (lane == 3) ? b.shi[0] : (lane == 4) ? b.shi[1] : b.shi[2];
+            a.sbox[(int64_t) g * 6 + lane] = v;
+        }
         __syncthreads();
         int nf = 1;
         auto test = [&](int node) { return node_test<KIND>(a, node, b); };
@@ -648,7 +667,7 @@ __global__ __launch_bounds__(kWave) void k_list_sph(ListArgs a) {
         }
         int32_t *out = a.out + (int64_t) g * a.cap;
         auto ptest = [&](int p) { return part_test<KIND>(a, p, b); };
-        const int count = emit_particles(a, fa, nf, out, a.cap, 0, test, ptest, s_pref, s_leaf);
+        const int count = emit_particles(a, fa, nf, out, a.cap, 0, test, ptest, s_pref, s_first);
         if (lane == 0) {
             a.cnt[g] = (int32_t) min((int64_t) count, a.cap);  // an overflowed list stays an in-bounds prefix
             if (count > a.cap) atomicOr(a.err, 1);
@@ -664,7 +683,7 @@ __global__ __launch_bounds__(kWave) void k_list_grav(ListArgs a) {
     int32_t *fa0 = a.frontier + (int64_t) blockIdx.x * 2 * a.fcap;
     int32_t *fb0 = fa0 + a.fcap;
     __shared__ int s_pref[kWave + 1];
-    __shared__ int s_leaf[kWave];
+    __shared__ int s_first[kWave];
     const GroupQueue Q = xcd_queue(a.ngroups, a.queue);
     for (int g = Q.next(); g < Q.end; g = Q.next()) {
         const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
@@ -696,15 +715,22 @@ __global__ __launch_bounds__(kWave) void k_list_grav(ListArgs a) {
                 const bool open = (node >= 0) && !acc;
                 if (l >= 1) {
                     int nc = 0, cfirst = 0;
-                    if (open) { cfirst = a.c0[node]; nc = a.c1[node] - cfirst; }
+                    if (node >= 0) {  // loaded beside the node's data, not after its test
+                        const int32_t c0 = a.c0[node], c1 = a.c1[node];
+                        if (open) { cfirst = c0; nc = c1 - c0; }
+                    }
                     const int incl = wave_incl_scan(nc);
                     const int pos0 = nn + incl - nc;
                     for (int c = 0; c < nc; ++c)
                         if (pos0 + c < a.fcap) fb[pos0 + c] = cfirst + c;
                     nn += __shfl(incl, kWave - 1, kWave);
                 } else {
-                    const int sz = open ? a.nend[node] - a.nstart[node] : 0;
-                    cj = emit_chunk(a, open ? node : 0, sz, jout, a.cap, cj, all, s_pref, s_leaf);
+                    int sz = 0, first = 0;
+                    if (node >= 0) {
+                        const int32_t st = a.nstart[node], en = a.nend[node];
+                        if (open) { first = st; sz = en - st; }
+                    }
+                    cj = emit_chunk(a, first, sz, jout, a.cap, cj, all, s_pref, s_first);
                 }
             }
             if (l >= 1) {
