@@ -346,6 +346,32 @@ __device__ __forceinline__ uint64_t dot_mask(const float4 *q, int nt, float tx, 
     return ((uint64_t) hi << 32) | lo;
 }
 
+// Symmetric form (inside either support): the bit is the sign of
+// t + min(-c, w - Q_j), t = |o_j|^2 - 2 o_i.o_j as above, c the lane's own
+// threshold, w = |o_i|^2 and Q_j the threshold of j's support (qs[jj]):
+// t <= c, or t + w <= Q_j.  A lane with c = -inf and w = +inf admits nothing.
+__device__ __forceinline__ uint32_t dot_bit_sym_(uint32_t m, const float4 &q, float Qj, float tx, float ty, float tz,
+                                                 float negc, float w) {
+    float t = fmaf(tx, q.x, q.w);
+    t = fmaf(ty, q.y, t);
+    t = fmaf(tz, q.z, t);
+    return __builtin_amdgcn_alignbit(m, __float_as_uint(t + fminf(negc, w - Qj)), 31);
+}
+__device__ __forceinline__ uint64_t dot_mask_sym(const float4 *q, const float *qs, int nt, float tx, float ty,
+                                                 float tz, float negc, float w) {
+    uint32_t lo = 0, hi = 0;
+    if (nt == kWave) {  // unrolled in part: the full unroll's hoisted LDS reads cost a wave of occupancy
+#pragma unroll 16
+        for (int jj = kWave - 1; jj >= 32; --jj) hi = dot_bit_sym_(hi, q[jj], qs[jj], tx, ty, tz, negc, w);
+#pragma unroll 16
+        for (int jj = 31; jj >= 0; --jj) lo = dot_bit_sym_(lo, q[jj], qs[jj], tx, ty, tz, negc, w);
+    } else {
+        for (int jj = nt - 1; jj >= 32; --jj) hi = dot_bit_sym_(hi, q[jj], qs[jj], tx, ty, tz, negc, w);
+        for (int jj = (nt < 32 ? nt : 32) - 1; jj >= 0; --jj) lo = dot_bit_sym_(lo, q[jj], qs[jj], tx, ty, tz, negc, w);
+    }
+    return ((uint64_t) hi << 32) | lo;
+}
+
 __device__ __forceinline__ int64_t list_begin(const ListRef &L, int g) {
     return L.off ? L.off[g] : (int64_t) g * L.cap;
 }

@@ -77,13 +77,14 @@ __device__ __forceinline__ double wave_max_(double v) {
 }
 
 // The support test runs in FP32 on offsets o = (float)(x - c) from the centre c
-// of the group's box; the FP64 pair code then evaluates the pairs it admits.
-// The FP32 test admits a superset of the in-support pairs: each offset
-// component rounds by at most 2^-24 |o|, and the FP32 difference and r^2 by a
-// few ulp more, so r_fp32 <= r + 2^-24 (|o_i|_1 + |o_j|_1) + 1e-6 r.  The cut
-// radius is H (1 + 1e-4) + pad(o_i) + pad(o_j) with pad(o) = 4 * 2^-24 |o|_1.
-// Pairs admitted beyond the support give W = gradW = 0 exactly, so every sum
-// is the one over the in-support pairs.
+// of the group's box, in dot form (|o_j|^2 - 2 o_i.o_j against a threshold,
+// see dot_mask and dot_cut_); the FP64 pair code then evaluates the pairs it
+// admits.  The thresholds make the FP32 test admit a superset of the
+// in-support pairs: each offset component rounds by at most 2^-24 |o|, which
+// the radius H (1 + 1e-4) + 2^-22 (|o_i|_1 + |o_j|_1) covers, and the FMA
+// chain's rounding, a few ulp of |o_i|^2 + |o_j|^2, is covered by
+// 2^-19 (|o_i|_1^2 + |o_j|_1^2).  Pairs admitted beyond the support give
+// W = gradW = 0 exactly, so every sum is the one over the in-support pairs.
 struct Frame {
     double cx, cy, cz;
 };
@@ -102,27 +103,10 @@ __device__ __forceinline__ float dot_cut_(float Hc, float wi, float ni, float nm
     return fmaf(r, r, -wi) + 1.9073486328125e-6f * (ni * ni + nmax * nmax);
 }
 __device__ __forceinline__ float norm1_(float ox, float oy, float oz) { return fabsf(ox) + fabsf(oy) + fabsf(oz); }
-__device__ __forceinline__ float pad_(float ox, float oy, float oz) {
-    return 2.384185791015625e-7f * (fabsf(ox) + fabsf(oy) + fabsf(oz));  // 4 * 2^-24 |o|_1
-}
 __device__ __forceinline__ float wave_maxf_(float v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
     return v;
-}
-
-// symmetric search: inside the lane's support or inside j's (o.w: j's padded
-// cut radius squared, with the group's largest i pad)
-__device__ __forceinline__ uint64_t support_mask_sym_(const float4 *o, int nt, float xi, float yi, float zi,
-                                                      float cut2, bool act) {
-    uint64_t mask = 0;
-    for (int jj = 0; jj < nt; ++jj) {
-        const float4 q = o[jj];
-        const float dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        mask |= (r2 <= cut2 || (act && r2 <= q.w)) ? (1ULL << jj) : 0ULL;
-    }
-    return mask;
 }
 
 // ---------------------------------------------------------------- density
@@ -339,9 +323,13 @@ __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
     const Tiles tl(a.L.cnt[g]);
     const Frame fr = group_frame(xi, yi, zi, act);
     const float oxi = (float) (xi - fr.cx), oyi = (float) (yi - fr.cy), ozi = (float) (zi - fr.cz);
-    const float padi = pad_(oxi, oyi, ozi);
-    const float cuti = (float) Hi * 1.0001f + padi;
-    const float padi_max = wave_maxf_(act ? padi : 0.f);
+    // dot-form support tests (see dot_mask_sym): the lane's own threshold per
+    // tile from dot_cut_, j's threshold Q_j with the group's largest |o_i|_1
+    const float Hci = (float) Hi * 1.0001f;
+    const float wi = fmaf(oxi, oxi, fmaf(oyi, oyi, ozi * ozi)), n1i = norm1_(oxi, oyi, ozi);
+    const float n1max = wave_maxf_(act ? n1i : 0.f);
+    const float wlane = act ? wi : INFINITY;
+    const float tx = -2.f * oxi, ty = -2.f * oyi, tz = -2.f * ozi;
     // v_sig bound of pass 1: with v0 the centre of the group's velocity box,
     // c_i + c_j - 3w <= c_i + c_j + 3|v_i - v_j| <= (c_i + 3|v_i - v0|) + (c_j + 3|v_j - v0|)
     const double cvx = 0.5 * (wave_min_(act ? vxi : INFINITY) + wave_max_(act ? vxi : -INFINITY));
@@ -350,10 +338,11 @@ __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
     const double bi = csi + 3.0 * sqrt((vxi - cvx) * (vxi - cvx) + (vyi - cvy) * (vyi - cvy) + (vzi - cvz) * (vzi - cvz));
     double ax = 0.0, ay = 0.0, az = 0.0, du = 0.0, vmax = 0.0;
     __shared__ HydroJ sj[kWave];
-    __shared__ float4 so[kWave];  // FP32 offsets, cut2 of j
+    __shared__ float4 so[kWave];  // FP32 offsets from the frame centre, |offset|^2
+    __shared__ float sQ[kWave];   // threshold of j's support
     for (int t = 0; t < tl.T; ++t) {
         const int k = tl.entry(t, lane);
-        float padj = 0.f;
+        float nj1 = 0.f;
         double bj = 0.0;
         if (k < tl.nj) {
             const int q = a.L.idx[lb + k];
@@ -364,11 +353,13 @@ __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
             const double ux = r.vx - cvx, uy = r.vy - cvy, uz = r.vz - cvz;
             bj = r.cs + 3.0 * sqrt(ux * ux + uy * uy + uz * uz);
             const float ox = (float) (r.x - fr.cx), oy = (float) (r.y - fr.cy), oz = (float) (r.z - fr.cz);
-            padj = pad_(ox, oy, oz);
-            const float cutj = (float) r.H * 1.0001f + padj + padi_max;
-            so[lane] = make_float4(ox, oy, oz, cutj * cutj);
+            so[lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
+            nj1 = norm1_(ox, oy, oz);
+            // as dot_cut_ with the roles of i and j exchanged, for the group's largest |o_i|_1
+            const float R = (float) r.H * 1.0001f + 2.384185791015625e-7f * (n1max + nj1);
+            sQ[lane] = fmaf(R, R, 1.9073486328125e-6f * (n1max * n1max + nj1 * nj1));
         }
-        const float cut = cuti + wave_maxf_(padj);
+        const float negc = act ? -dot_cut_(Hci, wi, n1i, wave_maxf_(nj1)) : INFINITY;
         // the tile can raise no lane's maximum when the bound, with a margin far
         // above the rounding of either side, stays below it: pass 1 is skipped
         const double bmax = wave_max_(bj);
@@ -401,7 +392,7 @@ __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
         // F_i = norm_i s(1 + 7s + 16s^2)(1-s)^7 / (r H_i + 1e-6 h_i); the two
         // reciprocals come from one, 1/(a_i a_j).  AVh is half the viscosity
         // AV of the reference, and the sums leave out K.
-        const uint64_t mask = support_mask_sym_(so, nt, oxi, oyi, ozi, act ? cut * cut : -1.f, act);
+        const uint64_t mask = dot_mask_sym(so, sQ, nt, tx, ty, tz, negc, wlane);
         for (uint64_t msk = mask; msk; msk &= msk - 1) {
             const HydroJ &J = sj[__builtin_ctzll(msk)];
             const double drx = xi - J.x, dry = yi - J.y, drz = zi - J.z;
