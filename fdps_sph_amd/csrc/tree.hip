@@ -272,19 +272,37 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_scatter(int64_t n, const in
     }
 }
 
-// node ranges and children of one level
-__global__ void k_node_ranges(int64_t cnt_l, int64_t off_l, int64_t n, int32_t *nstart, int32_t *nend,
-                              int32_t *c0, int32_t *c1, int64_t cnt_below, int64_t off_below) {
-    const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= cnt_l) return;
-    const int64_t id = off_l + q;
-    const int32_t st = nstart[id];
-    const int32_t en = (q + 1 < cnt_l) ? nstart[id + 1] : (int32_t) n;
-    nend[id] = en;
-    if (off_below < 0) { c0[id] = -1; c1[id] = -1; return; }
-    const int32_t *below = nstart + off_below;
-    c0[id] = (int32_t) (off_below + lb_i32(below, (int) cnt_below, st));
-    c1[id] = (int32_t) (off_below + lb_i32(below, (int) cnt_below, en));
+// node counts and offsets of every level, passed by value (level = blockIdx.y)
+struct LevelLayout {
+    int64_t cnt[kMaxLevels], off[kMaxLevels];
+};
+
+// node starts of every level into the flattened node array
+__global__ void k_copy_levels(int64_t n, LevelLayout lv, const int32_t *sel, int32_t *nstart) {
+    const int l = blockIdx.y;
+    for (int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; q < lv.cnt[l]; q += (int64_t) gridDim.x * blockDim.x)
+        nstart[lv.off[l] + q] = sel[(int64_t) l * n + q];
+}
+
+// node ranges and children of every level
+__global__ void k_node_ranges(int64_t n, LevelLayout lv, int32_t *nstart, int32_t *nend, int32_t *c0, int32_t *c1) {
+    const int l = blockIdx.y;
+    const int64_t cnt_l = lv.cnt[l], off_l = lv.off[l];
+    for (int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; q < cnt_l; q += (int64_t) gridDim.x * blockDim.x) {
+        const int64_t id = off_l + q;
+        const int32_t st = nstart[id];
+        const int32_t en = (q + 1 < cnt_l) ? nstart[id + 1] : (int32_t) n;
+        nend[id] = en;
+        if (l == 0) {
+            c0[id] = -1;
+            c1[id] = -1;
+        } else {
+            const int64_t off_below = lv.off[l - 1];
+            const int32_t *below = nstart + off_below;
+            c0[id] = (int32_t) (off_below + lb_i32(below, (int) lv.cnt[l - 1], st));
+            c1[id] = (int32_t) (off_below + lb_i32(below, (int) lv.cnt[l - 1], en));
+        }
+    }
 }
 
 struct MomArgs {
@@ -848,15 +866,19 @@ static void build_levels(Tree &t, const TreeParams &tp, int64_t n, const PArrays
     t.comx.ensure(nn); t.comy.ensure(nn); t.comz.ensure(nn);
     t.lox.ensure(nn); t.loy.ensure(nn); t.loz.ensure(nn);
     t.hix.ensure(nn); t.hiy.ensure(nn); t.hiz.ensure(nn); t.maxH.ensure(nn);
-    for (int l = 0; l < L; ++l)
-        SPH_HIP(hipMemcpyAsync(t.nstart.get() + t.level_off[l], t.sel_out.get() + (size_t) l * n,
-                               sizeof(int32_t) * cnts[l], hipMemcpyDeviceToDevice, s));
-    for (int l = 0; l < L; ++l) {
-        const int64_t cl = cnts[l];
-        const int64_t offb = (l == 0) ? -1 : t.level_off[l - 1];
-        const int64_t cb = (l == 0) ? 0 : cnts[l - 1];
-        hipLaunchKernelGGL(k_node_ranges, dim3(nblk(cl, 256)), dim3(256), 0, s, cl, t.level_off[l], n, t.nstart.get(),
-                           t.nend.get(), t.c0.get(), t.c1.get(), cb, offb);
+    {  // one launch per job for all levels
+        LevelLayout lv{};
+        int64_t cmax = 1;
+        for (int l = 0; l < L; ++l) {
+            lv.cnt[l] = cnts[l];
+            lv.off[l] = t.level_off[l];
+            cmax = std::max<int64_t>(cmax, cnts[l]);
+        }
+        const dim3 grid((unsigned) std::min<int64_t>(nblk(cmax, 256), 1024), (unsigned) L);
+        hipLaunchKernelGGL(k_copy_levels, grid, dim3(256), 0, s, n, lv, t.sel_out.get(), t.nstart.get());
+        SPH_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_node_ranges, grid, dim3(256), 0, s, n, lv, t.nstart.get(), t.nend.get(), t.c0.get(),
+                           t.c1.get());
         SPH_LAUNCH_CHECK();
     }
     MomArgs ma{t.nstart.get(), t.nend.get(), t.c0.get(), t.c1.get(), t.mass.get(), t.mx.get(), t.my.get(), t.mz.get(),
