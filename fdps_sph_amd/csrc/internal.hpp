@@ -372,6 +372,74 @@ __device__ __forceinline__ uint64_t dot_mask_sym(const float4 *q, const float *q
     return ((uint64_t) hi << 32) | lo;
 }
 
+// The same mask from the matrix cores.  t_ij - c_i is the bilinear form
+// (o_j, w_j, 1, 0) . (-2 o_i, 1, -c_i, 0): twelve v_mfma_f32_32x32x2f32 per
+// 64 x 64 tile (four 32 x 32 blocks, K = 6 in three steps) instead of 64 x 3
+// FMAs per lane, then one sign-bit shift per pair.  Operand layout of the
+// instruction: lane l supplies A[l%32][l/32] and B[l/32][l%32]; the result
+// holds column l%32, register r row 8(r/4) + 4(l/32) + r%4.  Column n of
+// column block ib is particle 32 ib + n: lane n supplies its even components
+// and lane n + 32 its odd ones, each lane giving its own particle's values
+// when its half is ib and its partner's (lane l^32) otherwise.  Rows j are fed
+// tile entries in the order that makes register r of lane half h the entry
+// 32 jb + 16 h + r, so each lane's 16 signs of a block are 16 consecutive
+// entries; the blocks of ib = h are this lane's particle, those of the other
+// ib its partner's, and lanes l and l^32 swap the latter.
+// (tx, ty, tz) = -2 o_i and negc = -c_i of this lane, (px, py, pz, pc) the
+// same of lane l^32; entries >= nt are cut off.  The rounding of the matrix
+// sums (a few ulp of max(H^2, |o|^2)) is of the order the thresholds of
+// dot_cut_ already allow for the FMA chain.
+typedef float f32x16_ __attribute__((ext_vector_type(16)));
+__device__ __forceinline__ uint32_t sign_bits16_(const f32x16_ &c) {
+    uint32_t w = 0;
+#pragma unroll
+    for (int r = 15; r >= 0; --r) w = __builtin_amdgcn_alignbit(w, __float_as_uint(c[r]), 31);
+    return w;
+}
+__device__ __forceinline__ uint64_t mfma_dot_mask(const float4 *q, int nt, float tx, float ty, float tz, float negc,
+                                                  float px, float py, float pz, float pc) {
+    const int lane = threadIdx.x, h = lane >> 5, m = lane & 31;
+    // j operands: half 0 supplies k = 0, 2, 4 (o_x, o_z, 1), half 1 k = 1, 3, 5 (o_y, w, 0)
+    const int e = 16 * ((m >> 2) & 1) + 4 * (m >> 3) + (m & 3);
+    const float4 q0 = q[e], q1 = q[32 + e];
+    const float a00 = h ? q0.y : q0.x, a01 = h ? q0.w : q0.z, a02 = h ? 0.f : 1.f;
+    const float a10 = h ? q1.y : q1.x, a11 = h ? q1.w : q1.z, a12 = a02;
+    // i operands: half 0 supplies k = 0, 2, 4 (-2 o_x, -2 o_z, -c), half 1 k = 1, 3, 5 (-2 o_y, 1, 0),
+    // of particle n (block 0) or 32 + n (block 1)
+    const float b00 = h ? py : tx, b01 = h ? 1.f : tz, b02 = h ? 0.f : negc;
+    const float b10 = h ? ty : px, b11 = h ? 1.f : pz, b12 = h ? 0.f : pc;
+    f32x16_ c;
+    c = f32x16_{};
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a00, b00, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a01, b01, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a02, b02, c, 0, 0, 0);
+    const uint32_t w00 = sign_bits16_(c);  // entries of row block 0, particles of column block 0
+    c = f32x16_{};
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a10, b00, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a11, b01, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a12, b02, c, 0, 0, 0);
+    const uint32_t w10 = sign_bits16_(c);
+    c = f32x16_{};
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a00, b10, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a01, b11, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a02, b12, c, 0, 0, 0);
+    const uint32_t w01 = sign_bits16_(c);
+    c = f32x16_{};
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a10, b10, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a11, b11, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a12, b12, c, 0, 0, 0);
+    const uint32_t w11 = sign_bits16_(c);
+    const uint32_t own0 = h ? w01 : w00, own1 = h ? w11 : w10;  // column block h: this lane's particle
+    const uint32_t par0 = h ? w00 : w01, par1 = h ? w10 : w11;  // the other: the partner's
+    const uint32_t recv = (uint32_t) __shfl_xor((int) (par0 | (par1 << 16)), 32, kWave);
+    const int so_ = 16 * h, sp_ = 16 - so_;
+    const uint32_t lo = (own0 << so_) | ((recv & 0xffffu) << sp_);
+    const uint32_t hi = (own1 << so_) | ((recv >> 16) << sp_);
+    uint64_t mask = ((uint64_t) hi << 32) | lo;
+    if (nt < kWave) mask &= (1ULL << nt) - 1;
+    return mask;
+}
+
 __device__ __forceinline__ int64_t list_begin(const ListRef &L, int g) {
     return L.off ? L.off[g] : (int64_t) g * L.cap;
 }

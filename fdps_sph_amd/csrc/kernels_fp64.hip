@@ -126,6 +126,7 @@ __global__ __launch_bounds__(kWave) void k_dens(DensArgs a) {
     const float Hc = (float) H * 1.0001f;
     const float wi = fmaf(oxi, oxi, fmaf(oyi, oyi, ozi * ozi)), n1i = norm1_(oxi, oyi, ozi);
     const float tx = -2.f * oxi, ty = -2.f * oyi, tz = -2.f * ozi;
+    const float ptx = __shfl_xor(tx, 32, kWave), pty = __shfl_xor(ty, 32, kWave), ptz = __shfl_xor(tz, 32, kWave);
     double dens = 0.0;
     int ngb = 0;
     const int64_t lb = list_begin(a.L, g);
@@ -144,9 +145,10 @@ __global__ __launch_bounds__(kWave) void k_dens(DensArgs a) {
             so[lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
             nj1 = norm1_(ox, oy, oz);
         }
-        const float c = act ? dot_cut_(Hc, wi, n1i, wave_maxf_(nj1)) : -INFINITY;
+        const float negc = act ? -dot_cut_(Hc, wi, n1i, wave_maxf_(nj1)) : INFINITY;
+        const float pnegc = __shfl_xor(negc, 32, kWave);
         __syncthreads();
-        const uint64_t mask = dot_mask(so, tl.count(t), tx, ty, tz, c);
+        const uint64_t mask = mfma_dot_mask(so, tl.count(t), tx, ty, tz, negc, ptx, pty, ptz, pnegc);
         for (uint64_t m = mask; m; m &= m - 1) {
             const double4 pj = sp[__builtin_ctzll(m)];
             const double dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
@@ -185,6 +187,7 @@ __global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
     const float Hc = (float) H * 1.0001f;
     const float wi = fmaf(oxi, oxi, fmaf(oyi, oyi, ozi * ozi)), n1i = norm1_(oxi, oyi, ozi);
     const float tx = -2.f * oxi, ty = -2.f * oyi, tz = -2.f * ozi;
+    const float ptx = __shfl_xor(tx, 32, kWave), pty = __shfl_xor(ty, 32, kWave), ptz = __shfl_xor(tz, 32, kWave);
     const double reg = 1.0e-6 * hi;
     // the sums leave out the factor K = -22 norm of gradW
     double divv = 0.0, rx = 0.0, ry = 0.0, rz = 0.0, gsw = 0.0, gsf = 0.0;
@@ -213,9 +216,10 @@ __global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
             so[lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
             nj1 = norm1_(ox, oy, oz);
         }
-        const float c = act ? dot_cut_(Hc, wi, n1i, wave_maxf_(nj1)) : -INFINITY;
+        const float negc = act ? -dot_cut_(Hc, wi, n1i, wave_maxf_(nj1)) : INFINITY;
+        const float pnegc = __shfl_xor(negc, 32, kWave);
         __syncthreads();
-        const uint64_t mask = dot_mask(so, tl.count(t), tx, ty, tz, c);
+        const uint64_t mask = mfma_dot_mask(so, tl.count(t), tx, ty, tz, negc, ptx, pty, ptz, pnegc);
         for (uint64_t msk = mask; msk; msk &= msk - 1) {
             const int jj = __builtin_ctzll(msk);
             const double4 pj = sp[jj];

@@ -93,9 +93,11 @@ __global__ __launch_bounds__(kWave) void k_dens_mixed(DensArgs a) {
     const double hid = a.hi[ip];
     const KernelConsts kc = kernel_consts(hid);
     const float H = kc.H, invH = kc.invH, norm = kc.norm;
-    // dot-form mask constants (see dot_mask): inside when |o_j - o_i|^2 < H^2 (1 + 1e-4)
-    const float cdot = act ? H * H * 1.0001f - fmaf(xi, xi, fmaf(yi, yi, zi * zi)) : -INFINITY;
+    // mask constants (see mfma_dot_mask): inside when |o_j - o_i|^2 < H^2 (1 + 1e-4)
+    const float negc = act ? fmaf(xi, xi, fmaf(yi, yi, zi * zi)) - H * H * 1.0001f : INFINITY;
     const float tx = -2.f * xi, ty = -2.f * yi, tz = -2.f * zi;
+    const float ptx = __shfl_xor(tx, 32, kWave), pty = __shfl_xor(ty, 32, kWave), ptz = __shfl_xor(tz, 32, kWave);
+    const float pnegc = __shfl_xor(negc, 32, kWave);
     Kahan dens;
     int ngb = 0;
     const int64_t lb = list_begin(a.L, g);
@@ -130,7 +132,7 @@ __global__ __launch_bounds__(kWave) void k_dens_mixed(DensArgs a) {
         }
         const int nt = tl.count(t);
         float part = 0.f;
-        const uint64_t mask = dot_mask(so[b], nt, tx, ty, tz, cdot);
+        const uint64_t mask = mfma_dot_mask(so[b], nt, tx, ty, tz, negc, ptx, pty, ptz, pnegc);
         for (uint64_t msk = mask; msk; msk &= msk - 1) {
             const int jj = __builtin_ctzll(msk);
             const float4 oj = so[b][jj];
@@ -173,8 +175,10 @@ __global__ __launch_bounds__(kWave) void k_drvt_mixed(DrvtArgs a) {
     const KernelConsts kc = kernel_consts(hid);
     const float H = kc.H, invH = kc.invH, norm = kc.norm;
     const float reg = (float) (1.0e-6 * hid);
-    const float cdot = act ? H * H * 1.0001f - fmaf(xi, xi, fmaf(yi, yi, zi * zi)) : -INFINITY;
+    const float negc = act ? fmaf(xi, xi, fmaf(yi, yi, zi * zi)) - H * H * 1.0001f : INFINITY;
     const float tx = -2.f * xi, ty = -2.f * yi, tz = -2.f * zi;
+    const float ptx = __shfl_xor(tx, 32, kWave), pty = __shfl_xor(ty, 32, kWave), ptz = __shfl_xor(tz, 32, kWave);
+    const float pnegc = __shfl_xor(negc, 32, kWave);
     Kahan divv, rx, ry, rz, gsum;
     const int64_t lb = list_begin(a.L, g);
     const int nj = a.L.cnt[g];
@@ -203,7 +207,7 @@ __global__ __launch_bounds__(kWave) void k_drvt_mixed(DrvtArgs a) {
         __syncthreads();
         const int nt = tl.count(t);
         float pd = 0.f, px = 0.f, py = 0.f, pz = 0.f, pg = 0.f;
-        const uint64_t mask = dot_mask(so, nt, tx, ty, tz, cdot);
+        const uint64_t mask = mfma_dot_mask(so, nt, tx, ty, tz, negc, ptx, pty, ptz, pnegc);
         for (uint64_t msk = mask; msk; msk &= msk - 1) {
             const int jj = __builtin_ctzll(msk);
             const float4 oj = so[jj];
