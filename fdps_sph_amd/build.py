@@ -22,13 +22,17 @@ ARCH = os.environ.get("FDPS_SPH_ARCH", "gfx950")
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# The support masks run on the matrix cores; their accumulators stay in
+# ordinary VGPRs (gfx950 allows it) instead of a separate block of AGPRs that
+# would cost waves of occupancy.
+MFMA_VGPR = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
 SOURCES = {
-    "kernels_fp64.hip": ["-ffp-contract=off"],
+    "kernels_fp64.hip": ["-ffp-contract=off", *MFMA_VGPR],
     "tree.hip": ["-ffp-contract=off"],
     "eos.hip": ["-ffp-contract=off"],
     "integrate.hip": ["-ffp-contract=off"],
     "api.hip": ["-ffp-contract=off"],
-    "kernels_mixed.hip": ["-ffp-contract=fast"],
+    "kernels_mixed.hip": ["-ffp-contract=fast", *MFMA_VGPR],
 }
 
 
