@@ -364,18 +364,21 @@ __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
             sQ[lane] = fmaf(R, R, 1.9073486328125e-6f * (n1max * n1max + nj1 * nj1));
         }
         const float negc = act ? -dot_cut_(Hci, wi, n1i, wave_maxf_(nj1)) : INFINITY;
-        // the tile can raise no lane's maximum when the bound, with a margin far
-        // above the rounding of either side, stays below it: pass 1 is skipped
-        const double bmax = wave_max_(bj);
-        const bool raise = act && !(bi + bmax < vmax * (1.0 - 1.0e-10));
+        // the entries that can raise some lane's maximum: b_j >= T, T the least
+        // over the lanes of vmax_i (1 - 1e-10) - b_i.  For every other entry
+        // b_i + b_j stays below each lane's maximum by a margin far above the
+        // rounding of either side, so pass 1 visits only these (none: the tile
+        // is skipped)
+        const double T = wave_min_(act ? vmax * (1.0 - 1.0e-10) - bi : INFINITY);
+        const uint64_t cand = __ballot(k < tl.nj && !(bj < T));
         __syncthreads();
         const int nt = tl.count(t);
-        const int n1 = __ballot(raise) ? nt : 0;
-        // pass 1, every j: v_sig_max.  v_sig = (c_i + c_j) - 3w with |w| =
-        // |dv.dr|/|dr| <= |dv|; when even that bound stays below the current
-        // maximum (with a margin far above the rounding of either side) the
-        // square root and division are skipped -- the maximum cannot change.
-        for (int jj = 0; jj < n1; ++jj) {
+        // pass 1, every entry that may matter: v_sig_max.  v_sig = (c_i + c_j) - 3w
+        // with |w| = |dv.dr|/|dr| <= |dv|; when even that bound stays below the
+        // current maximum (with a margin far above the rounding of either side)
+        // the square root and division are skipped -- the maximum cannot change.
+        for (uint64_t cm = cand; cm; cm &= cm - 1) {
+            const int jj = __builtin_ctzll(cm);
             const double drx = xi - sj[jj].x, dry = yi - sj[jj].y, drz = zi - sj[jj].z;
             const double dvx = vxi - sj[jj].vx, dvy = vyi - sj[jj].vy, dvz = vzi - sj[jj].vz;
             const double dvdr = dvx * drx + dvy * dry + dvz * drz;

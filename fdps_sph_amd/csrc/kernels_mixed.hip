@@ -51,6 +51,11 @@ __device__ __forceinline__ float wave_maxf(float v) {
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
     return v;
 }
+__device__ __forceinline__ float wave_minf(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, kWave));
+    return v;
+}
 __device__ __forceinline__ double wave_centre(double v, bool act) {
     const double lo = wave_min(act ? v : INFINITY);
     const double hi = wave_max(act ? v : -INFINITY);
@@ -322,16 +327,17 @@ __global__ __launch_bounds__(kWave) void k_hydro_mixed(HydroArgs a) {
             snorm[lane] = (float) nj3;
             scut[lane] = (float) (Hj * Hj) * 1.0001f;
         }
-        // the tile can raise no lane's v_sig maximum when the bound, with a
-        // margin far above the FP32 rounding, stays below it
-        const float bmax = wave_maxf(bnd);
-        const bool raise = act && !(bi + bmax < vmax * (1.f - 1.0e-5f));
+        // the entries that can raise some lane's v_sig maximum: bound_j >= T,
+        // T the least over the lanes of vmax_i (1 - 1e-5) - b_i (a margin far
+        // above the FP32 rounding); no other entry can change any maximum
+        const float T = wave_minf(act ? vmax * (1.f - 1.0e-5f) - bi : INFINITY);
+        const uint64_t cand = __ballot(k < nj && !(bnd < T));
         __syncthreads();
         const int nt = tl.count(t);
         float px = 0.f, py = 0.f, pz = 0.f, pu = 0.f;
-        // pass 1, every j: v_sig_max and the mask of pairs inside either support
+        // pass 1: v_sig_max and the mask of pairs inside either support
         uint64_t mask = 0;
-        if (__ballot(raise)) {
+        if (__popcll(cand) > 48) {  // most of the tile: one fused loop
             for (int jj = 0; jj < nt; ++jj) {
                 const float drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
                 const float dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
@@ -341,11 +347,20 @@ __global__ __launch_bounds__(kWave) void k_hydro_mixed(HydroArgs a) {
                 vmax = fmaxf(vmax, csi + scs[jj] - 3.f * w);
                 mask |= (r2 < rcut2i || (act && r2 < scut[jj])) ? (1ULL << jj) : 0ULL;
             }
-        } else {
+        } else {  // the mask over every entry, v_sig over the candidates only
             for (int jj = 0; jj < nt; ++jj) {
                 const float drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
                 const float r2 = drx * drx + dry * dry + drz * drz;
                 mask |= (r2 < rcut2i || (act && r2 < scut[jj])) ? (1ULL << jj) : 0ULL;
+            }
+            for (uint64_t cm = cand; cm; cm &= cm - 1) {
+                const int jj = __builtin_ctzll(cm);
+                const float drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
+                const float dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
+                const float dvdr = dvx * drx + dvy * dry + dvz * drz;
+                const float r2 = drx * drx + dry * dry + drz * drz;
+                const float w = (dvdr < 0.f) ? dvdr * rsq(r2 + 1e-30f) : 0.f;
+                vmax = fmaxf(vmax, csi + scs[jj] - 3.f * w);
             }
         }
         // pass 2: force terms of this lane's in-support pairs
