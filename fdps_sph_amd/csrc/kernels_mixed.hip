@@ -420,7 +420,7 @@ __device__ __forceinline__ void grav_tile_mixed(const float *sx, const float *sy
 #pragma unroll 4
     for (int jj = 0; jj < nt; ++jj) {
         const float drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
-        const float r2 = drx * drx + dry * dry + drz * drz + eps2;
+        const float r2 = fmaf(drz, drz, fmaf(dry, dry, fmaf(drx, drx, eps2)));  // r^2 + eps^2 in three FMAs
         const float inv = rsq(r2);
         const float gm_inv = sgm[jj] * inv;
         const float gm_inv3 = gm_inv * inv * inv;
