@@ -64,8 +64,10 @@ KERNEL_SYMBOL = {"dens": ("k_dens", False), "drvt": ("k_drvt", False), "hydro": 
 
 
 def kernel_symbol(kernel, mixed):
+    """The device symbol of an interaction kernel (k_hydro, k_hydro_mixed, ...),
+    as rocprofv3 and profiles/hbm_traffic.json name it."""
     sym, always64 = KERNEL_SYMBOL[kernel]
-    return f"{sym}_{'mixed' if mixed and not always64 else 'fp64'}"
+    return f"{sym}_mixed" if mixed and not always64 else sym
 STAGES = ["kick_drift", "sort_tree", "density_x4", "eos", "derivative", "hydro", "gravity", "final_dt"]
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
