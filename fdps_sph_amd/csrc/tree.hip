@@ -384,14 +384,20 @@ __global__ void k_sys_root(int64_t cnt_top, int64_t off_top, const int32_t *nsta
 // i-groups: inside node q of the packing level, consecutive leaves are packed
 // greedily while a group stays within n_group particles; flags mark each
 // group's first particle
-__global__ void k_pack_groups(int64_t cnt_l, int64_t off_l, const int32_t *nstart, const int32_t *nend, int nleaf,
-                              int n_group, uint8_t *flags) {
+__global__ void k_pack_groups(int64_t cnt_l, int64_t off_l, const int32_t *nstart, const int32_t *nend,
+                              const int32_t *c0, const int32_t *c1, int depth, int n_group, uint8_t *flags) {
     const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= cnt_l) return;
-    const int64_t id = off_l + q;
-    const int a = lb_i32(nstart, nleaf, nstart[id]);
-    const int b = lb_i32(nstart, nleaf, nend[id]);
+    // the node's leaves: down `depth` levels along the first and the last child
+    // (every node has at least one child); leaves are nodes 0 .. of level 0
+    int a = (int) (off_l + q), last = a;
+    for (int l = 0; l < depth; ++l) {
+        a = c0[a];
+        last = c1[last] - 1;
+    }
+    const int b = last + 1;
     int run = 0;
+#pragma unroll 8
     for (int c = a; c < b; ++c) {
         const int sz = nend[c] - nstart[c];
         if (c == a || run + sz > n_group) {
@@ -904,7 +910,7 @@ static void build_levels(Tree &t, const TreeParams &tp, int64_t n, const PArrays
     SPH_LAUNCH_CHECK();
     const int64_t cpl = cnts[pl];
     hipLaunchKernelGGL(k_pack_groups, dim3(nblk(cpl, 64)), dim3(64), 0, s, cpl, t.level_off[pl], t.nstart.get(),
-                       t.nend.get(), (int) cnts[0], tp.n_group, t.flags.get());
+                       t.nend.get(), t.c0.get(), t.c1.get(), pl, tp.n_group, t.flags.get());
     SPH_LAUNCH_CHECK();
     size_t bytes = t.sel_tmp_bytes;
     SPH_HIP(hipcub::DeviceSelect::Flagged(t.sel_tmp.get(), bytes, it, t.flags.get(), t.sel_out.get() + (size_t) L * n,
