@@ -915,6 +915,8 @@ void sphgpu_sim_default_params(sphgpu_sim_params *p) {
     p->list_cap_epj = 8192;
     p->eos_density_fp64 = 1;
     p->list_cap_spj = 8192;
+    p->list_cap_front = 4096;
+    p->list_front_lds = kFrontLds;
 }
 
 int32_t sphgpu_sim_init(sphgpu_ctx *c, const sphgpu_sim_params *prm, int64_t n, const double *pos, const double *vel,
@@ -971,6 +973,10 @@ int32_t sphgpu_sim_init(sphgpu_ctx *c, const sphgpu_sim_params *prm, int64_t n, 
         S->lsph.cap = prm->list_cap_sph > 0 ? prm->list_cap_sph : 4096;
         S->lgrav.cap = prm->list_cap_epj > 0 ? prm->list_cap_epj : 8192;
         S->lgrav.scap = prm->list_cap_spj > 0 ? prm->list_cap_spj : 8192;
+        for (Lists *L : {&S->lsph, &S->lgrav}) {
+            L->fcap = prm->list_cap_front > 0 ? prm->list_cap_front : 4096;
+            L->flds = (prm->list_front_lds > 0 && prm->list_front_lds < kFrontLds) ? prm->list_front_lds : kFrontLds;
+        }
         sim_bind(*S);
         hipStream_t s = c->stream;
         PArrays &P = S->P;

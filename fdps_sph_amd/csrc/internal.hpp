@@ -247,12 +247,15 @@ void tree_sort_and_build(Tree &t, const TreeParams &tp, int64_t n, PArrays &P, P
 void tree_build_only(Tree &t, const TreeParams &tp, int64_t n, const PArrays &P, hipStream_t s);
 void tree_update_maxH(Tree &t, const double *h, hipStream_t s);
 
+// BFS frontier entries a list-walk wave keeps in LDS (two buffers of this many)
+constexpr int kFrontLds = 512;
 struct Lists {
     DevBuf<int32_t> idx, cnt;
     DevBuf<int32_t> sidx, scnt;
     int64_t cap = 0, scap = 0;
     DevBuf<int32_t> frontier;
-    int64_t fcap = 0;        // per-workgroup BFS frontier capacity
+    int64_t fcap = 0;        // per-workgroup BFS frontier capacity in global memory
+    int flds = kFrontLds;    // frontier entries kept in LDS before the global part
     DevBuf<int32_t> err;       // overflow bits, construction counter, per-XCD group counters
     DevBuf<int32_t> flag;      // gather-list validity: 1 when some group outgrew its list
     PinnedBuf<int32_t> hflag;  // host copy of the overflow bits and the construction counter

@@ -436,6 +436,7 @@ struct ListArgs {
     int64_t scap;
     int32_t *frontier;
     int64_t fcap;
+    int flds;  // frontier entries in LDS (<= kFrontLds)
     double theta2;
     double hfac;    // gather search radius = 2.5 h hfac (hfac > 1 makes a reusable superset list)
     int32_t *queue; // 8 per-XCD group counters, zero at launch
@@ -564,16 +565,33 @@ __device__ __forceinline__ GroupQueue xcd_queue(int ngroups, int32_t *ctr8) {
     return Q;
 }
 
+// A BFS frontier of one wave: its first n (<= kFrontLds) entries live in LDS,
+// the rest in the workgroup's slice of global scratch (fcap entries).  Most
+// frontiers fit in LDS, so a level's node ids are neither written back to L2
+// nor fetched from it before the node data they index can be loaded.
+struct Front {
+    int32_t *s;  // LDS part
+    int32_t *g;  // global part
+    int n;       // entries in the LDS part
+    __device__ __forceinline__ int32_t get(int i) const { return i < n ? s[i] : g[i - n]; }
+    __device__ __forceinline__ void put(int i, int32_t v) const {
+        if (i < n) s[i] = v;
+        else g[i - n] = v;
+    }
+};
+__device__ __forceinline__ int64_t front_cap(const ListArgs &a) { return a.flds + a.fcap; }
+
 // expand the frontier of one level: children of the nodes that pass `test`
 template <class Test>
-__device__ __forceinline__ int expand_level(const ListArgs &a, int32_t *fa, int nf, int32_t *fb, Test test) {
+__device__ __forceinline__ int expand_level(const ListArgs &a, const Front &fa, int nf, const Front &fb, Test test) {
     const int lane = threadIdx.x;
+    const int64_t fcap = front_cap(a);
     int nn = 0;
     for (int base = 0; base < nf; base += kWave) {
         const int idx = base + lane;
         int nc = 0, cfirst = 0;
         if (idx < nf) {
-            const int node = fa[idx];
+            const int node = fa.get(idx);
             const int32_t c0 = a.c0[node], c1 = a.c1[node];  // loaded beside the node's box, not after its test
             if (test(node)) { cfirst = c0; nc = c1 - c0; }
         }
@@ -581,12 +599,12 @@ __device__ __forceinline__ int expand_level(const ListArgs &a, int32_t *fa, int 
         const int tot = __shfl(incl, kWave - 1, kWave);
         const int pos0 = nn + incl - nc;
         for (int c = 0; c < nc; ++c)
-            if (pos0 + c < a.fcap) fb[pos0 + c] = cfirst + c;
+            if (pos0 + c < fcap) fb.put(pos0 + c, cfirst + c);
         nn += tot;
     }
-    if (nn > a.fcap) {
+    if (nn > fcap) {
         if (lane == 0) atomicOr(a.err, 4);
-        nn = (int) a.fcap;
+        nn = (int) fcap;
     }
     __syncthreads();
     return nn;
@@ -646,14 +664,14 @@ __device__ __forceinline__ int emit_chunk(const ListArgs &a, int first, int sz, 
 // emit the particles of the leaves fa[0..nf) that pass leaf_ok, filtered by
 // part_ok, in leaf order then index order
 template <class LeafOk, class PartOk>
-__device__ __forceinline__ int emit_particles(const ListArgs &a, const int32_t *fa, int nf, int32_t *out, int64_t cap,
+__device__ __forceinline__ int emit_particles(const ListArgs &a, const Front &fa, int nf, int32_t *out, int64_t cap,
                                               int count, LeafOk leaf_ok, PartOk part_ok, int *s_pref, int *s_first) {
     const int lane = threadIdx.x;
     for (int base = 0; base < nf; base += kWave) {
         const int idx = base + lane;
         int sz = 0, first = 0;
         if (idx < nf) {
-            const int leaf = fa[idx];
+            const int leaf = fa.get(idx);
             const int32_t st = a.nstart[leaf], en = a.nend[leaf];  // loaded beside the leaf's box
             if (leaf_ok(leaf)) { first = st; sz = en - st; }
         }
@@ -670,13 +688,14 @@ __global__ __launch_bounds__(kWave) void k_list_sph(ListArgs a) {
     int32_t *fb0 = fa0 + a.fcap;
     __shared__ int s_pref[kWave + 1];
     __shared__ int s_first[kWave];
+    __shared__ int32_t s_front[2 * kFrontLds];
     const GroupQueue Q = xcd_queue(a.ngroups, a.queue);
     for (int g = Q.next(); g < Q.end; g = Q.next()) {
         const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
         GBox b;
         group_box(a, i0, ni, b);
-        int32_t *fa = fa0, *fb = fb0;
-        if (lane == 0) fa[0] = a.sys_root[a.sys[i0]];
+        Front fa{s_front, fa0, a.flds}, fb{s_front + kFrontLds, fb0, a.flds};
+        if (lane == 0) fa.put(0, a.sys_root[a.sys[i0]]);
         if (a.sbox && lane < 6) {  // selects, not an index by lane: the box stays in registers
             const double v = (lane == 0) ? b.slo[0] : (lane == 1) ? b.slo[1] : (lane == 2) ? b.slo[2]
                            : (lane == 3) ? b.shi[0] : (lane == 4) ? b.shi[1] : b.shi[2];
@@ -687,7 +706,7 @@ __global__ __launch_bounds__(kWave) void k_list_sph(ListArgs a) {
         auto test = [&](int node) { return node_test<KIND>(a, node, b); };
         for (int l = a.nlevels - 1; l >= 1; --l) {
             nf = expand_level(a, fa, nf, fb, test);
-            int32_t *t = fa; fa = fb; fb = t;
+            const Front t = fa; fa = fb; fb = t;
         }
         int32_t *out = a.out + (int64_t) g * a.cap;
         auto ptest = [&](int p) { return part_test<KIND>(a, p, b); };
@@ -708,13 +727,15 @@ __global__ __launch_bounds__(kWave) void k_list_grav(ListArgs a) {
     int32_t *fb0 = fa0 + a.fcap;
     __shared__ int s_pref[kWave + 1];
     __shared__ int s_first[kWave];
+    __shared__ int32_t s_front[2 * kFrontLds];
+    const int64_t fcap = front_cap(a);
     const GroupQueue Q = xcd_queue(a.ngroups, a.queue);
     for (int g = Q.next(); g < Q.end; g = Q.next()) {
         const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
         GBox b;
         group_box(a, i0, ni, b);
-        int32_t *fa = fa0, *fb = fb0;
-        if (lane == 0) fa[0] = a.sys_root[a.sys[i0]];
+        Front fa{s_front, fa0, a.flds}, fb{s_front + kFrontLds, fb0, a.flds};
+        if (lane == 0) fa.put(0, a.sys_root[a.sys[i0]]);
         __syncthreads();
         int nf = 1, cj = 0, cs = 0;
         int32_t *jout = a.out + (int64_t) g * a.cap;
@@ -728,7 +749,7 @@ __global__ __launch_bounds__(kWave) void k_list_grav(ListArgs a) {
             int nn = 0;
             for (int base = 0; base < nf; base += kWave) {
                 const int idx = base + lane;
-                const int node = (idx < nf) ? fa[idx] : -1;
+                const int node = (idx < nf) ? fa.get(idx) : -1;
                 const bool acc = (node >= 0) && accept_node(a, node, b);
                 const uint64_t m = __ballot(acc);
                 if (acc) {
@@ -746,7 +767,7 @@ __global__ __launch_bounds__(kWave) void k_list_grav(ListArgs a) {
                     const int incl = wave_incl_scan(nc);
                     const int pos0 = nn + incl - nc;
                     for (int c = 0; c < nc; ++c)
-                        if (pos0 + c < a.fcap) fb[pos0 + c] = cfirst + c;
+                        if (pos0 + c < fcap) fb.put(pos0 + c, cfirst + c);
                     nn += __shfl(incl, kWave - 1, kWave);
                 } else {
                     int sz = 0, first = 0;
@@ -758,13 +779,13 @@ __global__ __launch_bounds__(kWave) void k_list_grav(ListArgs a) {
                 }
             }
             if (l >= 1) {
-                if (nn > a.fcap) {
+                if (nn > fcap) {
                     if (lane == 0) atomicOr(a.err, 4);
-                    nn = (int) a.fcap;
+                    nn = (int) fcap;
                 }
                 __syncthreads();
                 nf = nn;
-                int32_t *t = fa; fa = fb; fb = t;
+                const Front t = fa; fa = fb; fb = t;
             }
         }
         if (lane == 0) {
@@ -1005,6 +1026,7 @@ static ListArgs list_args(const Tree &t, const TreeParams &tp, const PArrays &P,
     a.sout = L.sidx.get(); a.scnt = L.scnt.get(); a.scap = L.scap;
     a.frontier = L.frontier.get();
     a.fcap = L.fcap;
+    a.flds = std::min(std::max(L.flds, 1), kFrontLds);
     a.theta2 = tp.theta * tp.theta;
     a.hfac = 1.0;
     a.sbox = nullptr;

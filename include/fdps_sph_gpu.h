@@ -152,6 +152,10 @@ typedef struct {
                                  density feeds the EoS, in FP64 (1, default) or FP32 (0).
                                  The EoS tables amplify density errors by
                                  d ln P / d ln rho (20 and more near the cold curve). */
+    int32_t list_cap_front;  /* per-workgroup tree-walk frontier capacity in global memory,
+                                default 4096 (grown on overflow like the list capacities) */
+    int32_t list_front_lds;  /* frontier entries a tree walk keeps in LDS before it spills
+                                to global memory (1..512), default 512 */
 } sphgpu_sim_params;
 
 void sphgpu_sim_default_params(sphgpu_sim_params *p);

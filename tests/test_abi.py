@@ -46,6 +46,7 @@ def test_errors_without_context():
     p = _lib.SimParams()
     L.sphgpu_sim_default_params(ctypes.byref(p))
     assert p.n_group == 64 and p.n_leaf == 16 and p.theta == 0.5
+    assert p.list_cap_front == 4096 and p.list_front_lds == 512
 
 
 def test_regrid_keeps_nodes_and_axes():

@@ -323,3 +323,23 @@ def test_list_overflow_redo(ctx64):
     assert big.list_builds() == small.list_builds()
     for f in ("POS", "VEL", "DENS", "SMTH", "ENG", "ACC", "GRAV", "ENGDOT", "DT", "ID"):
         assert np.array_equal(big.get(f), small.get(f)), f
+
+
+def test_frontier_spill_and_overflow(ctx64):
+    """Tree walks keep their frontier in LDS and continue it in global memory.
+    With one LDS entry every walk runs through the global part; with a global
+    part of 4 entries as well every force evaluation overflows the frontier,
+    grows it and runs again.  Both give the default run's results bit for
+    bit."""
+    ic = IC.granite_impact(9000, 1000, seed=1002)
+    ref = Simulation(ctx64, ic)
+    spill = Simulation(ctx64, ic, list_front_lds=1)
+    tiny = Simulation(ctx64, ic, list_front_lds=1, list_cap_front=4)
+    for sim in (ref, spill, tiny):
+        sim.step(2)
+    _print(f"[frontier spill] builds {ref.list_builds()} {spill.list_builds()} {tiny.list_builds()}")
+    assert ref.list_builds() == spill.list_builds() == tiny.list_builds()
+    for f in ("POS", "VEL", "DENS", "SMTH", "ENG", "ACC", "GRAV", "ENGDOT", "DT", "ID", "NGB"):
+        a = ref.get(f)
+        assert np.array_equal(a, spill.get(f)), f
+        assert np.array_equal(a, tiny.get(f)), f
