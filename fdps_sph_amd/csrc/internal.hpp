@@ -323,6 +323,22 @@ struct Tiles {
     __device__ __forceinline__ int count(int t) const { return (nj - t + T - 1) / T; }
 };
 
+// Dot-form mask threshold of one lane for one tile (see dot_mask): the
+// in-support test |x_j - x_i|^2 <= H^2 evaluated through FP32 offsets needs
+// room for the offsets' rounding (radius + 2^-22 (|o_i|_1 + N)) and for the
+// FMA chain's (2^-19 (|o_i|_1^2 + N^2)), N = largest |o_j|_1 of the tile.
+__device__ __forceinline__ float dot_cut_(float Hc, float wi, float ni, float nmax) {
+    const float r = Hc + 2.384185791015625e-7f * (ni + nmax);
+    return fmaf(r, r, -wi) + 1.9073486328125e-6f * (ni * ni + nmax * nmax);
+}
+// The same room for j's support tested from any lane of the group: Q_j for
+// mfma_dot_mask_sym with n1max the group's largest |o_i|_1.
+__device__ __forceinline__ float dot_cut_j_(float Hcj, float nj, float n1max) {
+    const float R = Hcj + 2.384185791015625e-7f * (n1max + nj);
+    return fmaf(R, R, 1.9073486328125e-6f * (n1max * n1max + nj * nj));
+}
+__device__ __forceinline__ float norm1_(float ox, float oy, float oz) { return fabsf(ox) + fabsf(oy) + fabsf(oz); }
+
 // Support mask of one j-tile in dot form.  q[jj] = (o.x, o.y, o.z, w) holds
 // FP32 offsets o_j of the tile's entries from the group centre and
 // w = |o_j|^2; with t = (-2 o_i) . o_j + w = |o_j - o_i|^2 - |o_i|^2, bit jj
@@ -345,32 +361,6 @@ __device__ __forceinline__ uint64_t dot_mask(const float4 *q, int nt, float tx, 
     } else {
         for (int jj = nt - 1; jj >= 32; --jj) hi = dot_bit_(hi, q[jj], tx, ty, tz, c);
         for (int jj = (nt < 32 ? nt : 32) - 1; jj >= 0; --jj) lo = dot_bit_(lo, q[jj], tx, ty, tz, c);
-    }
-    return ((uint64_t) hi << 32) | lo;
-}
-
-// Symmetric form (inside either support): the bit is the sign of
-// t + min(-c, w - Q_j), t = |o_j|^2 - 2 o_i.o_j as above, c the lane's own
-// threshold, w = |o_i|^2 and Q_j the threshold of j's support (qs[jj]):
-// t <= c, or t + w <= Q_j.  A lane with c = -inf and w = +inf admits nothing.
-__device__ __forceinline__ uint32_t dot_bit_sym_(uint32_t m, const float4 &q, float Qj, float tx, float ty, float tz,
-                                                 float negc, float w) {
-    float t = fmaf(tx, q.x, q.w);
-    t = fmaf(ty, q.y, t);
-    t = fmaf(tz, q.z, t);
-    return __builtin_amdgcn_alignbit(m, __float_as_uint(t + fminf(negc, w - Qj)), 31);
-}
-__device__ __forceinline__ uint64_t dot_mask_sym(const float4 *q, const float *qs, int nt, float tx, float ty,
-                                                 float tz, float negc, float w) {
-    uint32_t lo = 0, hi = 0;
-    if (nt == kWave) {  // unrolled in part: the full unroll's hoisted LDS reads cost a wave of occupancy
-#pragma unroll 16
-        for (int jj = kWave - 1; jj >= 32; --jj) hi = dot_bit_sym_(hi, q[jj], qs[jj], tx, ty, tz, negc, w);
-#pragma unroll 16
-        for (int jj = 31; jj >= 0; --jj) lo = dot_bit_sym_(lo, q[jj], qs[jj], tx, ty, tz, negc, w);
-    } else {
-        for (int jj = nt - 1; jj >= 32; --jj) hi = dot_bit_sym_(hi, q[jj], qs[jj], tx, ty, tz, negc, w);
-        for (int jj = (nt < 32 ? nt : 32) - 1; jj >= 0; --jj) lo = dot_bit_sym_(lo, q[jj], qs[jj], tx, ty, tz, negc, w);
     }
     return ((uint64_t) hi << 32) | lo;
 }
@@ -434,6 +424,61 @@ __device__ __forceinline__ uint64_t mfma_dot_mask(const float4 *q, int nt, float
     const uint32_t w11 = sign_bits16_(c);
     const uint32_t own0 = h ? w01 : w00, own1 = h ? w11 : w10;  // column block h: this lane's particle
     const uint32_t par0 = h ? w00 : w01, par1 = h ? w10 : w11;  // the other: the partner's
+    const uint32_t recv = (uint32_t) __shfl_xor((int) (par0 | (par1 << 16)), 32, kWave);
+    const int so_ = 16 * h, sp_ = 16 - so_;
+    const uint32_t lo = (own0 << so_) | ((recv & 0xffffu) << sp_);
+    const uint32_t hi = (own1 << so_) | ((recv >> 16) << sp_);
+    uint64_t mask = ((uint64_t) hi << 32) | lo;
+    if (nt < kWave) mask &= (1ULL << nt) - 1;
+    return mask;
+}
+
+// Symmetric form from the matrix cores (inside either support): the bit is
+// set when t <= c_i or t + w_i <= Q_j, t = |o_j|^2 - 2 o_i.o_j as above, c_i
+// the lane's own threshold, w_i = |o_i|^2 and Q_j the threshold of j's support
+// (qs[jj], see dot_cut_j_).  Rows (o_j, w_j, 1, -Q_j) against the columns
+// (-2 o_i, 1, -c_i, 0) give C1 = t - c_i; adding the columns (d_i, 1) of the
+// last pair to C1 in place gives C2 = C1 + d_i - Q_j = t + w_i - Q_j with
+// d_i = w_i + c_i.  Four MFMAs per 32 x 32 block (sixteen per tile) into one
+// set of 16 accumulators, and a bit is set when either result's sign is.  The
+// chained sum adds one rounding of the magnitudes already covered by the
+// thresholds' room.  Layout and operands as in mfma_dot_mask; qs[jj] = Q_j,
+// d = d_i of this lane (0 where negc = +inf, which admits nothing: C2 stays
+// +inf) and pd that of lane l^32.
+__device__ __forceinline__ uint32_t sym_block_(float a0, float a1, float a2, float b0, float b1, float b2, float b3) {
+    f32x16_ c = f32x16_{};
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, b2, c, 0, 0, 0);
+    const uint32_t own = sign_bits16_(c);
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, b3, c, 0, 0, 0);
+    return own | sign_bits16_(c);
+}
+__device__ __forceinline__ uint64_t mfma_dot_mask_sym(const float4 *q, const float *qs, int nt, float tx, float ty,
+                                                      float tz, float negc, float d, float px, float py, float pz,
+                                                      float pc, float pd) {
+    const int lane = threadIdx.x, h = lane >> 5, m = lane & 31;
+    // j operands: half 0 supplies k = 0, 2, 4 (o_x, o_z, 1), half 1 k = 1, 3, 5 (o_y, w_j, 0 for C1 and
+    // -Q_j for C2: C1's k = 5 column entry is 0, so the row may carry -Q_j throughout)
+    const int e = 16 * ((m >> 2) & 1) + 4 * (m >> 3) + (m & 3);
+    const float4 q0 = q[e], q1 = q[32 + e];
+    const float a00 = h ? q0.y : q0.x, a01 = h ? q0.w : q0.z, a02 = h ? -qs[e] : 1.f;
+    const float a10 = h ? q1.y : q1.x, a11 = h ? q1.w : q1.z, a12 = h ? -qs[32 + e] : 1.f;
+    // i operands: half 0 supplies k = 0, 2 (-2 o_x, -2 o_z) and k = 4 (-c_i; then d_i), half 1 k = 1, 3
+    // (-2 o_y, 1) and k = 5 (0; then 1)
+    const float b00 = h ? py : tx, b01 = h ? 1.f : tz, b02 = h ? 0.f : negc, b03 = h ? 1.f : d;
+    const float b10 = h ? ty : px, b11 = h ? 1.f : pz, b12 = h ? 0.f : pc, b13 = h ? 1.f : pd;
+    // one block at a time (scheduling barriers), the form measured in the
+    // hydro kernels; interleaving the four blocks saves no registers there
+    const uint32_t w00 = sym_block_(a00, a01, a02, b00, b01, b02, b03);
+    __builtin_amdgcn_sched_barrier(0);
+    const uint32_t w10 = sym_block_(a10, a11, a12, b00, b01, b02, b03);
+    __builtin_amdgcn_sched_barrier(0);
+    const uint32_t w01 = sym_block_(a00, a01, a02, b10, b11, b12, b13);
+    __builtin_amdgcn_sched_barrier(0);
+    const uint32_t w11 = sym_block_(a10, a11, a12, b10, b11, b12, b13);
+    const uint32_t own0 = h ? w01 : w00, own1 = h ? w11 : w10;
+    const uint32_t par0 = h ? w00 : w01, par1 = h ? w10 : w11;
     const uint32_t recv = (uint32_t) __shfl_xor((int) (par0 | (par1 << 16)), 32, kWave);
     const int so_ = 16 * h, sp_ = 16 - so_;
     const uint32_t lo = (own0 << so_) | ((recv & 0xffffu) << sp_);

@@ -94,15 +94,6 @@ __device__ __forceinline__ Frame group_frame(double x, double y, double z, bool 
     const double lz = wave_min_(act ? z : INFINITY), hz = wave_max_(act ? z : -INFINITY);
     return {0.5 * (lx + hx), 0.5 * (ly + hy), 0.5 * (lz + hz)};
 }
-// Dot-form mask threshold of one lane for one tile (see dot_mask): the
-// in-support test |x_j - x_i|^2 <= H^2 evaluated through FP32 offsets needs
-// room for the offsets' rounding (radius + 2^-22 (|o_i|_1 + N)) and for the
-// FMA chain's (2^-19 (|o_i|_1^2 + N^2)), N = largest |o_j|_1 of the tile.
-__device__ __forceinline__ float dot_cut_(float Hc, float wi, float ni, float nmax) {
-    const float r = Hc + 2.384185791015625e-7f * (ni + nmax);
-    return fmaf(r, r, -wi) + 1.9073486328125e-6f * (ni * ni + nmax * nmax);
-}
-__device__ __forceinline__ float norm1_(float ox, float oy, float oz) { return fabsf(ox) + fabsf(oy) + fabsf(oz); }
 __device__ __forceinline__ float wave_maxf_(float v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
@@ -310,7 +301,9 @@ __global__ void k_pack_hydro(int64_t n, const double *x, const double *y, const 
     if (k < n) out[k] = make_hydro_rec(x[k], y[k], z[k], vx[k], vy[k], vz[k], m[k], rho[k], cs[k], bal[k], gs[k],
                                        p[k], h[k]);
 }
-__global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
+// 4 waves per SIMD (128 registers, seven spilled outside the pair loops):
+// measured 2.09 ms at 1M against 2.14 ms at the compiler's 3 waves.
+__global__ __launch_bounds__(kWave, 4) void k_hydro(HydroArgs a) {
     const int g = xcd_remap(blockIdx.x, a.ngroups);
     const int lane = threadIdx.x;
     const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
@@ -329,13 +322,13 @@ __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
     const Tiles tl(a.L.cnt[g]);
     const Frame fr = group_frame(xi, yi, zi, act);
     const float oxi = (float) (xi - fr.cx), oyi = (float) (yi - fr.cy), ozi = (float) (zi - fr.cz);
-    // dot-form support tests (see dot_mask_sym): the lane's own threshold per
+    // dot-form support tests (see mfma_dot_mask_sym): the lane's own threshold per
     // tile from dot_cut_, j's threshold Q_j with the group's largest |o_i|_1
     const float Hci = (float) Hi * 1.0001f;
     const float wi = fmaf(oxi, oxi, fmaf(oyi, oyi, ozi * ozi)), n1i = norm1_(oxi, oyi, ozi);
     const float n1max = wave_maxf_(act ? n1i : 0.f);
-    const float wlane = act ? wi : INFINITY;
     const float tx = -2.f * oxi, ty = -2.f * oyi, tz = -2.f * ozi;
+    const float ptx = __shfl_xor(tx, 32, kWave), pty = __shfl_xor(ty, 32, kWave), ptz = __shfl_xor(tz, 32, kWave);
     // v_sig bound of pass 1: with v0 the centre of the group's velocity box,
     // c_i + c_j - 3w <= c_i + c_j + 3|v_i - v_j| <= (c_i + 3|v_i - v0|) + (c_j + 3|v_j - v0|)
     const double cvx = 0.5 * (wave_min_(act ? vxi : INFINITY) + wave_max_(act ? vxi : -INFINITY));
@@ -361,11 +354,11 @@ __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
             const float ox = (float) (r.x - fr.cx), oy = (float) (r.y - fr.cy), oz = (float) (r.z - fr.cz);
             so[lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
             nj1 = norm1_(ox, oy, oz);
-            // as dot_cut_ with the roles of i and j exchanged, for the group's largest |o_i|_1
-            const float R = (float) r.H * 1.0001f + 2.384185791015625e-7f * (n1max + nj1);
-            sQ[lane] = fmaf(R, R, 1.9073486328125e-6f * (n1max * n1max + nj1 * nj1));
+            sQ[lane] = dot_cut_j_((float) r.H * 1.0001f, nj1, n1max);
         }
         const float negc = act ? -dot_cut_(Hci, wi, n1i, wave_maxf_(nj1)) : INFINITY;
+        const float pnegc = __shfl_xor(negc, 32, kWave);
+        const float dsym = act ? wi - negc : 0.f, pdsym = __shfl_xor(dsym, 32, kWave);
         // the entries that can raise some lane's maximum: b_j >= T, T the least
         // over the lanes of vmax_i (1 - 1e-10) - b_i.  For every other entry
         // b_i + b_j stays below each lane's maximum by a margin far above the
@@ -401,7 +394,7 @@ __global__ __launch_bounds__(kWave) void k_hydro(HydroArgs a) {
         // F_i = norm_i s(1 + 7s + 16s^2)(1-s)^7 / (r H_i + 1e-6 h_i); the two
         // reciprocals come from one, 1/(a_i a_j).  AVh is half the viscosity
         // AV of the reference, and the sums leave out K.
-        const uint64_t mask = dot_mask_sym(so, sQ, nt, tx, ty, tz, negc, wlane);
+        const uint64_t mask = mfma_dot_mask_sym(so, sQ, nt, tx, ty, tz, negc, dsym, ptx, pty, ptz, pnegc, pdsym);
         for (uint64_t msk = mask; msk; msk &= msk - 1) {
             const HydroJ &J = sj[__builtin_ctzll(msk)];
             const double drx = xi - J.x, dry = yi - J.y, drz = zi - J.z;

@@ -260,8 +260,9 @@ __global__ __launch_bounds__(kWave) void k_drvt_mixed(DrvtArgs a) {
 }
 
 // ---------------------------------------------------------------- hydro force
-// At least 5 waves per SIMD (four small spills; measured 1.66 -> 1.48 ms at 1M).
-__global__ __launch_bounds__(kWave, 5) void k_hydro_mixed(HydroArgs a) {
+// 4 waves per SIMD: at 5 waves (96 registers) 27 spill, and the kernel took
+// 1.33 ms at 1M against 1.20 ms at 4 waves (128 registers, two spilled).
+__global__ __launch_bounds__(kWave, 4) void k_hydro_mixed(HydroArgs a) {
     const int g = xcd_remap(blockIdx.x, a.ngroups);
     const int lane = threadIdx.x;
     const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
@@ -277,7 +278,13 @@ __global__ __launch_bounds__(kWave, 5) void k_hydro_mixed(HydroArgs a) {
     const KernelConsts kc = kernel_consts(hid);
     const float Hi = kc.H, invHi = kc.invH, normi = kc.norm;
     const float regi = (float) (1.0e-6 * hid);
-    const float rcut2i = act ? Hi * Hi * 1.0001f : -1.f;
+    // dot-form support tests from the matrix cores (mfma_dot_mask_sym), with
+    // the thresholds that make them a superset of the in-support pairs
+    const float Hci = (float) (kSupport * hid) * 1.0001f;
+    const float wi = fmaf(xi, xi, fmaf(yi, yi, zi * zi)), n1i = norm1_(xi, yi, zi);
+    const float n1max = wave_maxf(act ? n1i : 0.f);
+    const float tx = -2.f * xi, ty = -2.f * yi, tz = -2.f * zi;
+    const float ptx = __shfl_xor(tx, 32, kWave), pty = __shfl_xor(ty, 32, kWave), ptz = __shfl_xor(tz, 32, kWave);
     const float rhoi = (float) rhoid, csi = (float) a.csi[ip], bali = (float) a.bali[ip];
     const float fAi = (float) (a.gsi[ip] * a.pi[ip] / (rhoid * rhoid));  // f_i P_i / rho_i^2
     const float gsi = (float) a.gsi[ip];
@@ -287,13 +294,14 @@ __global__ __launch_bounds__(kWave, 5) void k_hydro_mixed(HydroArgs a) {
     float vmax = 0.f;
     const int64_t lb = list_begin(a.L, g);
     const int nj = a.L.cnt[g];
-    __shared__ float sx[kWave], sy[kWave], sz[kWave], svx[kWave], svy[kWave], svz[kWave], sm[kWave];
+    __shared__ float4 so[kWave];  // FP32 offset from the centre, |offset|^2
+    __shared__ float svx[kWave], svy[kWave], svz[kWave], sm[kWave], sQ[kWave];
     __shared__ float srho[kWave], scs[kWave], sbal[kWave], sgs[kWave], sA[kWave], sinvH[kWave], sH[kWave],
-        sreg[kWave], snorm[kWave], scut[kWave];
+        sreg[kWave], snorm[kWave];
     const Tiles tl(nj);
     for (int t = 0; t < tl.T; ++t) {
         const int k = tl.entry(t, lane);
-        float bnd = 0.f;
+        float bnd = 0.f, nj1 = 0.f;
         if (k < nj) {
             const int q = a.L.idx[lb + k];
             double x, y, z, vx, vy, vz, m, rj, cj, bj, gj, Aj, regj, Hj, nj3;
@@ -310,13 +318,15 @@ __global__ __launch_bounds__(kWave, 5) void k_hydro_mixed(HydroArgs a) {
                 Hj = kSupport * hj;
                 nj3 = kNormD / (Hj * Hj * Hj);
             }
-            sx[lane] = (float) (x - cx);
-            sy[lane] = (float) (y - cy);
-            sz[lane] = (float) (z - cz);
-            svx[lane] = (float) (vx - cvx);
-            svy[lane] = (float) (vy - cvy);
-            svz[lane] = (float) (vz - cvz);
-            bnd = (float) cj + 3.f * sqrtf(svx[lane] * svx[lane] + svy[lane] * svy[lane] + svz[lane] * svz[lane]);
+            const float ox = (float) (x - cx), oy = (float) (y - cy), oz = (float) (z - cz);
+            so[lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
+            nj1 = norm1_(ox, oy, oz);
+            sQ[lane] = dot_cut_j_((float) Hj * 1.0001f, nj1, n1max);
+            const float ux = (float) (vx - cvx), uy = (float) (vy - cvy), uz = (float) (vz - cvz);
+            svx[lane] = ux;
+            svy[lane] = uy;
+            svz[lane] = uz;
+            bnd = (float) cj + 3.f * sqrtf(ux * ux + uy * uy + uz * uz);
             sm[lane] = (float) m;
             srho[lane] = (float) rj;
             scs[lane] = (float) cj;
@@ -327,8 +337,10 @@ __global__ __launch_bounds__(kWave, 5) void k_hydro_mixed(HydroArgs a) {
             sinvH[lane] = (float) (1.0 / Hj);
             sreg[lane] = (float) regj;
             snorm[lane] = (float) nj3;
-            scut[lane] = (float) (Hj * Hj) * 1.0001f;
         }
+        const float negc = act ? -dot_cut_(Hci, wi, n1i, wave_maxf(nj1)) : INFINITY;
+        const float pnegc = __shfl_xor(negc, 32, kWave);
+        const float dsym = act ? wi - negc : 0.f, pdsym = __shfl_xor(dsym, 32, kWave);
         // the entries that can raise some lane's v_sig maximum: bound_j >= T,
         // T the least over the lanes of vmax_i (1 - 1e-5) - b_i (a margin far
         // above the FP32 rounding); no other entry can change any maximum
@@ -337,38 +349,24 @@ __global__ __launch_bounds__(kWave, 5) void k_hydro_mixed(HydroArgs a) {
         __syncthreads();
         const int nt = tl.count(t);
         float px = 0.f, py = 0.f, pz = 0.f, pu = 0.f;
-        // pass 1: v_sig_max and the mask of pairs inside either support
-        uint64_t mask = 0;
-        if (__popcll(cand) > 48) {  // most of the tile: one fused loop
-            for (int jj = 0; jj < nt; ++jj) {
-                const float drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
-                const float dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
-                const float dvdr = dvx * drx + dvy * dry + dvz * drz;
-                const float r2 = drx * drx + dry * dry + drz * drz;
-                const float w = (dvdr < 0.f) ? dvdr * rsq(r2 + 1e-30f) : 0.f;
-                vmax = fmaxf(vmax, csi + scs[jj] - 3.f * w);
-                mask |= (r2 < rcut2i || (act && r2 < scut[jj])) ? (1ULL << jj) : 0ULL;
-            }
-        } else {  // the mask over every entry, v_sig over the candidates only
-            for (int jj = 0; jj < nt; ++jj) {
-                const float drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
-                const float r2 = drx * drx + dry * dry + drz * drz;
-                mask |= (r2 < rcut2i || (act && r2 < scut[jj])) ? (1ULL << jj) : 0ULL;
-            }
-            for (uint64_t cm = cand; cm; cm &= cm - 1) {
-                const int jj = __builtin_ctzll(cm);
-                const float drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
-                const float dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
-                const float dvdr = dvx * drx + dvy * dry + dvz * drz;
-                const float r2 = drx * drx + dry * dry + drz * drz;
-                const float w = (dvdr < 0.f) ? dvdr * rsq(r2 + 1e-30f) : 0.f;
-                vmax = fmaxf(vmax, csi + scs[jj] - 3.f * w);
-            }
+        // pass 1: v_sig_max over the candidates
+        for (uint64_t cm = cand; cm; cm &= cm - 1) {
+            const int jj = __builtin_ctzll(cm);
+            const float4 oj = so[jj];
+            const float drx = xi - oj.x, dry = yi - oj.y, drz = zi - oj.z;
+            const float dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
+            const float dvdr = dvx * drx + dvy * dry + dvz * drz;
+            const float r2 = drx * drx + dry * dry + drz * drz;
+            const float w = (dvdr < 0.f) ? dvdr * rsq(r2 + 1e-30f) : 0.f;
+            vmax = fmaxf(vmax, csi + scs[jj] - 3.f * w);
         }
+        // the pairs inside either support (a superset: pairs beyond both give zero terms)
+        const uint64_t mask = mfma_dot_mask_sym(so, sQ, nt, tx, ty, tz, negc, dsym, ptx, pty, ptz, pnegc, pdsym);
         // pass 2: force terms of this lane's in-support pairs
         for (uint64_t msk = mask; msk; msk &= msk - 1) {
             const int jj = __builtin_ctzll(msk);
-            const float drx = xi - sx[jj], dry = yi - sy[jj], drz = zi - sz[jj];
+            const float4 oj = so[jj];
+            const float drx = xi - oj.x, dry = yi - oj.y, drz = zi - oj.z;
             const float dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
             const float dvdr = dvx * drx + dvy * dry + dvz * drz;
             const float r2 = drx * drx + dry * dry + drz * drz;
