@@ -142,6 +142,7 @@ struct SimState {
     Lists lsph, lgrav;
     DevBuf<double> dt_sys, time_sys, dt_part;
     hipEvent_t ev[kStages + 1] = {};
+    hipEvent_t xev[4] = {};  // cross-stream order of a force evaluation (no timing)
     double stage_ms[kStages] = {0};
     bool ready = false;
     // per-kernel statistics (density x4, derivative, hydro, gravity per step)
@@ -162,6 +163,8 @@ struct SimState {
     ~SimState() {
         for (auto &e : ev)
             if (e) (void) hipEventDestroy(e);
+        for (auto &e : xev)
+            if (e) (void) hipEventDestroy(e);
         for (auto &p : kev)
             for (auto &e : p)
                 if (e) (void) hipEventDestroy(e);
@@ -172,6 +175,7 @@ struct sphgpu_ctx {
     int device = 0;
     int precision = SPHGPU_PREC_FP64;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // list walks that overlap the force kernels (default priority)
     MatHost mats[SPHGPU_MAX_MATERIALS];
     DevBuf<MatDev> d_mats;
     DevBuf<int32_t> iota;
@@ -511,8 +515,24 @@ void sim_bind(SimState &S) {
 // host waits once, at the end; when some list overflowed, the capacities grow
 // and the evaluation runs again from the saved h -- the only input it
 // overwrites (the density passes update h).
+//
+// Two list walks run on a second stream, beside force kernels they do not feed:
+// the walks wait on dependent loads and leave the VALU idle, the kernels are
+// VALU-bound.  The gravity lists need only the tree (the walk reads no h), so
+// they are built while the density passes run; the symmetric lists of the hydro
+// pass need the final h and replace the gather lists, so they are built after
+// the derivative pass, while the gravity kernel runs.  With kernel statistics
+// on, everything runs in this order on one stream, so that each kernel's timing
+// is its own.
 void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
     hipStream_t s = c->stream;
+    hipStream_t s2 = S.stats ? s : c->stream2;
+    auto signal = [&](hipEvent_t e, hipStream_t from) {  // order later work on the other stream after `from`'s so far
+        if (s2 != s) SPH_HIP(hipEventRecord(e, from));
+    };
+    auto await = [&](hipStream_t on, hipEvent_t e) {
+        if (s2 != s) SPH_HIP(hipStreamWaitEvent(on, e, 0));
+    };
     const int64_t n = S.n;
     const bool mixed = c->precision == SPHGPU_PREC_MIXED;
     auto rec_ev = [&](int k) {
@@ -554,6 +574,10 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
         };
         lists_reset(L, s);
         lists_reset(Lg, s);
+        signal(S.xev[0], s);  // tree, h restored, counters cleared
+        await(s2, S.xev[0]);
+        enqueue_lists(S.tree, S.tp, P, 2, Lg, 1.0, s2);
+        signal(S.xev[1], s2);  // gravity lists
         // 4 density passes, each over lists that cover the current h
         // (src/main.cpp:138-140).  One gather list, built with a margin on h,
         // serves every pass (and the derivative pass) while each group's
@@ -588,22 +612,13 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
             kend();
         }
         rec_ev(5);
-        tree_update_maxH(S.tree, P.h, s);
-        lists_fetch(L, s, 1);  // gather-list bits and construction counter, before the symmetric lists reuse L
-        enqueue_lists(S.tree, S.tp, P, 1, L, 1.0, s);
-        {
-            HydroRec *rec = S.hrec.ensure(n);
-            launch_pack_hydro(n, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.m, P.rho, P.cs, P.bal, P.gs, P.p, P.h, rec, s);
-            HydroArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal,
-                        P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal, P.m,
-                        lref(), P.ax, P.ay, P.az, P.du, P.dti, rec};
-            kbegin(&L, false, 2);
-            if (mixed) launch_hydro_mixed(a, s);
-            else launch_hydro_fp64(a, s);
-            kend();
-        }
-        rec_ev(6);
-        enqueue_lists(S.tree, S.tp, P, 2, Lg, 1.0, s);
+        signal(S.xev[2], s);  // final h; the derivative pass is done with the gather lists
+        await(s2, S.xev[2]);
+        tree_update_maxH(S.tree, P.h, s2);
+        lists_fetch(L, s2, 1);  // gather-list bits and construction counter, before the symmetric lists reuse L
+        enqueue_lists(S.tree, S.tp, P, 1, L, 1.0, s2);
+        signal(S.xev[3], s2);  // symmetric lists
+        await(s, S.xev[1]);
         {
             const Tree &t = S.tree;
             GravArgs a{G, gs, P.x, P.y, P.z, nullptr, P.h, P.x, P.y, P.z, P.m,
@@ -614,6 +629,19 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
             kbegin(&Lg, true, 3);
             if (mixed) launch_grav_mixed(a, s);
             else launch_grav_fp64(a, s);
+            kend();
+        }
+        rec_ev(6);
+        {
+            HydroRec *rec = S.hrec.ensure(n);
+            launch_pack_hydro(n, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.m, P.rho, P.cs, P.bal, P.gs, P.p, P.h, rec, s);
+            await(s, S.xev[3]);
+            HydroArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal,
+                        P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal, P.m,
+                        lref(), P.ax, P.ay, P.az, P.du, P.dti, rec};
+            kbegin(&L, false, 2);
+            if (mixed) launch_hydro_mixed(a, s);
+            else launch_hydro_fp64(a, s);
             kend();
         }
         rec_ev(7);
@@ -670,6 +698,7 @@ int32_t sphgpu_create(sphgpu_ctx **ctx, int32_t device, int32_t precision) {
         c->precision = precision;
         SPH_HIP(hipSetDevice(device));
         SPH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        SPH_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
         upload_mats(c.get());
         *ctx = c.release();
     });
@@ -680,10 +709,12 @@ int32_t sphgpu_destroy(sphgpu_ctx *ctx) {
         if (!ctx) return;
         (void) hipSetDevice(ctx->device);
         (void) hipStreamSynchronize(ctx->stream);
+        if (ctx->stream2) (void) hipStreamSynchronize(ctx->stream2);
         ctx->sim.reset();
-        hipStream_t s = ctx->stream;
+        hipStream_t s = ctx->stream, s2 = ctx->stream2;
         delete ctx;
         (void) hipStreamDestroy(s);
+        if (s2) (void) hipStreamDestroy(s2);
     });
 }
 
@@ -1006,6 +1037,7 @@ int32_t sphgpu_sim_init(sphgpu_ctx *c, const sphgpu_sim_params *prm, int64_t n, 
         const int nchunk = (int) std::max<int64_t>(1, (maxsys + 4095) / 4096);
         S->dt_part.ensure((size_t) nsys * nchunk);
         for (auto &e : S->ev) SPH_HIP(hipEventCreate(&e));
+        for (auto &e : S->xev) SPH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         // src/main.cpp:69-104: initialize, forces, dt, external force
         launch_init_h(n, P, s);
         sim_forces(c, *S, false);

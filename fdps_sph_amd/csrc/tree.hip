@@ -484,7 +484,7 @@ __device__ __forceinline__ void group_box(const ListArgs &a, int i0, int ni, GBo
     const bool act = lane < ni;
     const int p = i0 + (act ? lane : 0);
     const double c[3] = {a.x[p], a.y[p], a.z[p]};
-    const double H = kSupport * a.h[p] * a.hfac;
+    const double H = a.h ? kSupport * a.h[p] * a.hfac : 0.0;  // the gravity walk reads no h
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         b.glo[k] = wave_min(act ? c[k] : INFINITY);
@@ -1069,6 +1069,7 @@ void enqueue_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int ki
     SPH_LAUNCH_CHECK();
     ListArgs a = list_args(t, tp, P, L);
     a.need = need;
+    if (kind == 2) a.h = nullptr;  // geometry only: h may be in flux on another stream
     if (kind != 0) a.builds = nullptr;  // the counter counts gather-list constructions
     if (kind == 0) {  // gather lists remember their search boxes (reuse check)
         L.sbox.ensure((size_t) G * 6);

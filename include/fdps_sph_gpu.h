@@ -184,9 +184,13 @@ enum {
 int32_t sphgpu_sim_get(sphgpu_ctx *ctx, int32_t field, void *out);
 /* dt and simulated time of each system. */
 int32_t sphgpu_sim_get_time(sphgpu_ctx *ctx, double *dt_sys, double *time_sys);
-/* Wall time in ms of each stage of the last step (hipEvents):
- * [0] kick/drift, [1] sort+tree, [2] density x4 (lists+kernels), [3] EoS,
- * [4] derivative, [5] hydro, [6] gravity, [7] final kick + dt; n <= 8. */
+/* Wall time in ms of each stage of the last step (hipEvents on the main stream):
+ * [0] kick/drift, [1] sort+tree, [2] density x4 (gather lists + kernels; the
+ * gravity lists are built beside them on a second stream), [3] EoS,
+ * [4] derivative, [5] gravity (the hydro pass's lists are built beside it),
+ * [6] hydro, [7] final kick + dt; n <= 8.  With kernel statistics on, the
+ * second stream's work runs in line: the gravity lists in [2], the hydro
+ * lists in [5]. */
 int32_t sphgpu_sim_stage_ms(sphgpu_ctx *ctx, double *ms, int32_t n);
 
 /* Gather-list constructions and force evaluations since sphgpu_sim_init.  A
