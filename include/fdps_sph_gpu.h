@@ -195,7 +195,7 @@ int32_t sphgpu_sim_stage_ms(sphgpu_ctx *ctx, double *ms, int32_t n);
 
 /* Gather-list constructions and force evaluations since sphgpu_sim_init.  A
  * force evaluation runs 4 density passes and one derivative pass; they share
- * one gather list (built with a 3 % margin on h) for as long as it covers every
+ * one gather list (built with an 8 % margin on h) for as long as it covers every
  * particle's support, so gather_builds / force_evals is 1 when h settles. */
 int32_t sphgpu_sim_list_builds(sphgpu_ctx *ctx, int64_t *gather_builds, int64_t *force_evals);
 
