@@ -519,7 +519,8 @@ void sim_bind(SimState &S) {
 // Two list walks run on a second stream, beside force kernels they do not feed:
 // the walks wait on dependent loads and leave the VALU idle, the kernels are
 // VALU-bound.  The gravity lists need only the tree (the walk reads no h), so
-// they are built while the density passes run; the symmetric lists of the hydro
+// they are built while the density passes run (after the first gather lists,
+// which the first pass waits for); the symmetric lists of the hydro
 // pass need the final h and replace the gather lists, so they are built after
 // the derivative pass, while the gravity kernel runs.  With kernel statistics
 // on, everything runs in this order on one stream, so that each kernel's timing
@@ -574,7 +575,8 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
         };
         lists_reset(L, s);
         lists_reset(Lg, s);
-        signal(S.xev[0], s);  // tree, h restored, counters cleared
+        enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s);
+        signal(S.xev[0], s);  // tree, h restored, counters cleared, first gather lists
         await(s2, S.xev[0]);
         enqueue_lists(S.tree, S.tp, P, 2, Lg, 1.0, s2);
         signal(S.xev[1], s2);  // gravity lists
@@ -583,9 +585,7 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
         // serves every pass (and the derivative pass) while each group's
         // search box still fits inside the box it was built for.
         for (int loop = 0; loop <= 3; ++loop) {
-            if (loop == 0) {
-                enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s);
-            } else {
+            if (loop > 0) {
                 enqueue_gather_check(S.tree, P, L, s);
                 enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s, L.flag.get());
             }
