@@ -569,9 +569,14 @@ __device__ __forceinline__ GroupQueue xcd_queue(int ngroups, int32_t *ctr8) {
 // the rest in the workgroup's slice of global scratch (fcap entries).  Most
 // frontiers fit in LDS, so a level's node ids are neither written back to L2
 // nor fetched from it before the node data they index can be loaded.
+// The two parts carry their address spaces in their types: through generic
+// pointers the compiler selects one of them per lane and issues flat loads and
+// stores, which wait on both the LDS and the vector-memory counters.
+typedef __attribute__((address_space(3))) int32_t lds_i32;
+typedef __attribute__((address_space(1))) int32_t glb_i32;
 struct Front {
-    int32_t *s;  // LDS part
-    int32_t *g;  // global part
+    lds_i32 *s;  // LDS part
+    glb_i32 *g;  // global part
     int n;       // entries in the LDS part
     __device__ __forceinline__ int32_t get(int i) const { return i < n ? s[i] : g[i - n]; }
     __device__ __forceinline__ void put(int i, int32_t v) const {
@@ -694,7 +699,8 @@ __global__ __launch_bounds__(kWave) void k_list_sph(ListArgs a) {
         const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
         GBox b;
         group_box(a, i0, ni, b);
-        Front fa{s_front, fa0, a.flds}, fb{s_front + kFrontLds, fb0, a.flds};
+        Front fa{(lds_i32 *) s_front, (glb_i32 *) fa0, a.flds};
+        Front fb{(lds_i32 *) (s_front + kFrontLds), (glb_i32 *) fb0, a.flds};
         if (lane == 0) fa.put(0, a.sys_root[a.sys[i0]]);
         if (a.sbox && lane < 6) {  // selects, not an index by lane: the box stays in registers
             const double v = (lane == 0) ? b.slo[0] : (lane == 1) ? b.slo[1] : (lane == 2) ? b.slo[2]
@@ -734,7 +740,8 @@ __global__ __launch_bounds__(kWave) void k_list_grav(ListArgs a) {
         const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
         GBox b;
         group_box(a, i0, ni, b);
-        Front fa{s_front, fa0, a.flds}, fb{s_front + kFrontLds, fb0, a.flds};
+        Front fa{(lds_i32 *) s_front, (glb_i32 *) fa0, a.flds};
+        Front fb{(lds_i32 *) (s_front + kFrontLds), (glb_i32 *) fb0, a.flds};
         if (lane == 0) fa.put(0, a.sys_root[a.sys[i0]]);
         __syncthreads();
         int nf = 1, cj = 0, cs = 0;
