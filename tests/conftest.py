@@ -30,3 +30,15 @@ def ctx64():
     c.load_table(2, E.PATHS["dunite"])
     yield c
     c.close()
+
+
+@pytest.fixture(scope="session")
+def ctxmix():
+    from fdps_sph_amd import Context
+    c = Context(0, "mixed")
+    from fdps_sph_amd import eos_tables as E
+    c.load_table(0, E.PATHS["granite"])
+    c.load_table(1, E.PATHS["iron"])
+    c.load_table(2, E.PATHS["dunite"])
+    yield c
+    c.close()
