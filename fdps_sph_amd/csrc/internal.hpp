@@ -456,16 +456,17 @@ __device__ __forceinline__ uint32_t sym_block_(float a0, float a1, float a2, flo
     c = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, b3, c, 0, 0, 0);
     return own | sign_bits16_(c);
 }
-__device__ __forceinline__ uint64_t mfma_dot_mask_sym(const float4 *q, const float *qs, int nt, float tx, float ty,
-                                                      float tz, float negc, float d, float px, float py, float pz,
-                                                      float pc, float pd) {
+// q(e) and qs(e) give entry e's (o_x, o_y, o_z, w) and Q.
+template <class QF, class QSF>
+__device__ __forceinline__ uint64_t mfma_dot_mask_sym(QF q, QSF qs, int nt, float tx, float ty, float tz, float negc,
+                                                      float d, float px, float py, float pz, float pc, float pd) {
     const int lane = threadIdx.x, h = lane >> 5, m = lane & 31;
     // j operands: half 0 supplies k = 0, 2, 4 (o_x, o_z, 1), half 1 k = 1, 3, 5 (o_y, w_j, 0 for C1 and
     // -Q_j for C2: C1's k = 5 column entry is 0, so the row may carry -Q_j throughout)
     const int e = 16 * ((m >> 2) & 1) + 4 * (m >> 3) + (m & 3);
-    const float4 q0 = q[e], q1 = q[32 + e];
-    const float a00 = h ? q0.y : q0.x, a01 = h ? q0.w : q0.z, a02 = h ? -qs[e] : 1.f;
-    const float a10 = h ? q1.y : q1.x, a11 = h ? q1.w : q1.z, a12 = h ? -qs[32 + e] : 1.f;
+    const float4 q0 = q(e), q1 = q(32 + e);
+    const float a00 = h ? q0.y : q0.x, a01 = h ? q0.w : q0.z, a02 = h ? -qs(e) : 1.f;
+    const float a10 = h ? q1.y : q1.x, a11 = h ? q1.w : q1.z, a12 = h ? -qs(32 + e) : 1.f;
     // i operands: half 0 supplies k = 0, 2 (-2 o_x, -2 o_z) and k = 4 (-c_i; then d_i), half 1 k = 1, 3
     // (-2 o_y, 1) and k = 5 (0; then 1)
     const float b00 = h ? py : tx, b01 = h ? 1.f : tz, b02 = h ? 0.f : negc, b03 = h ? 1.f : d;

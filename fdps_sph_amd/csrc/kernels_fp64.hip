@@ -88,6 +88,17 @@ __device__ __forceinline__ double wave_max_(double v) {
 struct Frame {
     double cx, cy, cz;
 };
+// The hydro pass's j record in LDS with the entry's FP32 offset record (the
+// support mask's operand) beside it: 144 bytes.  In the pair loop each lane
+// reads the record of its own j; at a stride of 128 bytes (the span of the
+// banks) the lanes' 16-byte reads all start on the same 4 banks and
+// serialise, at 144 bytes they start on 8 positions, which lets the banks
+// serve 8 lanes per cycle.
+struct alignas(16) HydroSlot {
+    HydroRec r;
+    float4 o;  // FP32 offset from the frame centre, |offset|^2
+};
+static_assert(sizeof(HydroSlot) == 144, "LDS record stride");
 __device__ __forceinline__ Frame group_frame(double x, double y, double z, bool act) {
     const double lx = wave_min_(act ? x : INFINITY), hx = wave_max_(act ? x : -INFINITY);
     const double ly = wave_min_(act ? y : INFINITY), hy = wave_max_(act ? y : -INFINITY);
@@ -336,9 +347,8 @@ __global__ __launch_bounds__(kWave, 4) void k_hydro(HydroArgs a) {
     const double cvz = 0.5 * (wave_min_(act ? vzi : INFINITY) + wave_max_(act ? vzi : -INFINITY));
     const double bi = csi + 3.0 * sqrt((vxi - cvx) * (vxi - cvx) + (vyi - cvy) * (vyi - cvy) + (vzi - cvz) * (vzi - cvz));
     double ax = 0.0, ay = 0.0, az = 0.0, du = 0.0, vmax = 0.0;
-    __shared__ HydroJ sj[kWave];
-    __shared__ float4 so[kWave];  // FP32 offsets from the frame centre, |offset|^2
-    __shared__ float sQ[kWave];   // threshold of j's support
+    __shared__ HydroSlot sj[kWave];
+    __shared__ float sQ[kWave];  // threshold of j's support
     for (int t = 0; t < tl.T; ++t) {
         const int k = tl.entry(t, lane);
         float nj1 = 0.f;
@@ -348,11 +358,11 @@ __global__ __launch_bounds__(kWave, 4) void k_hydro(HydroArgs a) {
             const HydroJ r = a.jrec ? a.jrec[q]
                                     : make_hydro_rec(a.xj[q], a.yj[q], a.zj[q], a.vxj[q], a.vyj[q], a.vzj[q], a.mj[q],
                                                      a.rhoj[q], a.csj[q], a.balj[q], a.gsj[q], a.pj[q], a.hj[q]);
-            sj[lane] = r;
+            sj[lane].r = r;
             const double ux = r.vx - cvx, uy = r.vy - cvy, uz = r.vz - cvz;
             bj = r.cs + 3.0 * sqrt(ux * ux + uy * uy + uz * uz);
             const float ox = (float) (r.x - fr.cx), oy = (float) (r.y - fr.cy), oz = (float) (r.z - fr.cz);
-            so[lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
+            sj[lane].o = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
             nj1 = norm1_(ox, oy, oz);
             sQ[lane] = dot_cut_j_((float) r.H * 1.0001f, nj1, n1max);
         }
@@ -374,10 +384,11 @@ __global__ __launch_bounds__(kWave, 4) void k_hydro(HydroArgs a) {
         // the square root and division are skipped -- the maximum cannot change.
         for (uint64_t cm = cand; cm; cm &= cm - 1) {
             const int jj = __builtin_ctzll(cm);
-            const double drx = xi - sj[jj].x, dry = yi - sj[jj].y, drz = zi - sj[jj].z;
-            const double dvx = vxi - sj[jj].vx, dvy = vyi - sj[jj].vy, dvz = vzi - sj[jj].vz;
+            const HydroJ &J = sj[jj].r;
+            const double drx = xi - J.x, dry = yi - J.y, drz = zi - J.z;
+            const double dvx = vxi - J.vx, dvy = vyi - J.vy, dvz = vzi - J.vz;
             const double dvdr = dvx * drx + dvy * dry + dvz * drz;
-            const double cij = csi + sj[jj].cs;
+            const double cij = csi + J.cs;
             if (dvdr < 0) {
                 const double room = vmax - cij;
                 const double dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
@@ -394,9 +405,10 @@ __global__ __launch_bounds__(kWave, 4) void k_hydro(HydroArgs a) {
         // F_i = norm_i s(1 + 7s + 16s^2)(1-s)^7 / (r H_i + 1e-6 h_i); the two
         // reciprocals come from one, 1/(a_i a_j).  AVh is half the viscosity
         // AV of the reference, and the sums leave out K.
-        const uint64_t mask = mfma_dot_mask_sym(so, sQ, nt, tx, ty, tz, negc, dsym, ptx, pty, ptz, pnegc, pdsym);
+        const uint64_t mask = mfma_dot_mask_sym([&](int e) { return sj[e].o; }, [&](int e) { return sQ[e]; }, nt, tx,
+                                                ty, tz, negc, dsym, ptx, pty, ptz, pnegc, pdsym);
         for (uint64_t msk = mask; msk; msk &= msk - 1) {
-            const HydroJ &J = sj[__builtin_ctzll(msk)];
+            const HydroJ &J = sj[__builtin_ctzll(msk)].r;
             const double drx = xi - J.x, dry = yi - J.y, drz = zi - J.z;
             const double dvx = vxi - J.vx, dvy = vyi - J.vy, dvz = vzi - J.vz;
             const double dvdr = fma(dvz, drz, fma(dvy, dry, dvx * drx));

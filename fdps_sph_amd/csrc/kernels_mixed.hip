@@ -361,7 +361,8 @@ __global__ __launch_bounds__(kWave, 4) void k_hydro_mixed(HydroArgs a) {
             vmax = fmaxf(vmax, csi + scs[jj] - 3.f * w);
         }
         // the pairs inside either support (a superset: pairs beyond both give zero terms)
-        const uint64_t mask = mfma_dot_mask_sym(so, sQ, nt, tx, ty, tz, negc, dsym, ptx, pty, ptz, pnegc, pdsym);
+        const uint64_t mask = mfma_dot_mask_sym([&](int e) { return so[e]; }, [&](int e) { return sQ[e]; }, nt, tx,
+                                                ty, tz, negc, dsym, ptx, pty, ptz, pnegc, pdsym);
         // pass 2: force terms of this lane's in-support pairs
         for (uint64_t msk = mask; msk; msk &= msk - 1) {
             const int jj = __builtin_ctzll(msk);
