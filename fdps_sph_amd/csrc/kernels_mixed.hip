@@ -260,8 +260,20 @@ __global__ __launch_bounds__(kWave) void k_drvt_mixed(DrvtArgs a) {
 }
 
 // ---------------------------------------------------------------- hydro force
+// One entry of a hydro tile in LDS, read by the pair loops as five 16-byte
+// loads.  Its 80-byte stride puts the lanes' reads (each of its own j) on 8
+// starting banks, so that the banks serve 8 lanes per cycle.
+struct alignas(16) HydroSlotF {
+    float4 o;  // FP32 offset from the centre, |offset|^2
+    float4 v;  // velocity offset from the velocity centre, sound speed
+    float4 k;  // rho, Balsara switch, 1/H, H
+    float4 w;  // kernel normalisation, gradient regularisation, grad-h term, mass
+    float4 a;  // A = f P / rho^2, -, -, -
+};
+static_assert(sizeof(HydroSlotF) == 80, "LDS record stride");
+
 // 4 waves per SIMD: at 5 waves (96 registers) 27 spill, and the kernel took
-// 1.33 ms at 1M against 1.20 ms at 4 waves (128 registers, two spilled).
+// 1.33 ms at 1M against 1.20 ms at 4 waves (124 registers).
 __global__ __launch_bounds__(kWave, 4) void k_hydro_mixed(HydroArgs a) {
     const int g = xcd_remap(blockIdx.x, a.ngroups);
     const int lane = threadIdx.x;
@@ -294,10 +306,8 @@ __global__ __launch_bounds__(kWave, 4) void k_hydro_mixed(HydroArgs a) {
     float vmax = 0.f;
     const int64_t lb = list_begin(a.L, g);
     const int nj = a.L.cnt[g];
-    __shared__ float4 so[kWave];  // FP32 offset from the centre, |offset|^2
-    __shared__ float svx[kWave], svy[kWave], svz[kWave], sm[kWave], sQ[kWave];
-    __shared__ float srho[kWave], scs[kWave], sbal[kWave], sgs[kWave], sA[kWave], sinvH[kWave], sH[kWave],
-        sreg[kWave], snorm[kWave];
+    __shared__ HydroSlotF slot[kWave];
+    __shared__ float sQ[kWave];
     const Tiles tl(nj);
     for (int t = 0; t < tl.T; ++t) {
         const int k = tl.entry(t, lane);
@@ -319,24 +329,17 @@ __global__ __launch_bounds__(kWave, 4) void k_hydro_mixed(HydroArgs a) {
                 nj3 = kNormD / (Hj * Hj * Hj);
             }
             const float ox = (float) (x - cx), oy = (float) (y - cy), oz = (float) (z - cz);
-            so[lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
             nj1 = norm1_(ox, oy, oz);
             sQ[lane] = dot_cut_j_((float) Hj * 1.0001f, nj1, n1max);
             const float ux = (float) (vx - cvx), uy = (float) (vy - cvy), uz = (float) (vz - cvz);
-            svx[lane] = ux;
-            svy[lane] = uy;
-            svz[lane] = uz;
             bnd = (float) cj + 3.f * sqrtf(ux * ux + uy * uy + uz * uz);
-            sm[lane] = (float) m;
-            srho[lane] = (float) rj;
-            scs[lane] = (float) cj;
-            sbal[lane] = (float) bj;
-            sgs[lane] = (float) gj;
-            sA[lane] = (float) Aj;
-            sH[lane] = (float) Hj;
-            sinvH[lane] = (float) (1.0 / Hj);
-            sreg[lane] = (float) regj;
-            snorm[lane] = (float) nj3;
+            HydroSlotF e;
+            e.o = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
+            e.v = make_float4(ux, uy, uz, (float) cj);
+            e.k = make_float4((float) rj, (float) bj, (float) (1.0 / Hj), (float) Hj);
+            e.w = make_float4((float) nj3, (float) regj, (float) gj, (float) m);
+            e.a = make_float4((float) Aj, 0.f, 0.f, 0.f);
+            slot[lane] = e;
         }
         const float negc = act ? -dot_cut_(Hci, wi, n1i, wave_maxf(nj1)) : INFINITY;
         const float pnegc = __shfl_xor(negc, 32, kWave);
@@ -352,45 +355,46 @@ __global__ __launch_bounds__(kWave, 4) void k_hydro_mixed(HydroArgs a) {
         // pass 1: v_sig_max over the candidates
         for (uint64_t cm = cand; cm; cm &= cm - 1) {
             const int jj = __builtin_ctzll(cm);
-            const float4 oj = so[jj];
+            const float4 oj = slot[jj].o, vj = slot[jj].v;
             const float drx = xi - oj.x, dry = yi - oj.y, drz = zi - oj.z;
-            const float dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
+            const float dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
             const float dvdr = dvx * drx + dvy * dry + dvz * drz;
             const float r2 = drx * drx + dry * dry + drz * drz;
             const float w = (dvdr < 0.f) ? dvdr * rsq(r2 + 1e-30f) : 0.f;
-            vmax = fmaxf(vmax, csi + scs[jj] - 3.f * w);
+            vmax = fmaxf(vmax, csi + vj.w - 3.f * w);
         }
         // the pairs inside either support (a superset: pairs beyond both give zero terms)
-        const uint64_t mask = mfma_dot_mask_sym([&](int e) { return so[e]; }, [&](int e) { return sQ[e]; }, nt, tx,
+        const uint64_t mask = mfma_dot_mask_sym([&](int e) { return slot[e].o; }, [&](int e) { return sQ[e]; }, nt, tx,
                                                 ty, tz, negc, dsym, ptx, pty, ptz, pnegc, pdsym);
         // pass 2: force terms of this lane's in-support pairs
         for (uint64_t msk = mask; msk; msk &= msk - 1) {
             const int jj = __builtin_ctzll(msk);
-            const float4 oj = so[jj];
+            const HydroSlotF &J = slot[jj];
+            const float4 oj = J.o, vj = J.v, kj = J.k, wj = J.w;
             const float drx = xi - oj.x, dry = yi - oj.y, drz = zi - oj.z;
-            const float dvx = vxi - svx[jj], dvy = vyi - svy[jj], dvz = vzi - svz[jj];
+            const float dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
             const float dvdr = dvx * drx + dvy * dry + dvz * drz;
             const float r2 = drx * drx + dry * dry + drz * drz;
             const float rinv = rsq(r2 + 1e-30f);
             const float w = (dvdr < 0.f) ? dvdr * rinv : 0.f;
-            const float v_sig = csi + scs[jj] - 3.f * w;
+            const float v_sig = csi + vj.w - 3.f * w;
             {
                 const float r = r2 * rinv;
-                const float AV = (-0.5f * v_sig * w) * rcp(rhoi + srho[jj]) * (bali + sbal[jj]);
+                const float AV = (-0.5f * v_sig * w) * rcp(rhoi + kj.x) * (bali + kj.y);
                 // W'(s) factors for h_i and h_j
                 const float si = r * invHi;
                 const float oi = fmaxf(1.f - si, 0.f);
                 const float oi2 = oi * oi, oi7 = oi2 * oi2 * oi2 * oi;
                 const float fi = normi * (-22.f * si * (1.f + si * (7.f + 16.f * si)) * oi7) * rcp(r * Hi + regi);
-                const float Hj = sH[jj];
-                const float sj = r * sinvH[jj];
+                const float Hj = kj.w;
+                const float sj = r * kj.z;
                 const float oj = fmaxf(1.f - sj, 0.f);
                 const float oj2 = oj * oj, oj7 = oj2 * oj2 * oj2 * oj;
-                const float fj = snorm[jj] * (-22.f * sj * (1.f + sj * (7.f + 16.f * sj)) * oj7) * rcp(r * Hj + sreg[jj]);
+                const float fj = wj.x * (-22.f * sj * (1.f + sj * (7.f + 16.f * sj)) * oj7) * rcp(r * Hj + wj.y);
                 // gradW_i = dr fi, gradW_j = dr fj, mean = 0.5 (fi gs_i + fj gs_j) dr
-                const float fm = 0.5f * (fi * gsi + fj * sgs[jj]);
-                const float m = sm[jj];
-                const float coef = m * (fAi * fi + sA[jj] * fj + AV * fm);
+                const float fm = 0.5f * (fi * gsi + fj * wj.z);
+                const float m = wj.w;
+                const float coef = m * (fAi * fi + J.a.x * fj + AV * fm);
                 px -= coef * drx;
                 py -= coef * dry;
                 pz -= coef * drz;
