@@ -391,12 +391,14 @@ __device__ __forceinline__ uint32_t sign_bits16_(const f32x16_ &c) {
     for (int r = 15; r >= 0; --r) w = __builtin_amdgcn_alignbit(w, __float_as_uint(c[r]), 31);
     return w;
 }
-__device__ __forceinline__ uint64_t mfma_dot_mask(const float4 *q, int nt, float tx, float ty, float tz, float negc,
-                                                  float px, float py, float pz, float pc) {
+// q(e) gives entry e's (o_x, o_y, o_z, w); the array form follows.
+template <class QF>
+__device__ __forceinline__ uint64_t mfma_dot_mask_f(QF q, int nt, float tx, float ty, float tz, float negc, float px,
+                                                    float py, float pz, float pc) {
     const int lane = threadIdx.x, h = lane >> 5, m = lane & 31;
     // j operands: half 0 supplies k = 0, 2, 4 (o_x, o_z, 1), half 1 k = 1, 3, 5 (o_y, w, 0)
     const int e = 16 * ((m >> 2) & 1) + 4 * (m >> 3) + (m & 3);
-    const float4 q0 = q[e], q1 = q[32 + e];
+    const float4 q0 = q(e), q1 = q(32 + e);
     const float a00 = h ? q0.y : q0.x, a01 = h ? q0.w : q0.z, a02 = h ? 0.f : 1.f;
     const float a10 = h ? q1.y : q1.x, a11 = h ? q1.w : q1.z, a12 = a02;
     // i operands: half 0 supplies k = 0, 2, 4 (-2 o_x, -2 o_z, -c), half 1 k = 1, 3, 5 (-2 o_y, 1, 0),
@@ -433,6 +435,10 @@ __device__ __forceinline__ uint64_t mfma_dot_mask(const float4 *q, int nt, float
     uint64_t mask = ((uint64_t) hi << 32) | lo;
     if (nt < kWave) mask &= (1ULL << nt) - 1;
     return mask;
+}
+__device__ __forceinline__ uint64_t mfma_dot_mask(const float4 *q, int nt, float tx, float ty, float tz, float negc,
+                                                  float px, float py, float pz, float pc) {
+    return mfma_dot_mask_f([q](int e) { return q[e]; }, nt, tx, ty, tz, negc, px, py, pz, pc);
 }
 
 // Symmetric form from the matrix cores (inside either support): the bit is

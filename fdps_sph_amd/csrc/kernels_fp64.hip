@@ -99,6 +99,19 @@ struct alignas(16) HydroSlot {
     float4 o;  // FP32 offset from the frame centre, |offset|^2
 };
 static_assert(sizeof(HydroSlot) == 144, "LDS record stride");
+// The derivative pass's j entry in LDS: position and mass, velocity and the
+// FP32 offset record in an 80-byte slot (stride on 8 start banks, as above).
+struct alignas(16) DrvtSlot {
+    double x, y, z, m, vx, vy, vz, pad;
+    float4 o;
+};
+static_assert(sizeof(DrvtSlot) == 80, "LDS record stride");
+// The density pass's: position and mass, and the FP32 offset record: 48 bytes.
+struct alignas(16) DensSlot {
+    double x, y, z, m;
+    float4 o;
+};
+static_assert(sizeof(DensSlot) == 48, "LDS record stride");
 __device__ __forceinline__ Frame group_frame(double x, double y, double z, bool act) {
     const double lx = wave_min_(act ? x : INFINITY), hx = wave_max_(act ? x : -INFINITY);
     const double ly = wave_min_(act ? y : INFINITY), hy = wave_max_(act ? y : -INFINITY);
@@ -135,8 +148,7 @@ __global__ __launch_bounds__(kWave, 5) void k_dens(DensArgs a) {
     int ngb = 0;
     const int64_t lb = list_begin(a.L, g);
     const Tiles tl(a.L.cnt[g]);
-    __shared__ double4 sp[kWave];  // x, y, z, m
-    __shared__ float4 so[kWave];   // FP32 offsets from the frame centre, |offset|^2
+    __shared__ DensSlot sj[kWave];
     for (int t = 0; t < tl.T; ++t) {
         const int k = tl.entry(t, lane);
         float nj1 = 0.f;
@@ -144,21 +156,21 @@ __global__ __launch_bounds__(kWave, 5) void k_dens(DensArgs a) {
             const int q = a.L.idx[lb + k];
             const double4 pq = a.jpos ? a.jpos[q] : make_double4(a.xj[q], a.yj[q], a.zj[q], a.mj[q]);
             const double x = pq.x, y = pq.y, z = pq.z;
-            sp[lane] = pq;
             const float ox = (float) (x - fr.cx), oy = (float) (y - fr.cy), oz = (float) (z - fr.cz);
-            so[lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
+            sj[lane] = DensSlot{x, y, z, pq.w, make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)))};
             nj1 = norm1_(ox, oy, oz);
         }
         const float negc = act ? -dot_cut_(Hc, wi, n1i, wave_maxf_(nj1)) : INFINITY;
         const float pnegc = __shfl_xor(negc, 32, kWave);
         __syncthreads();
-        const uint64_t mask = mfma_dot_mask(so, tl.count(t), tx, ty, tz, negc, ptx, pty, ptz, pnegc);
+        const uint64_t mask = mfma_dot_mask_f([&](int e) { return sj[e].o; }, tl.count(t), tx, ty, tz, negc, ptx, pty,
+                                              ptz, pnegc);
         for (uint64_t m = mask; m; m &= m - 1) {
-            const double4 pj = sp[__builtin_ctzll(m)];
-            const double dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+            const DensSlot &J = sj[__builtin_ctzll(m)];
+            const double dx = J.x - xi, dy = J.y - yi, dz = J.z - zi;
             const double s = dist_(norm2_(dx, dy, dz)).r * invH;  // r / H with 1/H hoisted out of the j loop
             const double o = fmax(1.0 - s, 0.0);
-            dens = fma(pj.w, wpoly_(s) * pow8_(o), dens);  // sum of m_j W_ij / norm
+            dens = fma(J.m, wpoly_(s) * pow8_(o), dens);  // sum of m_j W_ij / norm
             ngb += (o > 0.0) ? 1 : 0;
         }
         __syncthreads();
@@ -197,9 +209,7 @@ __global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
     double divv = 0.0, rx = 0.0, ry = 0.0, rz = 0.0, gsw = 0.0, gsf = 0.0;
     const int64_t lb = list_begin(a.L, g);
     const Tiles tl(a.L.cnt[g]);
-    __shared__ double4 sp[kWave];  // x, y, z, m
-    __shared__ double4 sv[kWave];  // vx, vy, vz, -
-    __shared__ float4 so[kWave];
+    __shared__ DrvtSlot sj[kWave];
     for (int t = 0; t < tl.T; ++t) {
         const int k = tl.entry(t, lane);
         float nj1 = 0.f;
@@ -214,34 +224,33 @@ __global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
                 vq = make_double4(a.vxj[q], a.vyj[q], a.vzj[q], 0.0);
             }
             const double x = pq.x, y = pq.y, z = pq.z;
-            sp[lane] = pq;
-            sv[lane] = vq;
             const float ox = (float) (x - fr.cx), oy = (float) (y - fr.cy), oz = (float) (z - fr.cz);
-            so[lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
+            sj[lane] = DrvtSlot{x, y, z, pq.w, vq.x, vq.y, vq.z, 0.0,
+                                make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)))};
             nj1 = norm1_(ox, oy, oz);
         }
         const float negc = act ? -dot_cut_(Hc, wi, n1i, wave_maxf_(nj1)) : INFINITY;
         const float pnegc = __shfl_xor(negc, 32, kWave);
         __syncthreads();
-        const uint64_t mask = mfma_dot_mask(so, tl.count(t), tx, ty, tz, negc, ptx, pty, ptz, pnegc);
+        const uint64_t mask = mfma_dot_mask_f([&](int e) { return sj[e].o; }, tl.count(t), tx, ty, tz, negc, ptx, pty,
+                                              ptz, pnegc);
         for (uint64_t msk = mask; msk; msk &= msk - 1) {
             const int jj = __builtin_ctzll(msk);
-            const double4 pj = sp[jj];
-            const double4 vj = sv[jj];
-            const double drx = xi - pj.x, dry = yi - pj.y, drz = zi - pj.z;
-            const double dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
+            const DrvtSlot &J = sj[jj];
+            const double drx = xi - J.x, dry = yi - J.y, drz = zi - J.z;
+            const double dvx = vxi - J.vx, dvy = vyi - J.vy, dvz = vzi - J.vz;
             const double r2 = norm2_(drx, dry, drz);
             const double r = dist_(r2).r;
             const double s = r * invH;
             const double o = fmax(1.0 - s, 0.0);
             const double o7 = pow7f_(o);
-            const double mf = pj.w * (dwds_(s, o7) * rcp_(fma(r, H, reg)));  // m_j gradW / (K dr)
+            const double mf = J.m * (dwds_(s, o7) * rcp_(fma(r, H, reg)));  // m_j gradW / (K dr)
             const double dvdr = fma(dvz, drz, fma(dvy, dry, dvx * drx));
             divv = fma(mf, dvdr, divv);                     // -divv / K: sum of m_j dv . gradW / K
             rx = fma(mf, fma(dvy, drz, -(dvz * dry)), rx);  // -rot / K: sum of m_j dv x gradW / K
             ry = fma(mf, fma(dvz, drx, -(dvx * drz)), ry);
             rz = fma(mf, fma(dvx, dry, -(dvy * drx)), rz);
-            gsw = fma(pj.w, wpoly_(s) * (o7 * o), gsw);  // sum of m_j W / norm
+            gsw = fma(J.m, wpoly_(s) * (o7 * o), gsw);  // sum of m_j W / norm
             gsf = fma(mf, r2, gsf);                       // sum of m_j dr . gradW / K
         }
         __syncthreads();
