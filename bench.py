@@ -530,6 +530,9 @@ def main():
     roofline = {"bound": rl["bound"], "achieved": rl["achieved"], "peak": rl["peak"], "unit": rl["unit"],
                 "frac": rl["frac"], "traffic": rl["traffic"], "kernel": kernel_symbol(dominant, main_prec == "mixed"),
                 "avg_ms": rl["avg_ms"], "per_launch": rl["per_launch"]}
+    if rl["bound"] == "valu":  # gravity: FMAs and a reciprocal square root per pair on the vector ALU, no MFMA
+        roofline["note"] = ("vector-ALU bound: peak = MI355X's published " + ("FP32" if main_prec == "mixed" else "FP64")
+                            + " vector rate; achieved = 38 flops per interaction x interactions / kernel time")
     out = {
         "metric": METRIC,
         "value": R["steps_per_s"],
