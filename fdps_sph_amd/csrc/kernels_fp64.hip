@@ -127,7 +127,9 @@ __device__ __forceinline__ float wave_maxf_(float v) {
 // ---------------------------------------------------------------- density
 // CalcDensity::operator() (src/force.h:9-23)
 // At least 5 waves per SIMD: the register cap this sets costs no spills and
-// measured faster than the compiler's own choice (4 waves).
+// measured faster than the compiler's own choice (4 waves); the kernel now fits
+// 71 registers (7 waves).  A cap for 8 waves (64 registers, two spilled)
+// measured slower: 0.508 -> 0.516 ms at 1M.
 __global__ __launch_bounds__(kWave, 5) void k_dens(DensArgs a) {
     const int g = xcd_remap(blockIdx.x, a.ngroups);
     const int lane = threadIdx.x;
