@@ -325,16 +325,18 @@ def test_list_overflow_redo(ctx64):
         assert np.array_equal(big.get(f), small.get(f)), f
 
 
-def test_frontier_spill_and_overflow(ctx64):
+@pytest.mark.parametrize("ctx", ["ctx64", "ctxmix"])
+def test_frontier_spill_and_overflow(ctx, request):
     """Tree walks keep their frontier in LDS and continue it in global memory.
     With one LDS entry every walk runs through the global part; with a global
     part of 4 entries as well every force evaluation overflows the frontier,
     grows it and runs again.  Both give the default run's results bit for
-    bit."""
+    bit, in either precision form."""
+    c = request.getfixturevalue(ctx)
     ic = IC.granite_impact(9000, 1000, seed=1002)
-    ref = Simulation(ctx64, ic)
-    spill = Simulation(ctx64, ic, list_front_lds=1)
-    tiny = Simulation(ctx64, ic, list_front_lds=1, list_cap_front=4)
+    ref = Simulation(c, ic)
+    spill = Simulation(c, ic, list_front_lds=1)
+    tiny = Simulation(c, ic, list_front_lds=1, list_cap_front=4)
     for sim in (ref, spill, tiny):
         sim.step(2)
     _print(f"[frontier spill] builds {ref.list_builds()} {spill.list_builds()} {tiny.list_builds()}")
