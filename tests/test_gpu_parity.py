@@ -309,14 +309,16 @@ def test_gather_list_reuse(ctx64):
     assert evals <= builds <= 2 * evals
 
 
-def test_list_overflow_redo(ctx64):
+@pytest.mark.parametrize("ctx", ["ctx64", "ctxmix"])
+def test_list_overflow_redo(ctx, request):
     """List capacities far too small for the lists: every force evaluation
     overflows, grows its capacities and runs again from the saved h.  The
     results equal those of a run whose capacities fit from the start, bit for
     bit, and the gather-list constructions are counted once."""
+    c = request.getfixturevalue(ctx)
     ic = IC.granite_impact(9000, 1000, seed=1001)
-    big = Simulation(ctx64, ic)
-    small = Simulation(ctx64, ic, list_cap_sph=16, list_cap_epj=16, list_cap_spj=16)
+    big = Simulation(c, ic)
+    small = Simulation(c, ic, list_cap_sph=16, list_cap_epj=16, list_cap_spj=16)
     big.step(2)
     small.step(2)
     _print(f"[list overflow redo] builds {big.list_builds()} vs {small.list_builds()}")
