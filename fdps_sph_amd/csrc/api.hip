@@ -463,6 +463,7 @@ void retrieve(sphgpu_ctx *c, int kind, int tag, int n_walk, const int32_t n_epi[
     Batch &b = batch_for(c, tag);
     if (!b.pending || b.kind != kind) throw Error(SPHGPU_ERR_TAG, "no batch of this kind dispatched on this tag");
     if (n_walk != b.n_walk) throw Error(SPHGPU_ERR_ARG, "n_walk differs from the dispatched batch");
+    if (n_walk > 0 && !n_epi) throw Error(SPHGPU_ERR_ARG, "null n_epi array");
     for (int w = 0; w < n_walk; ++w)
         if (n_epi[w] != b.n_epi[w]) throw Error(SPHGPU_ERR_ARG, "n_epi differs from the dispatched batch");
     if (n_walk > 0 && !res) throw Error(SPHGPU_ERR_ARG, "null res array");
@@ -1206,7 +1207,11 @@ int32_t sphgpu_sim_lists(sphgpu_ctx *c, int32_t kind, int64_t *n_groups, int64_t
         if (kind < 0 || kind > 2) throw Error(SPHGPU_ERR_ARG, "kind must be 0, 1 or 2");
         if (!n_groups || !n_j || !n_s) throw Error(SPHGPU_ERR_ARG, "null size output");
         hipStream_t s = c->stream;
-        tree_sort_and_build(S.tree, S.tp, S.n, S.P, S.Palt, s);
+        // The particles are in key order after every force evaluation (init and
+        // each step sort before evaluating and do not move them afterwards), so
+        // the tree is rebuilt in place: a re-sort would permute only the fields
+        // the step carries and leave the others in the alternate buffers.
+        tree_build_only(S.tree, S.tp, S.n, S.P, s);
         Lists &L = (kind == 2) ? S.lgrav : S.lsph;
         if (kind == 2) build_lists_grav(S.tree, S.tp, S.P, L, s);
         else {

@@ -89,7 +89,13 @@ class Context:
         return out
 
     # ------------------------------------------------------------ multiwalk functors
-    def _run(self, kind, walks_i, walks_j, di, dj, dres, tag, walks_s=None):
+    _TYPES = {"dens": (S.EPI_DENS, S.EPJ_DENS, S.RES_DENS), "drvt": (S.EPI_DRVT, S.EPJ_DRVT, S.RES_DRVT),
+              "hydro": (S.EPI_HYDRO, S.EPJ_HYDRO, S.RES_HYDRO), "grav": (S.EPI_GRAV, S.EPJ_GRAV, S.RES_GRAV)}
+
+    def dispatch(self, kind: str, walks_i, walks_j, walks_s=None, tag: int = 0) -> np.ndarray:
+        """FDPS multiwalk dispatch of one batch of walks on `tag`; returns the
+        n_epi array that the matching retrieve takes."""
+        di, dj, _ = self._TYPES[kind]
         ai, ni, pi = _walk_arrays(walks_i, di)
         aj, nj, pj = _walk_arrays(walks_j, dj)
         nw = len(ai)
@@ -104,22 +110,33 @@ class Context:
                 check(disp(self._h, tag, nw, pi, pn, pj, pnj, ps, ns.ctypes.data_as(C.POINTER(C.c_int32))))
         else:
             check(disp(self._h, tag, nw, pi, pn, pj, pnj))
-        res = [np.zeros(len(a), dtype=dres) for a in ai]
+        return ni
+
+    def retrieve(self, kind: str, n_epi: np.ndarray, tag: int = 0):
+        """FDPS multiwalk retrieve of the batch dispatched on `tag`."""
+        dres = self._TYPES[kind][2]
+        n_epi = np.ascontiguousarray(n_epi, dtype=np.int32)
+        nw = len(n_epi)
+        res = [np.zeros(int(k), dtype=dres) for k in n_epi]
         pr = (C.c_void_p * max(1, nw))(*[r.ctypes.data for r in res])
-        check(getattr(self._L, f"sphgpu_{kind}_retrieve")(self._h, tag, nw, pn, pr))
+        check(getattr(self._L, f"sphgpu_{kind}_retrieve")(self._h, tag, nw, n_epi.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                          pr))
         return res
 
+    def _run(self, kind, walks_i, walks_j, tag, walks_s=None):
+        return self.retrieve(kind, self.dispatch(kind, walks_i, walks_j, walks_s, tag), tag)
+
     def calc_density(self, walks_i, walks_j, tag=0):
-        return self._run("dens", walks_i, walks_j, S.EPI_DENS, S.EPJ_DENS, S.RES_DENS, tag)
+        return self._run("dens", walks_i, walks_j, tag)
 
     def calc_derivative(self, walks_i, walks_j, tag=0):
-        return self._run("drvt", walks_i, walks_j, S.EPI_DRVT, S.EPJ_DRVT, S.RES_DRVT, tag)
+        return self._run("drvt", walks_i, walks_j, tag)
 
     def calc_hydro(self, walks_i, walks_j, tag=0):
-        return self._run("hydro", walks_i, walks_j, S.EPI_HYDRO, S.EPJ_HYDRO, S.RES_HYDRO, tag)
+        return self._run("hydro", walks_i, walks_j, tag)
 
     def calc_gravity(self, walks_i, walks_j, walks_s=None, tag=0):
-        return self._run("grav", walks_i, walks_j, S.EPI_GRAV, S.EPJ_GRAV, S.RES_GRAV, tag, walks_s)
+        return self._run("grav", walks_i, walks_j, tag, walks_s)
 
 
 def default_params(**kw) -> SimParams:
