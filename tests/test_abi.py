@@ -5,6 +5,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 from fdps_sph_amd import _lib
 from fdps_sph_amd import structs as S
@@ -49,13 +50,46 @@ def test_errors_without_context():
     assert p.list_cap_front == 4096 and p.list_front_lds == 512
 
 
-def test_regrid_keeps_nodes_and_axes():
+@pytest.mark.parametrize("name,kw", [("granite", dict(clip=False, umin_factor=1.0)),
+                                     ("iron", dict(clip=False, umin_factor=1.0)),
+                                     ("dunite", {})])
+def test_conversion_reproduces_shipped_tables(name, kw):
+    """eos/ANEOS_conversion.py turned the iSALE ANEOS tables (*.table.txt) into
+    the shipped rho-u tables, printed to 9 significant digits; converting the
+    same source at nu = 120 gives every cell to that precision (granite and
+    iron by the script's earlier form: u_min = min u, no P clip)."""
     from fdps_sph_amd import eos_tables as E
-    c = E.read(E.PATHS["iron"])
-    r = E.regrid(c, 200, 200)
+    src = E.read_source(E.SOURCE_PATHS[name])
+    ship = E.read(E.PATHS[name])
+    c = E.convert(src, nu=120, **kw)
+    assert c.shape == ship.shape == (120, 120, 6)
+    zero = ship == 0  # clipped pressures: exactly 0 on both sides
+    assert np.array_equal(c[zero], ship[zero])
+    rel = np.abs(c[~zero] - ship[~zero]) / np.abs(ship[~zero])
+    assert rel.max() <= 6e-9, (name, rel.max())
+    if name == "dunite":  # its source has negative pressures; the clip removes them
+        raw = E.convert(src, nu=120, clip=False)
+        assert (raw[:, :, 3] < 0).sum() > 0 and np.array_equal(c[:, :, 3], np.maximum(raw[:, :, 3], 0))
+
+
+def test_regrid_200():
+    """200 log-spaced densities and energies: the first and last rows are the
+    source's first and last densities, converted exactly as at the source
+    rows; every other row lies between its two source rows."""
+    from fdps_sph_amd import eos_tables as E
+    src = E.read_source(E.SOURCE_PATHS["iron"])
+    r = E.regrid("iron", 200, 200)
+    rows = E.convert(src, nu=200)
     assert r.shape == (200, 200, 6)
-    assert r[0, 0, 0] == c[0, 0, 0] and r[-1, 0, 0] == c[-1, 0, 0]
-    assert np.all(np.diff(np.log(r[:, 0, 0])) > 0)
-    assert np.allclose(np.diff(np.log(r[1:-1, 0, 0])), np.diff(np.log(r[1:-1, 0, 0]))[0], rtol=1e-9)
-    # the regridded table reproduces the source at a shared interior node
-    assert np.isfinite(r[:, :, 2:]).all()
+    lr = np.log(r[:, 0, 0])
+    assert r[0, 0, 0] == src["density"][0] and r[-1, 0, 0] == src["density"][-1]
+    assert np.allclose(np.diff(lr), (lr[-1] - lr[0]) / 199, rtol=1e-9)
+    assert np.array_equal(r[:, :, 1], rows[:1, :, 1].repeat(200, 0))  # one energy axis for every row
+    assert np.array_equal(r[0], rows[0]) and np.array_equal(r[-1], rows[-1])
+    assert (r[:, :, 3] >= 0).all() and np.isfinite(r).all()
+    # a new row between source rows k-1 and k: at every source temperature its
+    # energy lies between theirs, so its T at a given u does as well
+    k = np.searchsorted(src["density"], r[1:-1, 0, 0])
+    lo, hi = rows[k - 1, :, 2], rows[k, :, 2]
+    T = r[1:-1, :, 2]
+    assert np.all((T >= np.minimum(lo, hi) * (1 - 1e-12)) & (T <= np.maximum(lo, hi) * (1 + 1e-12)))
