@@ -150,11 +150,16 @@ def default_params(**kw) -> SimParams:
 
 
 class Simulation:
-    """Particles resident in HBM; each step runs src/main.cpp:125-173 on the GPU."""
+    """Particles resident in HBM; each step runs src/main.cpp:125-173 on the GPU.
+
+    A context holds one simulation: creating another on the same context
+    replaces it, and the replaced object then raises on every call."""
 
     def __init__(self, ctx: Context, ic: dict, **params):
         self.ctx = ctx
         self._L = lib()
+        ctx._sim_serial = getattr(ctx, "_sim_serial", 0) + 1
+        self._serial = ctx._sim_serial
         self.params = default_params(**params)
         pos = np.ascontiguousarray(ic["pos"], dtype=np.float64)
         vel = np.ascontiguousarray(ic["vel"], dtype=np.float64)
@@ -171,8 +176,14 @@ class Simulation:
                                       ptr(dens), ptr(eng), ptr(mat), ptr(sys_), ptr(ids)))
         self.nsys = max(1, self.params.nsys)
 
+    @property
+    def _h(self):
+        if getattr(self.ctx, "_sim_serial", None) != self._serial:
+            raise RuntimeError("this Simulation was replaced by a newer one on the same context")
+        return self.ctx.handle
+
     def step(self, n: int = 1):
-        check(self._L.sphgpu_sim_step(self.ctx.handle, int(n)))
+        check(self._L.sphgpu_sim_step(self._h, int(n)))
 
     def get(self, field: str) -> np.ndarray:
         f = field.upper()
@@ -184,38 +195,38 @@ class Simulation:
             out = np.empty(self.n, dtype=np.int64)
         else:
             out = np.empty(self.n)
-        check(self._L.sphgpu_sim_get(self.ctx.handle, FIELD[f], ptr(out)))
+        check(self._L.sphgpu_sim_get(self._h, FIELD[f], ptr(out)))
         return out
 
     def time(self):
         dt = np.empty(self.nsys)
         t = np.empty(self.nsys)
-        check(self._L.sphgpu_sim_get_time(self.ctx.handle, ptr(dt), ptr(t)))
+        check(self._L.sphgpu_sim_get_time(self._h, ptr(dt), ptr(t)))
         return dt, t
 
     def stage_ms(self) -> np.ndarray:
         ms = np.zeros(8)
-        check(self._L.sphgpu_sim_stage_ms(self.ctx.handle, ptr(ms), 8))
+        check(self._L.sphgpu_sim_stage_ms(self._h, ptr(ms), 8))
         return ms
 
     def list_builds(self) -> tuple[int, int]:
         """(gather-list constructions, force evaluations) since init."""
         g, f = C.c_int64(), C.c_int64()
-        check(self._L.sphgpu_sim_list_builds(self.ctx.handle, C.byref(g), C.byref(f)))
+        check(self._L.sphgpu_sim_list_builds(self._h, C.byref(g), C.byref(f)))
         return g.value, f.value
 
     def set_stats(self, on: bool = True):
-        check(self._L.sphgpu_sim_set_stats(self.ctx.handle, int(bool(on))))
+        check(self._L.sphgpu_sim_set_stats(self._h, int(bool(on))))
 
     def kernel_stats(self) -> dict:
         names = ["dens", "drvt", "hydro", "grav", "dens_fp64"]
         arr = (KernelStat * len(names))()
-        check(self._L.sphgpu_sim_kernel_stats(self.ctx.handle, arr, len(names)))
+        check(self._L.sphgpu_sim_kernel_stats(self._h, arr, len(names)))
         return {names[k]: {f: getattr(arr[k], f) for f, _ in KernelStat._fields_} for k in range(len(names))}
 
     def lists(self, kind: int):
         ng, nj, ns = C.c_int64(), C.c_int64(), C.c_int64()
-        h = self.ctx.handle
+        h = self._h
         check(self._L.sphgpu_sim_lists(h, kind, C.byref(ng), C.byref(nj), C.byref(ns), None, None, None, None, None))
         G = ng.value
         gstart = np.empty(G + 1, dtype=np.int32)
@@ -233,9 +244,9 @@ class Simulation:
 
     def nodes(self) -> np.ndarray:
         nn = C.c_int64()
-        check(self._L.sphgpu_sim_nodes(self.ctx.handle, C.byref(nn), None))
+        check(self._L.sphgpu_sim_nodes(self._h, C.byref(nn), None))
         m = np.empty((nn.value, 10))
-        check(self._L.sphgpu_sim_nodes(self.ctx.handle, C.byref(nn), ptr(m)))
+        check(self._L.sphgpu_sim_nodes(self._h, C.byref(nn), ptr(m)))
         return m
 
     def state(self) -> dict:

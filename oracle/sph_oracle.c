@@ -215,16 +215,29 @@ static inline double absmax3_(const double v[3])
     return m;
 }
 
+/* |d gradW / d ln h| per component (central difference), the response of a
+ * kernel gradient to a relative change of the smoothing length. */
+static void gradW_dlnh_(const double dr[3], double h, double out[3])
+{
+    const double d = 1e-5;
+    double a[3], b[3];
+    orc_gradW(dr, h * (1.0 + d), a);
+    orc_gradW(dr, h * (1.0 - d), b);
+    for (int k = 0; k < 3; ++k) out[k] = fabs(a[k] - b[k]) / (2.0 * d);
+}
+
 void orc_hydro_term_scale(const orc_epi_hydro *epi, int nip, const orc_epj_hydro *epj, int njp, double *acc_scale,
                           double *du_scale)
 {
     for (int i = 0; i < nip; ++i) {
         const orc_epi_hydro *ith = &epi[i];
         const double Ai = ith->grad_smth * ith->pres / (ith->dens * ith->dens);
+        /* |A| plus its response to a relative change of rho through P: f c^2 / rho */
+        const double Si = fabs(Ai) + ith->grad_smth * ith->snds * ith->snds / ith->dens;
         double sa = 0, su = 0;
         for (int j = 0; j < njp; ++j) {
             const orc_epj_hydro *jth = &epj[j];
-            double dr[3], dv[3], gwi[3], gwj[3], gw[3], t[3];
+            double dr[3], dv[3], gwi[3], gwj[3], gw[3], di[3], dj[3], t[3];
             for (int k = 0; k < 3; ++k) { dr[k] = ith->pos[k] - jth->pos[k]; dv[k] = ith->vel[k] - jth->vel[k]; }
             const double dvdr = dot3_(dv, dr);
             const double w_ij = (dvdr < 0) ? dvdr / sqrt(dot3_(dr, dr)) : 0;
@@ -232,20 +245,19 @@ void orc_hydro_term_scale(const orc_epi_hydro *epi, int nip, const orc_epj_hydro
             const double AV = -0.5 * v_sig * w_ij / (0.5 * (ith->dens + jth->dens)) * 0.5 * (ith->Bal + jth->Bal);
             orc_gradW(dr, ith->smth, gwi);
             orc_gradW(dr, jth->smth, gwj);
+            gradW_dlnh_(dr, ith->smth, di);
+            gradW_dlnh_(dr, jth->smth, dj);
             for (int k = 0; k < 3; ++k) gw[k] = 0.5 * (gwi[k] * ith->grad_smth + gwj[k] * jth->grad_smth);
             const double Aj = jth->grad_smth * jth->pres / (jth->dens * jth->dens);
-            /* dAV/dw: w_ij inherits the rounding of the velocities it is built from */
-            const double dAV = (dvdr < 0) ? 0.5 * fabs(ith->snds + jth->snds - 6.0 * w_ij) /
-                                                (0.5 * (ith->dens + jth->dens)) * 0.5 * (ith->Bal + jth->Bal)
-                                          : 0.0;
-            const double vs = sqrt(dot3_(ith->vel, ith->vel)) + sqrt(dot3_(jth->vel, jth->vel));
-            for (int k = 0; k < 3; ++k)
-                t[k] = jth->mass * (fabs(Ai * gwi[k]) + fabs(Aj * gwj[k]) + (fabs(AV) + dAV * vs) * fabs(gw[k]));
+            const double Sj = fabs(Aj) + jth->grad_smth * jth->snds * jth->snds / jth->dens;
+            double su_k = 0;
+            for (int k = 0; k < 3; ++k) {
+                const double gbar = fabs(gw[k]) + 0.5 * (ith->grad_smth * di[k] + jth->grad_smth * dj[k]);
+                t[k] = jth->mass * (Si * (fabs(gwi[k]) + di[k]) + Sj * (fabs(gwj[k]) + dj[k]) + fabs(AV) * gbar);
+                su_k += fabs(dv[k]) * gbar;
+            }
             sa += absmax3_(t);
-            /* dv = v_i - v_j cancels for bodies in bulk motion: a 1-ulp change of
-               either velocity moves dv by ~eps (|v_i| + |v_j|) */
-            su += jth->mass * ((fabs(Ai) + 0.5 * fabs(AV)) * vs * sqrt(dot3_(gw, gw)) +
-                               0.5 * dAV * vs * fabs(dot3_(dv, gw)));
+            su += jth->mass * (Si + 0.5 * fabs(AV)) * su_k;
         }
         acc_scale[i] = sa;
         du_scale[i] = su;
@@ -260,17 +272,16 @@ void orc_drvt_term_scale(const orc_epi_drvt *epi, int nip, const orc_epj_drvt *e
         double sd = 0, sr = 0;
         for (int j = 0; j < njp; ++j) {
             const orc_epj_drvt *jth = &epj[j];
-            double dr[3], dv[3], gw[3], c[3];
+            double dr[3], dv[3], gw[3], dg[3], g[3], c[3];
             for (int k = 0; k < 3; ++k) { dr[k] = ith->pos[k] - jth->pos[k]; dv[k] = ith->vel[k] - jth->vel[k]; }
             orc_gradW(dr, ith->smth, gw);
-            /* conditioning of dv = v_i - v_j, as for du in orc_hydro_term_scale */
-            const double vs = sqrt(dot3_(ith->vel, ith->vel)) + sqrt(dot3_(jth->vel, jth->vel));
-            const double gn = sqrt(dot3_(gw, gw));
-            sd += jth->mass * (fabs(dot3_(dv, gw)) + vs * gn);
-            c[0] = fabs(dv[1] * gw[2]) + fabs(dv[2] * gw[1]);
-            c[1] = fabs(dv[2] * gw[0]) + fabs(dv[0] * gw[2]);
-            c[2] = fabs(dv[0] * gw[1]) + fabs(dv[1] * gw[0]);
-            sr += jth->mass * (absmax3_(c) + vs * gn);
+            gradW_dlnh_(dr, ith->smth, dg);
+            for (int k = 0; k < 3; ++k) g[k] = fabs(gw[k]) + dg[k];
+            sd += jth->mass * (fabs(dv[0]) * g[0] + fabs(dv[1]) * g[1] + fabs(dv[2]) * g[2]);
+            c[0] = fabs(dv[1]) * g[2] + fabs(dv[2]) * g[1];
+            c[1] = fabs(dv[2]) * g[0] + fabs(dv[0]) * g[2];
+            c[2] = fabs(dv[0]) * g[1] + fabs(dv[1]) * g[0];
+            sr += jth->mass * absmax3_(c);
         }
         div_scale[i] = sd / ith->dens;
         rot_scale[i] = sr / ith->dens;

@@ -126,14 +126,17 @@ int64_t orc_list_sph(const orc_tree *t, const orc_state *s, int64_t g, int kind,
 int orc_list_grav(const orc_tree *t, const orc_state *s, int64_t g, double theta,
                   int32_t *epj, int64_t cap_j, int64_t *nj, int32_t *spj, int64_t cap_s, int64_t *ns);
 
-/* Size of the summed terms, per i (test diagnostics: the scale on which sums
- * that cancel are compared).  hydro: sum_j |m_j (A_i gWi + A_j gWj + AV gW)|_max
- * and sum_j m_j (|A_i| + |AV|/2) (|v_i| + |v_j|) |gW|; derivative:
- * sum_j m_j (|dv.gW| + (|v_i| + |v_j|) |gW|) / rho_i and the same with
- * |dv x gW|_max; gravity: sum_j G m_j / (r^2 + eps_i^2).  The (|v_i| + |v_j|)
- * factors bound how far dv = v_i - v_j moves under a 1-ulp change of either
- * velocity (bodies in bulk motion make dv a cancelling difference); the
- * AV terms add the same through w_ij. */
+/* Scale on which sums whose terms cancel are compared (test diagnostics),
+ * per i: the size of the summed terms plus their first-order response to a
+ * relative change of the inputs that two correct evaluations compute only to
+ * rounding -- the smoothing lengths (|d gradW / d ln h|, large for pairs near
+ * the edge of the support, where gradW ~ (1 - s)^7) and, through the
+ * pressure, the density (f c^2 / rho beside |A| = |f P / rho^2|).  hydro:
+ * sum_j max_k m_j (S_i (|gWi_k| + |dgWi_k|) + S_j (...) + |AV| (|gW_k| +
+ * |dgW_k|)) and sum_j m_j (S_i + |AV|/2) sum_k |dv_k| (|gW_k| + |dgW_k|);
+ * derivative: sum_j m_j sum_k |dv_k| (|gW_k| + |dgW_k|) / rho_i and the same
+ * with the terms of dv x gW; gravity: sum_j G m_j / (r^2 + eps_i^2).
+ * Positions and velocities are taken as shared bit for bit. */
 void orc_hydro_term_scale(const orc_epi_hydro *epi, int nip, const orc_epj_hydro *epj, int njp, double *acc_scale,
                           double *du_scale);
 void orc_drvt_term_scale(const orc_epi_drvt *epi, int nip, const orc_epj_drvt *epj, int njp, double *div_scale,

@@ -1,7 +1,8 @@
-"""Batching form: several independent impacts in one device-resident
-simulation (one launch per stage over the groups of every system), against the
-oracle's multi-system restatement; lists never cross systems and each system
-keeps its own dt and time."""
+"""Batching form: independent impacts of uneven particle counts in one
+device-resident simulation (one launch per stage over the groups of every
+system), against the oracle's multi-system restatement, in both precision
+forms.  Lists never cross systems and each system keeps its own dt and time.
+Comparisons run step by step from the GPU's own state (tests/parity.py)."""
 import numpy as np
 import pytest
 
@@ -9,20 +10,21 @@ from fdps_sph_amd import Simulation
 from fdps_sph_amd import eos_tables as E
 from fdps_sph_amd import ic as IC
 
-from . import helpers as H
-from .test_gpu_parity import TOL, _compare, _ic_state
+from .parity import fp64_check, mixed_check, run_per_step
 
 pytestmark = pytest.mark.gpu
+NSYS = 8
 
 
 @pytest.fixture(scope="module")
 def batch_ic():
-    return IC.batch_of_impacts(n_systems=3, n_min=600, n_max=2500, seed=3)
+    # log-uniform counts in [600, 4000], speeds U[5, 15] km/s, angles U[0, 75] deg
+    return IC.batch_of_impacts(n_systems=NSYS, n_min=600, n_max=4000, seed=3)
 
 
 def test_batch_lists_stay_in_system(ctx64, batch_ic):
     print("[batch lists]", flush=True)
-    sim = Simulation(ctx64, batch_ic, nsys=3)
+    sim = Simulation(ctx64, batch_ic, nsys=NSYS)
     for kind in (0, 1, 2):
         L = sim.lists(kind)
         sys_ = sim.get("SYS")
@@ -34,18 +36,21 @@ def test_batch_lists_stay_in_system(ctx64, batch_ic):
 
 
 def test_batch_matches_oracle(ctx64, oracle, batch_ic):
-    print("[batch vs oracle, 3 systems]", flush=True)
-    sim = Simulation(ctx64, batch_ic, nsys=3)
-    os_ = _ic_state(oracle, batch_ic)
+    print(f"[batch vs oracle, {NSYS} systems, {len(batch_ic['mass'])} particles, counts "
+          f"{batch_ic['counts'].tolist()}]", flush=True)
+    sim = Simulation(ctx64, batch_ic, nsys=NSYS)
     tabs = [oracle.Table(E.PATHS[k]) for k in ("granite", "iron", "dunite")]
-    osim = oracle.Sim(os_, tabs, oracle.params(nsys=3))
-    osim.init()
-    _compare(sim.state(), os_, tabs)
-    sim.step(1)
-    osim.step()
-    dt, t = sim.time()
-    print("   dt per system", dt, "oracle", osim.dt, flush=True)
-    assert len(set(dt.tolist())) == 3  # independent time steps
-    assert H.rel_err(dt, osim.dt) <= TOL
-    assert H.rel_err(t, osim.time) <= TOL
-    _compare(sim.state(), os_, tabs)
+    prm = oracle.params(nsys=NSYS)
+    run_per_step(sim, oracle, batch_ic, tabs, prm, 2, fp64_check(tabs, prm))
+    dt, _ = sim.time()
+    assert len(set(dt.tolist())) == NSYS  # independent time steps
+
+
+def test_batch_precision_form_matches_oracle(ctxmix, oracle, batch_ic):
+    """The batching form in the precision form (the 64-impact workload's
+    arithmetic) against the FP64 oracle on the same inputs, each step."""
+    print(f"[mixed batch vs oracle, {NSYS} systems]", flush=True)
+    sim = Simulation(ctxmix, batch_ic, nsys=NSYS)
+    tabs = [oracle.Table(E.PATHS[k]) for k in ("granite", "iron", "dunite")]
+    prm = oracle.params(nsys=NSYS)
+    run_per_step(sim, oracle, batch_ic, tabs, prm, 2, mixed_check(tabs, prm), dt_tol=1e-5)

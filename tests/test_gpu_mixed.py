@@ -82,8 +82,8 @@ def test_mixed_simulation_vs_fp64(ctx64, ctx32):
     ia, ib = np.argsort(sa["id"]), np.argsort(sb["id"])
     # size of the summed terms of the FP64 state, per particle
     from oracle import oracle as O
-    from .test_gpu_parity import _oracle_state
-    sc = O.state_term_scales(_oracle_state(O, sa), O.params())
+    from .parity import oracle_state
+    sc = O.state_term_scales(oracle_state(O, sa), O.params())
 
     def err(f, scale=None):
         x, y = sb[f][ib], sa[f][ia]

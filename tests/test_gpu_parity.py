@@ -1,7 +1,8 @@
 """GPU parity: the HIP path through the C-ABI against the CPU oracle on the
-same seeded inputs.  Tolerance for the plain FP64 path: 1e-12 relative
-(cancellation-prone vector sums are normalised by the size of the summed
-terms, see helpers.rel_err); integer outputs (neighbour counts, lists) exact."""
+same inputs.  Tolerance for the plain FP64 path: 1e-12 relative (sums whose
+terms cancel are normalised by the size of the summed terms); integer outputs
+(neighbour counts, lists) exact.  Full steps are compared one at a time, each
+from the GPU's own state loaded into the oracle (tests/parity.py)."""
 import numpy as np
 import pytest
 
@@ -10,9 +11,9 @@ from fdps_sph_amd import eos_tables as E
 from fdps_sph_amd import ic as IC
 
 from . import helpers as H
+from .parity import TOL, compare, fp64_check, ic_state, oracle_state, run_per_step
 
 pytestmark = pytest.mark.gpu
-TOL = 1.0e-12
 
 
 def _print(*a):
@@ -140,40 +141,6 @@ def test_eos_unknown_material(ctx64):
 
 
 # ------------------------------------------------------------------ lists and tree
-def _oracle_state(oracle, st):
-    n = len(st["mass"])
-    s = oracle.State(n)
-    s.x[:], s.y[:], s.z[:] = st["pos"].T
-    s.vx[:], s.vy[:], s.vz[:] = st["vel"].T
-    s.ax[:], s.ay[:], s.az[:] = st["acc"].T
-    s.gx[:], s.gy[:], s.gz[:] = st["grav"].T
-    s.vhx[:], s.vhy[:], s.vhz[:] = st["velhalf"].T
-    s.rotx[:], s.roty[:], s.rotz[:] = st["rotv"].T
-    for a, b in [("pot", "pot"), ("m", "mass"), ("h", "smth"), ("rho", "dens"), ("u", "eng"), ("p", "pres"),
-                 ("cs", "snds"), ("divv", "divv"), ("gs", "gradh"), ("bal", "bal"), ("du", "engdot"),
-                 ("dti", "dt"), ("uh", "enghalf")]:
-        getattr(s, a)[:] = st[b]
-    s.mat[:] = st["mat"]
-    s.sys[:] = st["sys"]
-    s.ngb[:] = st["ngb"]
-    s.id[:] = st["id"]
-    return s
-
-
-def _ic_state(oracle, ic):
-    n = len(ic["mass"])
-    s = oracle.State(n)
-    s.x[:], s.y[:], s.z[:] = ic["pos"].T
-    s.vx[:], s.vy[:], s.vz[:] = ic["vel"].T
-    s.m[:] = ic["mass"]
-    s.rho[:] = ic["dens"]
-    s.u[:] = ic["eng"]
-    s.mat[:] = ic["mat"]
-    if "sys" in ic:
-        s.sys[:] = ic["sys"]
-    return s
-
-
 @pytest.fixture(scope="module")
 def small_impact():
     return IC.granite_impact(1800, 200, seed=1000)
@@ -185,7 +152,7 @@ def test_lists_match_bruteforce(ctx64, oracle, small_impact):
     for kind in (0, 1):
         L = sim.lists(kind)
         st = sim.state()
-        os_ = _oracle_state(oracle, st)
+        os_ = oracle_state(oracle, st)
         t = oracle.Tree(os_, oracle.params())
         assert np.array_equal(t.group_start(), L["group_start"])
         if kind == 1:
@@ -198,7 +165,7 @@ def test_lists_match_bruteforce(ctx64, oracle, small_impact):
             assert np.array_equal(t.list_sph(g, kind, brute=False), ref)
     L = sim.lists(2)
     st = sim.state()
-    os_ = _oracle_state(oracle, st)
+    os_ = oracle_state(oracle, st)
     t = oracle.Tree(os_, oracle.params())
     mom = sim.nodes()
     assert mom.shape[0] == t.nnodes
@@ -211,89 +178,88 @@ def test_lists_match_bruteforce(ctx64, oracle, small_impact):
         assert np.array_equal(L["s_idx"][L["s_off"][g]:L["s_off"][g + 1]], es), g
 
 
+# ------------------------------------------------------------------ multiwalk boundary
+def test_two_tags_in_flight_and_empty_lists(ctx64, oracle):
+    """FDPS double-buffers multiwalk batches (tag_max > 1): batches of
+    different kinds are dispatched on tags 0, 1 and 2 before any retrieve,
+    then retrieved out of order.  Walks with empty j-lists give the reference
+    functors' values: dt = 0.6 h / 0 = inf for hydro, zero force and
+    potential for gravity."""
+    _print("[two tags in flight, empty j-lists]")
+    c = H.cloud(500, 31)
+    wh = H.walks_from(c, [64, 30, 5], 31, j_frac=0.5)
+    wg = H.walks_from(c, [20, 64, 64], 32, j_frac=0.7)
+    wd = H.walks_from(c, [40, 64], 33, j_frac=0.6)
+    hi = [H.epi_hydro(c, i) for i, _ in wh]
+    hj = [H.epj_hydro(c, j) for _, j in wh]
+    hj[2] = hj[2][:0]
+    gi = [H.epi_grav(c, i) for i, _ in wg]
+    gj = [H.epj_grav(c, j) for _, j in wg]
+    gj[0] = gj[0][:0]
+    spj = [np.zeros(0, dtype=oracle.SPJ_MONO) for _ in wg]
+    di = [H.epi_dens(c, i) for i, _ in wd]
+    dj = [H.epj_dens(c, j) for _, j in wd]
+    n0 = ctx64.dispatch("hydro", hi, hj, tag=0)
+    n1 = ctx64.dispatch("grav", gi, gj, spj, tag=1)
+    n2 = ctx64.dispatch("dens", di, dj, tag=2)
+    # a retrieve with a null n_epi array is an argument error and leaves the batch pending
+    from fdps_sph_amd import _lib
+    import ctypes as C
+    pr = (C.c_void_p * 3)()
+    assert ctx64._L.sphgpu_grav_retrieve(ctx64.handle, 1, 3, None, pr) == _lib.ERR_ARG
+    rg = ctx64.retrieve("grav", n1, tag=1)
+    rd = ctx64.retrieve("dens", n2, tag=2)
+    rh = ctx64.retrieve("hydro", n0, tag=0)
+    for a, b, r in zip(hi, hj, rh):
+        ref = oracle.calc_hydro(a, b)
+        if len(b) == 0:
+            assert np.all(np.isinf(r["dt"])) and np.all(np.isinf(ref["dt"]))
+            assert np.all(r["acc"] == 0) and np.all(r["eng_dot"] == 0)
+            continue
+        sa, su = oracle.hydro_term_scale(a, b)
+        assert H.rel_err(r["acc"], ref["acc"], sa[:, None]) <= TOL
+        assert H.rel_err(r["eng_dot"], ref["eng_dot"], su) <= TOL
+        assert H.rel_err(r["dt"], ref["dt"]) <= TOL
+    for a, b, r in zip(gi, gj, rg):
+        ref = oracle.calc_grav(a, b)
+        if len(b) == 0:
+            assert np.all(r["acc"] == 0) and np.all(r["pot"] == 0)
+            continue
+        assert H.rel_err(r["pot"], ref["pot"]) <= TOL
+        assert H.rel_err(r["acc"], ref["acc"], np.abs(ref["acc"]).max() * 1e-3) <= TOL
+    for a, b, r in zip(di, dj, rd):
+        ref = oracle.calc_density(a, b)
+        assert H.rel_err(r["dens"], ref["dens"]) <= TOL
+    from fdps_sph_amd import SphGpuError
+    with pytest.raises(SphGpuError):  # each tag was retrieved once
+        ctx64.retrieve("hydro", n0, tag=0)
+
+
 # ------------------------------------------------------------------ full step
-def _pres_scale(tables, rho, u, mat, rel=1e-7):
-    """|rho dP/drho| + |u dP/du| of the EoS tables at each particle's (rho, u)
-    (central differences of the oracle's bilinear lookup): how far P moves
-    per unit of relative change in its inputs."""
-    out = np.zeros(len(rho))
-    for k in range(len(rho)):
-        t, r, e = tables[int(mat[k])], float(rho[k]), float(u[k])
-        dr = (t.pressure(r * (1 + rel), e) - t.pressure(r * (1 - rel), e)) / (2 * rel)
-        de = (t.pressure(r, e * (1 + rel)) - t.pressure(r, e * (1 - rel))) / (2 * rel) if e != 0 else 0.0
-        out[k] = abs(dr) + abs(de)
-    return out
-
-
-def _compare(st, os_, tables, tol=TOL, prm=None):
-    """Per-particle relative errors; sums whose terms cancel (acc, du/dt,
-    gravity, div v, rot v) are measured against the size of their summed terms
-    (the oracle's per-particle sum of |term|) where that exceeds the value."""
-    from oracle import oracle as O
-    assert np.array_equal(st["id"], os_.id)
-    assert np.array_equal(st["ngb"], os_.ngb)
-    sc = O.state_term_scales(os_, prm if prm is not None else O.params(nsys=int(os_.sys.max()) + 1))
-    errs = {}
-    errs["dens"] = H.rel_err(st["dens"], os_.rho)
-    errs["smth"] = H.rel_err(st["smth"], os_.h)
-    acc = np.stack([os_.ax, os_.ay, os_.az], 1)
-    errs["acc"] = H.rel_err(st["acc"], acc, sc["acc"][:, None])
-    grav = np.stack([os_.gx, os_.gy, os_.gz], 1)
-    errs["grav"] = H.rel_err(st["grav"], grav, sc["grav"][:, None])
-    errs["du"] = H.rel_err(st["engdot"], os_.du, sc["du"])
-    errs["divv"] = H.rel_err(st["divv"], os_.divv, sc["div"])
-    errs["rotv"] = H.rel_err(st["rotv"], np.stack([os_.rotx, os_.roty, os_.rotz], 1), sc["rot"][:, None])
-    errs["gradh"] = H.rel_err(st["gradh"], os_.gs)
-    errs["pot"] = H.rel_err(st["pot"], os_.pot)
-    errs["dt"] = H.rel_err(st["dt"], os_.dti)
-    errs["pos"] = H.rel_err(st["pos"], np.stack([os_.x, os_.y, os_.z], 1), 1e6)
-    # P is a table lookup (bit-exact, see test_eos_bilinear) of rho and u, which
-    # carry their own rounding-level differences: P is compared on the scale of
-    # the bulk modulus rho c^2 and of the table's own sensitivity to its inputs
-    # (the bilinear interpolant can be far steeper than c^2 between grid points)
-    psc = np.maximum(os_.rho * os_.cs ** 2, _pres_scale(tables, os_.rho, os_.u, os_.mat))
-    errs["pres"] = H.rel_err(st["pres"], os_.p, psc)
-    _print("   max rel err:", {k: f"{v:.2e}" for k, v in errs.items()})
-    for k, v in errs.items():
-        assert v <= tol, (k, v)
+def _tables(oracle, grid=None):
+    return [oracle.Table(cells=E.table(k, grid)) for k in ("granite", "iron", "dunite")]
 
 
 def test_full_step_reference_snapshot(ctx64, oracle):
     """2,000 particles cut from the reference's own input/tar.dat (evolved,
-    half-vaporised core and mantle, granite and iron tables): init + 1 step."""
+    half-vaporised core and mantle, granite and iron tables): init + 2 steps."""
     import os
     from fdps_sph_amd import snapshot as SN
     _print("[full step, reference snapshot subset]")
     ic = SN.to_ic(SN.read_ascii(os.path.join(os.path.dirname(__file__), "golden", "tar_subset_2000.dat")))
-    sim = Simulation(ctx64, ic)
-    os_ = _ic_state(oracle, ic)
-    os_.id[:] = ic["id"]
-    tabs = [oracle.Table(E.PATHS[k]) for k in ("granite", "iron", "dunite")]
-    osim = oracle.Sim(os_, tabs, oracle.params())
-    osim.init()
-    _compare(sim.state(), os_, tabs)
-    sim.step(1)
-    osim.step()
-    _compare(sim.state(), os_, tabs)
+    tabs = _tables(oracle)
+    prm = oracle.params()
+    run_per_step(Simulation(ctx64, ic), oracle, ic, tabs, prm, 2, fp64_check(tabs, prm))
 
 
 def test_full_step_granite_10k(ctx64, oracle):
-    """10,000-particle granite impact (9,000 + 1,000): init + 2 steps vs oracle."""
+    """The 10,000-particle granite impact (9,000 + 1,000, 9 km/s, 45 deg):
+    init + 2 steps."""
     _print("[full step granite 10k]")
     ic = IC.granite_impact(9000, 1000, seed=1000)
-    sim = Simulation(ctx64, ic)
-    os_ = _ic_state(oracle, ic)
-    tabs = [oracle.Table(E.PATHS["granite"]), oracle.Table(E.PATHS["iron"]), oracle.Table(E.PATHS["dunite"])]
-    osim = oracle.Sim(os_, tabs, oracle.params(fast_lists=1))
-    osim.init()
-    _compare(sim.state(), os_, tabs)
-    for step in range(2):
-        sim.step(1)
-        osim.step()
-        dt, t = sim.time()
-        _print(f"   step {step + 1}: dt gpu {dt[0]!r} oracle {osim.dt[0]!r}")
-        assert abs(dt[0] - osim.dt[0]) <= TOL * osim.dt[0]
-        _compare(sim.state(), os_, tabs)
+    tabs = _tables(oracle)
+    prm = oracle.params()
+    run_per_step(Simulation(ctx64, ic), oracle, ic, tabs, prm, 2, fp64_check(tabs, prm))
 
 
 def test_gather_list_reuse(ctx64):
@@ -309,6 +275,19 @@ def test_gather_list_reuse(ctx64):
     assert evals <= builds <= 2 * evals
 
 
+_FIELDS = ("POS", "VEL", "DENS", "SMTH", "ENG", "ACC", "GRAV", "ENGDOT", "DT", "ID", "NGB", "PRES", "DIVV")
+
+
+def _run(ctx, ic, nsteps, stats=False, **params):
+    """Run a simulation to completion and keep its outputs (a context holds
+    one simulation at a time)."""
+    sim = Simulation(ctx, ic, **params)
+    if stats:
+        sim.set_stats(True)
+    sim.step(nsteps)
+    return {f: sim.get(f) for f in _FIELDS}, sim.list_builds()
+
+
 @pytest.mark.parametrize("ctx", ["ctx64", "ctxmix"])
 def test_list_overflow_redo(ctx, request):
     """List capacities far too small for the lists: every force evaluation
@@ -317,14 +296,12 @@ def test_list_overflow_redo(ctx, request):
     bit, and the gather-list constructions are counted once."""
     c = request.getfixturevalue(ctx)
     ic = IC.granite_impact(9000, 1000, seed=1001)
-    big = Simulation(c, ic)
-    small = Simulation(c, ic, list_cap_sph=16, list_cap_epj=16, list_cap_spj=16)
-    big.step(2)
-    small.step(2)
-    _print(f"[list overflow redo] builds {big.list_builds()} vs {small.list_builds()}")
-    assert big.list_builds() == small.list_builds()
-    for f in ("POS", "VEL", "DENS", "SMTH", "ENG", "ACC", "GRAV", "ENGDOT", "DT", "ID"):
-        assert np.array_equal(big.get(f), small.get(f)), f
+    big, bb = _run(c, ic, 2)
+    small, sb = _run(c, ic, 2, list_cap_sph=16, list_cap_epj=16, list_cap_spj=16)
+    _print(f"[list overflow redo] builds {bb} vs {sb}")
+    assert bb == sb
+    for f in _FIELDS:
+        assert np.array_equal(big[f], small[f]), f
 
 
 @pytest.mark.parametrize("ctx", ["ctx64", "ctxmix"])
@@ -336,17 +313,14 @@ def test_frontier_spill_and_overflow(ctx, request):
     bit, in either precision form."""
     c = request.getfixturevalue(ctx)
     ic = IC.granite_impact(9000, 1000, seed=1002)
-    ref = Simulation(c, ic)
-    spill = Simulation(c, ic, list_front_lds=1)
-    tiny = Simulation(c, ic, list_front_lds=1, list_cap_front=4)
-    for sim in (ref, spill, tiny):
-        sim.step(2)
-    _print(f"[frontier spill] builds {ref.list_builds()} {spill.list_builds()} {tiny.list_builds()}")
-    assert ref.list_builds() == spill.list_builds() == tiny.list_builds()
-    for f in ("POS", "VEL", "DENS", "SMTH", "ENG", "ACC", "GRAV", "ENGDOT", "DT", "ID", "NGB"):
-        a = ref.get(f)
-        assert np.array_equal(a, spill.get(f)), f
-        assert np.array_equal(a, tiny.get(f)), f
+    ref, rb = _run(c, ic, 2)
+    spill, sb = _run(c, ic, 2, list_front_lds=1)
+    tiny, tb = _run(c, ic, 2, list_front_lds=1, list_cap_front=4)
+    _print(f"[frontier spill] builds {rb} {sb} {tb}")
+    assert rb == sb == tb
+    for f in _FIELDS:
+        assert np.array_equal(ref[f], spill[f]), f
+        assert np.array_equal(ref[f], tiny[f]), f
 
 
 @pytest.mark.parametrize("ctx", ["ctx64", "ctxmix"])
@@ -357,12 +331,36 @@ def test_second_stream_matches_serial(ctx, request):
     for bit, over steps whose force evaluations rebuild and reuse lists."""
     c = request.getfixturevalue(ctx)
     ic = IC.granite_impact(20000, 2000, seed=1003)
-    over = Simulation(c, ic)
-    serial = Simulation(c, ic)
-    serial.set_stats(True)
-    for sim in (over, serial):
-        sim.step(3)
-    _print(f"[second stream] builds {over.list_builds()} vs {serial.list_builds()}")
-    assert over.list_builds() == serial.list_builds()
-    for f in ("POS", "VEL", "DENS", "SMTH", "ENG", "ACC", "GRAV", "ENGDOT", "DT", "ID", "NGB", "PRES", "DIVV"):
-        assert np.array_equal(over.get(f), serial.get(f)), f
+    over, ob = _run(c, ic, 3)
+    serial, sb = _run(c, ic, 3, stats=True)
+    _print(f"[second stream] builds {ob} vs {sb}")
+    assert ob == sb
+    for f in _FIELDS:
+        assert np.array_equal(over[f], serial[f]), f
+
+
+def test_replaced_simulation_raises(ctx64):
+    ic = IC.granite_impact(450, 50, seed=5)
+    a = Simulation(ctx64, ic)
+    Simulation(ctx64, ic)
+    with pytest.raises(RuntimeError):
+        a.step(1)
+
+
+def test_list_export_leaves_the_state(ctx64):
+    """sphgpu_sim_lists rebuilds the tree and lists of the current state; a
+    single size-only call (no arrays) between steps changes nothing that
+    follows, bit for bit."""
+    import ctypes as C
+    ic = IC.granite_impact(9000, 1000, seed=1004)
+    ref, _ = _run(ctx64, ic, 2)
+    sim = Simulation(ctx64, ic)
+    sim.step(1)
+    ng, nj, ns = C.c_int64(), C.c_int64(), C.c_int64()
+    from fdps_sph_amd._lib import check
+    check(sim._L.sphgpu_sim_lists(sim.ctx.handle, 1, C.byref(ng), C.byref(nj), C.byref(ns), None, None, None, None,
+                                  None))
+    assert ng.value > 0 and nj.value > 0
+    sim.step(1)
+    for f in _FIELDS:
+        assert np.array_equal(ref[f], sim.get(f)), f
