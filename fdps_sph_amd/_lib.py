@@ -21,6 +21,7 @@ ERR_TAG = -3
 ERR_HIP = -4
 ERR_EOS = -5
 ERR_STATE = -6
+NOALLOC_MAX_N = 1048576  # SPHGPU_NOALLOC_MAX_N
 
 PREC_FP64 = 0
 PREC_MIXED = 1
@@ -79,6 +80,7 @@ def lib() -> C.CDLL:
     L.sphgpu_eos_load_table.argtypes = [P, i32, C.c_char_p]
     L.sphgpu_eos_set_table.argtypes = [P, i32, i32, i32, P]
     L.sphgpu_eos_pressure.argtypes = [P, i64, P, P, P, P, P]
+    L.sphgpu_eos_pressure_noalloc.argtypes = [P, i64, P, P, P, P, P]
     L.sphgpu_eos_property.argtypes = [P, i64, P, P, P, i32, P]
     L.sphgpu_eos_internal_energy.argtypes = [P, i64, P, P, P, i32, P]
     L.sphgpu_sim_default_params.argtypes = [C.POINTER(SimParams)]

@@ -3,11 +3,24 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 
 #include "../../include/fdps_sph_gpu.h"
 #include "internal.hpp"
 
 using namespace sphgpu;
+
+// ------------------------------------------------------------------ fixed scratch (no-alloc EoS entry)
+// Staging of sphgpu_eos_pressure_noalloc: material ids, density and energy in,
+// pressure and sound speed out, for at most SPHGPU_NOALLOC_MAX_N particles, each
+// field at a 256-byte aligned offset.  Every GPU the library runs on holds its
+// own copy until the process ends.  In host code `scratch` names only a host
+// stand-in: the device address comes from hipGetSymbolAddress, looked up once
+// per device when a context is created (bind_scratch) and used everywhere else.
+constexpr int64_t kNoAllocMaxN = SPHGPU_NOALLOC_MAX_N;
+constexpr size_t kNoAllocAlign = 256;
+constexpr size_t kNoAllocBytes = (size_t) kNoAllocMaxN * (sizeof(int32_t) + 4 * sizeof(double)) + 5 * kNoAllocAlign;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 namespace {
 
@@ -185,6 +198,7 @@ struct sphgpu_ctx {
     // EoS staging
     DevBuf<double> e_in1, e_in2, e_out1, e_out2;
     DevBuf<int32_t> e_mat;
+    void *scratch = nullptr;  // this device's address of the fixed no-alloc scratch
 };
 
 namespace {
@@ -194,6 +208,24 @@ void set_device(sphgpu_ctx *c) { SPH_HIP(hipSetDevice(c->device)); }
 void check_ctx(sphgpu_ctx *c) {
     if (!c) throw Error(SPHGPU_ERR_ARG, "null context");
     set_device(c);
+}
+
+// device address of `scratch` on each device ordinal, looked up once
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
+
+void *bind_scratch(int device) {  // `device` is current
+    if (device < 0 || device >= kMaxDevices) throw Error(SPHGPU_ERR_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Error(SPHGPU_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[device] = p;
+    }
+    return g_scratch[device];
 }
 
 void upload_mats(sphgpu_ctx *c) {
@@ -698,6 +730,7 @@ int32_t sphgpu_create(sphgpu_ctx **ctx, int32_t device, int32_t precision) {
         c->device = device;
         c->precision = precision;
         SPH_HIP(hipSetDevice(device));
+        c->scratch = bind_scratch(device);
         SPH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         SPH_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
         upload_mats(c.get());
@@ -900,6 +933,42 @@ int32_t sphgpu_eos_pressure(sphgpu_ctx *c, int64_t n, const int32_t *material, c
                             c->e_out1.get(), c->e_out2.get(), c->stream);
         SPH_HIP(hipMemcpyAsync(pres, c->e_out1.get(), sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
         SPH_HIP(hipMemcpyAsync(snds, c->e_out2.get(), sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        SPH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int32_t sphgpu_eos_pressure_noalloc(sphgpu_ctx *c, int64_t n, const int32_t *material, const double *dens,
+                                    const double *eng, double *pres, double *snds) {
+    return guard([&] {
+        check_ctx(c);
+        if (n < 0) throw Error(SPHGPU_ERR_ARG, "negative n");
+        if (n > kNoAllocMaxN)
+            throw Error(SPHGPU_ERR_CAPACITY, "n exceeds the fixed scratch of SPHGPU_NOALLOC_MAX_N (1048576) particles; "
+                                             "sphgpu_eos_pressure takes any n");
+        if (n == 0) return;
+        if (!material || !dens || !eng || !pres || !snds) throw Error(SPHGPU_ERR_ARG, "null array");
+        for (int64_t i = 0; i < n; ++i) {
+            const int m = material[i];
+            if (m < 0 || m >= SPHGPU_MAX_MATERIALS || c->mats[m].nr <= 0)
+                throw Error(SPHGPU_ERR_EOS, "material id " + std::to_string(m) + " has no EoS table");
+        }
+        if (!c->scratch) throw Error(SPHGPU_ERR_STATE, "context has no scratch address");
+        // field offsets inside this device's scratch (every output is written
+        // before it is read back, so nothing needs clearing between calls)
+        auto up = [](size_t b) { return (b + kNoAllocAlign - 1) & ~(kNoAllocAlign - 1); };
+        unsigned char *base = (unsigned char *) c->scratch;
+        const size_t o_mat = 0, o_rho = up(o_mat + sizeof(int32_t) * n), o_u = up(o_rho + sizeof(double) * n),
+                     o_p = up(o_u + sizeof(double) * n), o_c = up(o_p + sizeof(double) * n);
+        if (o_c + sizeof(double) * n > kNoAllocBytes) throw Error(SPHGPU_ERR_CAPACITY, "scratch layout overflow");
+        int32_t *d_mat = (int32_t *) (base + o_mat);
+        double *d_rho = (double *) (base + o_rho), *d_u = (double *) (base + o_u);
+        double *d_p = (double *) (base + o_p), *d_c = (double *) (base + o_c);
+        SPH_HIP(hipMemcpyAsync(d_mat, material, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+        SPH_HIP(hipMemcpyAsync(d_rho, dens, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        SPH_HIP(hipMemcpyAsync(d_u, eng, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        launch_eos_pressure(n, d_mat, d_rho, d_u, c->d_mats.get(), SPHGPU_MAX_MATERIALS, d_p, d_c, c->stream);
+        SPH_HIP(hipMemcpyAsync(pres, d_p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        SPH_HIP(hipMemcpyAsync(snds, d_c, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
         SPH_HIP(hipStreamSynchronize(c->stream));
     });
 }

@@ -126,6 +126,19 @@ int32_t sphgpu_eos_set_table(sphgpu_ctx *ctx, int32_t material, int32_t nrho, in
 /* CalcPressure (src/force.h:26-32): pres/snds of n particles (host arrays). */
 int32_t sphgpu_eos_pressure(sphgpu_ctx *ctx, int64_t n, const int32_t *material, const double *dens,
                             const double *eng, double *pres, double *snds);
+/* CalcPressure with no allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as sphgpu_eos_pressure, bit for bit.  Inputs and
+ * outputs are staged through one fixed device array of SPHGPU_NOALLOC_MAX_N
+ * particles (36 MiB), which each GPU the library runs on holds until the
+ * process ends; n above SPHGPU_NOALLOC_MAX_N returns SPHGPU_ERR_CAPACITY.  All
+ * calls on one GPU share that array, so calls that overlap on one GPU (from
+ * other host threads or other contexts on the same device) race.  The EoS
+ * tables must be loaded beforehand (loading a table allocates). */
+#define SPHGPU_NOALLOC_MAX_N 1048576
+int32_t sphgpu_eos_pressure_noalloc(sphgpu_ctx *ctx, int64_t n, const int32_t *material, const double *dens,
+                                    const double *eng, double *pres, double *snds);
 /* EoS_t::{Temperature,Pressure,SoundSpeed,Entropy} as one property (2..5). */
 int32_t sphgpu_eos_property(sphgpu_ctx *ctx, int64_t n, const int32_t *material, const double *dens,
                             const double *eng, int32_t prop, double *out);
