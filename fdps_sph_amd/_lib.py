@@ -79,6 +79,7 @@ def lib() -> C.CDLL:
     L.sphgpu_grav_retrieve.argtypes = [P, i32, i32, PI, PP]
     L.sphgpu_eos_load_table.argtypes = [P, i32, C.c_char_p]
     L.sphgpu_eos_set_table.argtypes = [P, i32, i32, i32, P]
+    L.sphgpu_eos_set_tillotson.argtypes = [P, i32, P]
     L.sphgpu_eos_pressure.argtypes = [P, i64, P, P, P, P, P]
     L.sphgpu_eos_pressure_noalloc.argtypes = [P, i64, P, P, P, P, P]
     L.sphgpu_eos_property.argtypes = [P, i64, P, P, P, i32, P]

@@ -128,6 +128,8 @@ struct Batch {
 };
 
 struct MatHost {
+    int kind = 0;  // 0: rho-u table, 1: analytic Tillotson
+    double till[10] = {};
     int nr = 0, nu = 0;
     DevBuf<double> rho, u, plane, full;
     bool sorted = false;
@@ -232,6 +234,8 @@ void upload_mats(sphgpu_ctx *c) {
     MatDev h[SPHGPU_MAX_MATERIALS];
     for (int m = 0; m < SPHGPU_MAX_MATERIALS; ++m) {
         const MatHost &t = c->mats[m];
+        h[m].kind = t.kind;
+        for (int k = 0; k < 10; ++k) h[m].till[k] = t.till[k];
         h[m].nr = t.nr;
         h[m].nu = t.nu;
         h[m].rho = t.rho.get();
@@ -264,6 +268,7 @@ void set_table(sphgpu_ctx *c, int material, int nr, int nu, const double *cells)
         if (k > 0 && full[k] < full[k - 1]) sorted = false;
     }
     MatHost &t = c->mats[material];
+    t.kind = 0;
     t.nr = nr;
     t.nu = nu;
     t.sorted = sorted;
@@ -276,6 +281,25 @@ void set_table(sphgpu_ctx *c, int material, int nr, int nu, const double *cells)
 
 // Table reader of src/EoS.h:290-385 (header "# nrow ncol", '#' lines skipped,
 // one cell of up to 6 numbers per line).
+bool mat_loaded(const sphgpu_ctx *c, int m) {
+    return m >= 0 && m < SPHGPU_MAX_MATERIALS && (c->mats[m].kind == 1 || c->mats[m].nr > 0);
+}
+
+void set_tillotson(sphgpu_ctx *c, int material, const double *prm) {
+    if (material < 0 || material >= SPHGPU_MAX_MATERIALS) throw Error(SPHGPU_ERR_ARG, "material id out of range");
+    if (!prm) throw Error(SPHGPU_ERR_ARG, "null Tillotson parameters");
+    for (int k = 0; k < 10; ++k)
+        if (!std::isfinite(prm[k])) throw Error(SPHGPU_ERR_EOS, "non-finite Tillotson parameter");
+    if (!(prm[0] > 0 && prm[1] > 0 && prm[2] >= 0 && prm[3] > prm[2]))
+        throw Error(SPHGPU_ERR_EOS, "Tillotson parameters need rho0 > 0, u0 > 0 and 0 <= u_iv < u_cv");
+    MatHost &t = c->mats[material];
+    t.kind = 1;
+    for (int k = 0; k < 10; ++k) t.till[k] = prm[k];
+    t.nr = t.nu = 0;
+    t.sorted = false;
+    upload_mats(c);
+}
+
 void load_table_file(sphgpu_ctx *c, int material, const char *path) {
     if (!path) throw Error(SPHGPU_ERR_ARG, "null path");
     FILE *fp = fopen(path, "r");
@@ -899,6 +923,13 @@ int32_t sphgpu_eos_load_table(sphgpu_ctx *c, int32_t material, const char *path)
     });
 }
 
+int32_t sphgpu_eos_set_tillotson(sphgpu_ctx *c, int32_t material, const double *params) {
+    return guard([&] {
+        check_ctx(c);
+        set_tillotson(c, material, params);
+    });
+}
+
 int32_t sphgpu_eos_set_table(sphgpu_ctx *c, int32_t material, int32_t nrho, int32_t nu, const double *cells) {
     return guard([&] {
         check_ctx(c);
@@ -912,7 +943,7 @@ void eos_io(sphgpu_ctx *c, int64_t n, const int32_t *material, const double *a, 
     if (n > 0 && (!material || !a || !b)) throw Error(SPHGPU_ERR_ARG, "null input array");
     for (int64_t i = 0; i < n; ++i) {
         const int m = material[i];
-        if (m < 0 || m >= SPHGPU_MAX_MATERIALS || c->mats[m].nr <= 0)
+        if (!mat_loaded(c, m))
             throw Error(SPHGPU_ERR_EOS, "material id " + std::to_string(m) + " has no EoS table");
     }
     SPH_HIP(hipMemcpyAsync(c->e_mat.ensure(n), material, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
@@ -949,7 +980,7 @@ int32_t sphgpu_eos_pressure_noalloc(sphgpu_ctx *c, int64_t n, const int32_t *mat
         if (!material || !dens || !eng || !pres || !snds) throw Error(SPHGPU_ERR_ARG, "null array");
         for (int64_t i = 0; i < n; ++i) {
             const int m = material[i];
-            if (m < 0 || m >= SPHGPU_MAX_MATERIALS || c->mats[m].nr <= 0)
+            if (!mat_loaded(c, m))
                 throw Error(SPHGPU_ERR_EOS, "material id " + std::to_string(m) + " has no EoS table");
         }
         if (!c->scratch) throw Error(SPHGPU_ERR_STATE, "context has no scratch address");
@@ -994,8 +1025,9 @@ int32_t sphgpu_eos_internal_energy(sphgpu_ctx *c, int64_t n, const int32_t *mate
         if (n > 0 && !eng) throw Error(SPHGPU_ERR_ARG, "null output array");
         eos_io(c, n, material, dens, ent);
         for (int64_t i = 0; i < n; ++i)
-            if (!c->mats[material[i]].sorted)
-                throw Error(SPHGPU_ERR_EOS, "InternalEnergy needs a table whose density column is non-decreasing");
+            if (c->mats[material[i]].kind != 0 || !c->mats[material[i]].sorted)
+                throw Error(SPHGPU_ERR_EOS, "InternalEnergy needs a table (not an analytic material) whose density column "
+                                             "is non-decreasing");
         launch_eos_internal_energy(n, c->e_mat.get(), c->e_in1.get(), c->e_in2.get(), c->d_mats.get(),
                                    SPHGPU_MAX_MATERIALS, abs_int ? 1 : 0, c->e_out1.get(), c->stream);
         SPH_HIP(hipMemcpyAsync(eng, c->e_out1.get(), sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
@@ -1035,7 +1067,7 @@ int32_t sphgpu_sim_init(sphgpu_ctx *c, const sphgpu_sim_params *prm, int64_t n, 
         if (nsys > 1 && !system) throw Error(SPHGPU_ERR_ARG, "nsys > 1 needs a system array");
         for (int64_t i = 0; i < n; ++i) {
             const int m = material[i];
-            if (m < 0 || m >= SPHGPU_MAX_MATERIALS || c->mats[m].nr <= 0)
+            if (!mat_loaded(c, m))
                 throw Error(SPHGPU_ERR_EOS, "material id " + std::to_string(m) + " has no EoS table");
         }
         std::unique_ptr<SimState> S(new SimState());

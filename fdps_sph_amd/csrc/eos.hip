@@ -48,18 +48,82 @@ __device__ __forceinline__ double bilinear(const MatDev &t, double v1, double v2
     return T[(int64_t) li * t.nu + ci];
 }
 
+__device__ __forceinline__ bool mat_ok(const MatDev &t) { return t.kind == 1 || t.nr > 0; }
+
+// Analytic Tillotson EoS: the compressed/cold branch P_co and the expanded
+// branch P_ex of src/EoS.h:91-103 (eta = rho/rho0, mu = eta - 1,
+// w = u/(u0 eta^2) + 1, z = 1/eta - 1), each with its exact partial
+// derivatives in rho and u.
+__device__ void till_co(const double *q, double rho, double u, double &P, double &dr, double &du) {
+    const double rho0 = q[0], u0 = q[1], A = q[4], B = q[5], a = q[6], b = q[7];
+    const double eta = rho / rho0, mu = eta - 1.0;
+    const double w = u / u0 / eta / eta + 1.0;
+    const double dw_dr = -2.0 * u / (u0 * eta * eta * rho), dw_du = 1.0 / (u0 * eta * eta);
+    const double k = a + b / w;
+    P = k * rho * u + A * mu + B * mu * mu;
+    dr = k * u - b * rho * u / (w * w) * dw_dr + A / rho0 + 2.0 * B * mu / rho0;
+    du = k * rho - b * rho * u / (w * w) * dw_du;
+}
+
+__device__ void till_ex(const double *q, double rho, double u, double &P, double &dr, double &du) {
+    const double rho0 = q[0], u0 = q[1], A = q[4], a = q[6], b = q[7], alpha = q[8], beta = q[9];
+    const double eta = rho / rho0, mu = eta - 1.0;
+    const double w = u / u0 / eta / eta + 1.0;
+    const double dw_dr = -2.0 * u / (u0 * eta * eta * rho), dw_du = 1.0 / (u0 * eta * eta);
+    const double z = 1.0 / eta - 1.0, dz_dr = -1.0 / (eta * rho);
+    const double ea = exp(-alpha * z), eb = exp(-beta * z * z);
+    const double F = b * rho * u / w + A * mu * ea;
+    const double dF_dr = b * u / w - b * rho * u / (w * w) * dw_dr + A / rho0 * ea - alpha * A * mu * ea * dz_dr;
+    const double dF_du = b * rho / w - b * rho * u / (w * w) * dw_du;
+    P = a * rho * u + F * eb;
+    dr = a * u + dF_dr * eb + F * eb * (-2.0 * beta * z * dz_dr);
+    du = a * rho + dF_du * eb;
+}
+
+// Tillotson's regimes: compressed (rho >= rho0) or cold (u <= u_iv) -> P_co;
+// expanded and hot (u >= u_cv) -> P_ex; linear blend in u in between.
+// c^2 = dP/drho|_u + P/rho^2 dP/du|_rho; c_s at least 1e-12 m/s, the floor the
+// ANEOS tables carry (at c_s = 0 the Balsara switch of a particle at rest
+// would be 0/0).
+__device__ void tillotson(const double *q, double rho, double u, double &P, double &cs) {
+    const double rho0 = q[0], uiv = q[2], ucv = q[3];
+    double dr, du;
+    if (rho >= rho0 || u <= uiv) {
+        till_co(q, rho, u, P, dr, du);
+    } else if (u >= ucv) {
+        till_ex(q, rho, u, P, dr, du);
+    } else {
+        double Pc, drc, duc, Pe, dre, due;
+        till_co(q, rho, u, Pc, drc, duc);
+        till_ex(q, rho, u, Pe, dre, due);
+        const double d = ucv - uiv;
+        P = ((u - uiv) * Pe + (ucv - u) * Pc) / d;
+        dr = ((u - uiv) * dre + (ucv - u) * drc) / d;
+        du = (Pe + (u - uiv) * due - Pc + (ucv - u) * duc) / d;
+    }
+    const double c2 = dr + P / (rho * rho) * du;
+    cs = sqrt(c2 > 1e-24 ? c2 : 1e-24);
+}
+
 __global__ void k_eos_pressure(int64_t n, const int32_t *mat, const double *rho, const double *u, const MatDev *mats,
                                int nmat, double *p, double *cs) {
     const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int m = mat[i];
-    if (m < 0 || m >= nmat || mats[m].nr <= 0) {
+    if (m < 0 || m >= nmat || !mat_ok(mats[m])) {
         p[i] = NAN;
         cs[i] = NAN;
         return;
     }
-    const MatDev t = mats[m];
     const double r = rho[i], e = u[i];
+    if (mats[m].kind == 1) {
+        double P, c;
+        tillotson(mats[m].till, r, e, P, c);
+        p[i] = P;
+        cs[i] = c;
+        return;
+    }
+    const MatDev &t = mats[m];
     p[i] = bilinear(t, r, e, 3);
     cs[i] = bilinear(t, r, e, 4);
 }
@@ -69,8 +133,14 @@ __global__ void k_eos_property(int64_t n, const int32_t *mat, const double *rho,
     const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int m = mat[i];
-    if (m < 0 || m >= nmat || mats[m].nr <= 0) {
+    if (m < 0 || m >= nmat || !mat_ok(mats[m])) {
         out[i] = NAN;
+        return;
+    }
+    if (mats[m].kind == 1) {  // an analytic material has P and c_s only
+        double P, c;
+        tillotson(mats[m].till, rho[i], u[i], P, c);
+        out[i] = prop == 3 ? P : (prop == 4 ? c : NAN);
         return;
     }
     out[i] = bilinear(mats[m], rho[i], u[i], prop);
@@ -137,7 +207,7 @@ __global__ void k_eos_internal_energy(int64_t n, const int32_t *mat, const doubl
     const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int m = mat[i];
-    if (m < 0 || m >= nmat || mats[m].nr <= 0 || !mats[m].sorted_rho) {
+    if (m < 0 || m >= nmat || mats[m].kind != 0 || mats[m].nr <= 0 || !mats[m].sorted_rho) {
         out[i] = NAN;
         return;
     }

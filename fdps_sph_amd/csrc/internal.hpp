@@ -194,6 +194,8 @@ void launch_grav_mixed(const GravArgs &a, hipStream_t s);
 
 // ------------------------------------------------------------------ EoS -- eos.hip
 struct MatDev {
+    int kind;             // 0: rho-u table, 1: analytic Tillotson (till)
+    double till[10];      // rho0, u0, u_iv, u_cv, A, B, a, b, alpha, beta (src/EoS.h:446 order)
     int nr, nu;
     const double *rho;    // nr
     const double *u;      // nu

@@ -46,6 +46,9 @@ SOURCE_PATHS = {
     "iron": os.path.join(TABLE_DIR, "iron___.table.txt"),
     "dunite": os.path.join(TABLE_DIR, "dunite_.table.txt"),
 }
+# Tillotson parameters of granite (rho0, u0, u_iv, u_cv, A, B, a, b, alpha,
+# beta), src/EoS.h:446
+GRANITE_TILLOTSON = (2680.0, 16.0e6, 3.5e6, 18.0e6, 18.0e9, 18.0e9, 0.5, 1.3, 5.0, 5.0)
 # tables whose shipped rho-u form was written before the P >= 0 clip
 UNCLIPPED = ("granite", "iron")
 

@@ -62,6 +62,13 @@ class Context:
         assert cells.ndim == 3 and cells.shape[2] == 6
         check(self._L.sphgpu_eos_set_table(self._h, int(material), cells.shape[0], cells.shape[1], ptr(cells)))
 
+    def set_tillotson(self, material: int, params):
+        """Analytic Tillotson material; params = (rho0, u0, u_iv, u_cv, A, B, a,
+        b, alpha, beta) as src/EoS.h:446 (see eos_tables.GRANITE_TILLOTSON)."""
+        prm = np.ascontiguousarray(params, dtype=np.float64)
+        assert prm.shape == (10,)
+        check(self._L.sphgpu_eos_set_tillotson(self._h, int(material), ptr(prm)))
+
     def pressure(self, material, dens, eng):
         mat = np.ascontiguousarray(material, dtype=np.int32)
         d = np.ascontiguousarray(dens, dtype=np.float64)

@@ -123,6 +123,17 @@ int32_t sphgpu_grav_retrieve(sphgpu_ctx *ctx, int32_t tag, int32_t n_walk, const
 int32_t sphgpu_eos_load_table(sphgpu_ctx *ctx, int32_t material, const char *path);
 /* Sets a table from cells[nrho][nu][6] = (rho, u, T, P, c_s, S). */
 int32_t sphgpu_eos_set_table(sphgpu_ctx *ctx, int32_t material, int32_t nrho, int32_t nu, const double *cells);
+/* Analytic Tillotson material (src/EoS.h:91-113): the compressed/cold
+ * branch P_co and the expanded branch P_ex of the reference's Tillotson
+ * class, which the reference defines but never calls, in Tillotson's regimes
+ * (P_co for rho >= rho0 or u <= u_iv, P_ex for u >= u_cv, linear in u between),
+ * c_s^2 = dP/drho + P/rho^2 dP/du from their exact derivatives, c_s at least
+ * 1e-12 m/s (the ANEOS tables' floor).  params =
+ * {rho0, u0, u_iv, u_cv, A, B, a, b, alpha, beta}, the constructor order of
+ * src/EoS.h:446 (granite: 2680, 16e6, 3.5e6, 18e6, 18e9, 18e9, 0.5, 1.3, 5, 5).
+ * Such a material gives P and c_s (properties 3, 4) and NaN for T and S;
+ * sphgpu_eos_internal_energy refuses it. */
+int32_t sphgpu_eos_set_tillotson(sphgpu_ctx *ctx, int32_t material, const double params[10]);
 /* CalcPressure (src/force.h:26-32): pres/snds of n particles (host arrays). */
 int32_t sphgpu_eos_pressure(sphgpu_ctx *ctx, int64_t n, const int32_t *material, const double *dens,
                             const double *eng, double *pres, double *snds);

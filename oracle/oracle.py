@@ -96,6 +96,9 @@ def lib(kind: str = "strict") -> C.CDLL:
     L.orc_table_from_cells.restype = P
     L.orc_table_from_cells.argtypes = [i, i, P]
     L.orc_table_free.argtypes = [P]
+    L.orc_table_tillotson.restype = P
+    L.orc_table_tillotson.argtypes = [P]
+    L.orc_tillotson.argtypes = [P, d, d, P, P]
     L.orc_table_dims.argtypes = [P, C.POINTER(i), C.POINTER(i)]
     L.orc_table_cells.restype = C.POINTER(d)
     L.orc_table_cells.argtypes = [P]
@@ -233,9 +236,13 @@ def balsara(div_v, rot_v, snds, smth, kind="strict") -> float:
 class Table:
     """An EoS table in the reference's rho-u format (src/EoS.h:290-385)."""
 
-    def __init__(self, path=None, cells=None, kind="strict"):
+    def __init__(self, path=None, cells=None, kind="strict", tillotson=None):
         self._L = lib(kind)
-        if path is not None:
+        if tillotson is not None:  # analytic Tillotson parameters (10)
+            self._prm_keep = np.ascontiguousarray(tillotson, dtype=np.float64)
+            assert self._prm_keep.shape == (10,)
+            self._t = self._L.orc_table_tillotson(_p(self._prm_keep))
+        elif path is not None:
             self._t = self._L.orc_table_load(path.encode())
             if not self._t:
                 raise IOError(f"cannot read EoS table {path}")

@@ -298,6 +298,8 @@ double orc_balsara(double div_v, const double rot_v[3], double snds, double smth
 /* EoS tables                                                               */
 /* ------------------------------------------------------------------------ */
 struct orc_table {
+    int kind;         /* 0: rho-u table, 1: analytic Tillotson (till[10]) */
+    double till[10];
     int nr, nu;
     double *cells;    /* [nr][nu][6] */
     double *rho_axis; /* column 0, field 0 of each row */
@@ -399,9 +401,86 @@ static int64_t lower_bound_(const double *a, int64_t n, double v)
     return first;
 }
 
-/* BilinearInterpolation::interpolate (src/interpolation.h:33-102). */
+/* Analytic Tillotson EoS.  The branches restate Tillotson::P_co / P_ex of
+ * src/EoS.h:91-103 (eta = rho/rho0, mu = eta - 1, w = u/(u0 eta^2) + 1):
+ *   P_co = (a + b/w) rho u + A mu + B mu^2
+ *   P_ex = a rho u + (b rho u / w + A mu exp(-alpha z)) exp(-beta z^2),  z = 1/eta - 1.
+ * The reference defines them but never calls them (its Tillotson::Pressure is
+ * the table lookup).  The regime selection is Tillotson's (compressed or cold:
+ * P_co; expanded and hot beyond u_cv: P_ex; linear blend in u between u_iv and
+ * u_cv), and c^2 = dP/drho|_u + P/rho^2 dP/du|_rho from the exact derivatives
+ * of those expressions, with c_s at least 1e-12 m/s: the floor the ANEOS
+ * tables carry, which keeps the Balsara switch |div v| / (|div v| + |rot v| +
+ * 1e-4 c_s/h) defined for a particle at rest in tension.  prm = (rho0, u0, u_iv, u_cv, A, B, a, b, alpha,
+ * beta), the order of the Tillotson constructor (src/EoS.h:446). */
+static void till_co_(const double *q, double rho, double u, double *P, double *dr, double *du)
+{
+    const double rho0 = q[0], u0 = q[1], A = q[4], B = q[5], a = q[6], b = q[7];
+    const double eta = rho / rho0, mu = eta - 1.0;
+    const double w = u / u0 / eta / eta + 1.0;
+    const double dw_dr = -2.0 * u / (u0 * eta * eta * rho), dw_du = 1.0 / (u0 * eta * eta);
+    const double k = a + b / w;
+    *P = k * rho * u + A * mu + B * mu * mu;
+    *dr = k * u - b * rho * u / (w * w) * dw_dr + A / rho0 + 2.0 * B * mu / rho0;
+    *du = k * rho - b * rho * u / (w * w) * dw_du;
+}
+
+static void till_ex_(const double *q, double rho, double u, double *P, double *dr, double *du)
+{
+    const double rho0 = q[0], u0 = q[1], A = q[4], a = q[6], b = q[7], alpha = q[8], beta = q[9];
+    const double eta = rho / rho0, mu = eta - 1.0;
+    const double w = u / u0 / eta / eta + 1.0;
+    const double dw_dr = -2.0 * u / (u0 * eta * eta * rho), dw_du = 1.0 / (u0 * eta * eta);
+    const double z = 1.0 / eta - 1.0, dz_dr = -1.0 / (eta * rho);
+    const double ea = exp(-alpha * z), eb = exp(-beta * z * z);
+    const double F = b * rho * u / w + A * mu * ea;
+    const double dF_dr = b * u / w - b * rho * u / (w * w) * dw_dr + A / rho0 * ea - alpha * A * mu * ea * dz_dr;
+    const double dF_du = b * rho / w - b * rho * u / (w * w) * dw_du;
+    *P = a * rho * u + F * eb;
+    *dr = a * u + dF_dr * eb + F * eb * (-2.0 * beta * z * dz_dr);
+    *du = a * rho + dF_du * eb;
+}
+
+void orc_tillotson(const double prm[10], double rho, double u, double *pres, double *snds)
+{
+    const double rho0 = prm[0], uiv = prm[2], ucv = prm[3];
+    double P, dr, du;
+    if (rho >= rho0 || u <= uiv) {
+        till_co_(prm, rho, u, &P, &dr, &du);
+    } else if (u >= ucv) {
+        till_ex_(prm, rho, u, &P, &dr, &du);
+    } else {
+        double Pc, drc, duc, Pe, dre, due;
+        till_co_(prm, rho, u, &Pc, &drc, &duc);
+        till_ex_(prm, rho, u, &Pe, &dre, &due);
+        const double d = ucv - uiv;
+        P = ((u - uiv) * Pe + (ucv - u) * Pc) / d;
+        dr = ((u - uiv) * dre + (ucv - u) * drc) / d;
+        du = (Pe + (u - uiv) * due - Pc + (ucv - u) * duc) / d;
+    }
+    const double c2 = dr + P / (rho * rho) * du;
+    *pres = P;
+    *snds = sqrt(c2 > 1e-24 ? c2 : 1e-24);
+}
+
+orc_table *orc_table_tillotson(const double prm[10])
+{
+    if (!prm) return NULL;
+    orc_table *t = (orc_table *) calloc(1, sizeof(orc_table));
+    t->kind = 1;
+    memcpy(t->till, prm, sizeof(double) * 10);
+    return t;
+}
+
+/* BilinearInterpolation::interpolate (src/interpolation.h:33-102); for an
+ * analytic Tillotson material P (prop 3) and c_s (prop 4), NaN otherwise. */
 double orc_eos_bilinear(const orc_table *t, double v1, double v2, int prop)
 {
+    if (t->kind == 1) {
+        double P, c;
+        orc_tillotson(t->till, v1, v2, &P, &c);
+        return prop == 3 ? P : (prop == 4 ? c : NAN);
+    }
     int64_t li = lower_bound_(t->rho_axis, t->nr, v1);
     int64_t ci = lower_bound_(t->u_axis, t->nu, v2);
     if (li > t->nr - 1) li = t->nr - 1;
@@ -464,6 +543,7 @@ static double lin_(double x1, double x2, double x, double q1, double q2)
 
 double orc_eos_internal_energy(const orc_table *t, double val1, double val2, int abs_int)
 {
+    if (t->kind != 0) return NAN;
     const uint32_t gl = 120;
     const double *v1 = t->full_rho, *v2 = t->full_S, *v3 = t->full_u;
     const int64_t N = t->nfull;

@@ -67,6 +67,11 @@ double orc_balsara(double div_v, const double rot_v[3], double snds, double smth
 typedef struct orc_table orc_table;
 orc_table *orc_table_load(const char *path);                          /* NULL on failure */
 orc_table *orc_table_from_cells(int nrho, int nu, const double *cells); /* cells[nrho][nu][6] */
+/* Analytic Tillotson material, prm = (rho0, u0, u_iv, u_cv, A, B, a, b,
+ * alpha, beta) (src/EoS.h:91-113, 446); evaluated by orc_eos_bilinear for
+ * props 3 (P) and 4 (c_s), NaN for the others; no cells (dims 0 x 0). */
+orc_table *orc_table_tillotson(const double prm[10]);
+void   orc_tillotson(const double prm[10], double rho, double u, double *pres, double *snds);
 void   orc_table_free(orc_table *t);
 int    orc_table_dims(const orc_table *t, int *nrho, int *nu);
 const double *orc_table_cells(const orc_table *t);                    /* [nrho][nu][6] */

@@ -133,6 +133,53 @@ def test_eos_internal_energy(ctx64, oracle):
     assert abs(one[0] - 1654186.594026882) <= 1e-15 * 1654186.594026882
 
 
+@pytest.fixture(scope="module")
+def ctx_till():
+    from fdps_sph_amd import Context
+    c = Context(0, "fp64")
+    c.set_tillotson(0, E.GRANITE_TILLOTSON)
+    c.load_table(1, E.PATHS["iron"])
+    yield c
+    c.close()
+
+
+def test_eos_tillotson(ctx_till, oracle):
+    """Analytic Tillotson granite (src/EoS.h:91-113, 446) in every regime:
+    P within 1e-12 of the size of its terms, c_s within 1e-12."""
+    _print("[eos tillotson]")
+    rho0, u0, uiv, ucv, A, B, a, b, alpha, beta = E.GRANITE_TILLOTSON
+    t = oracle.Table(tillotson=E.GRANITE_TILLOTSON)
+    rng = np.random.default_rng(17)
+    rho = rho0 * np.exp(rng.uniform(np.log(0.05), np.log(3.0), 4000))
+    u = np.exp(rng.uniform(np.log(1e3), np.log(1e8), 4000))
+    rho = np.concatenate([rho, [rho0, rho0, 0.8 * rho0, 0.8 * rho0]])
+    u = np.concatenate([u, [1e6, 0.0, uiv, ucv]])
+    m = np.zeros(len(rho), np.int32)
+    p, cs = ctx_till.pressure(m, rho, u)
+    ref = np.array([t.pressure(r, e) for r, e in zip(rho, u)])
+    ref_c = np.array([t.sound_speed(r, e) for r, e in zip(rho, u)])
+    eta = rho / rho0
+    mu = eta - 1
+    scale = a * rho * u + b * rho * u / (u / u0 / eta ** 2 + 1) + A * np.abs(mu) + B * mu * mu
+    assert H.rel_err(p, ref, scale) <= TOL
+    assert H.rel_err(cs, ref_c, 1e-12 * ref_c.max()) <= TOL
+    assert np.all(np.isnan(ctx_till.property(m, rho, u, 2))) and np.all(np.isnan(ctx_till.property(m, rho, u, 5)))
+    assert np.array_equal(ctx_till.property(m, rho, u, 3), p)
+    from fdps_sph_amd import SphGpuError
+    with pytest.raises(SphGpuError):
+        ctx_till.internal_energy(m[:3], rho[:3], u[:3])
+
+
+def test_full_step_tillotson_granite(ctx_till, oracle):
+    """A granite impact (4,500 + 500) on the analytic Tillotson material:
+    init + 2 steps, each from the GPU's state."""
+    _print("[full step, analytic Tillotson granite 5k]")
+    ic = IC.granite_impact(4500, 500, seed=1007)
+    tabs = [oracle.Table(tillotson=E.GRANITE_TILLOTSON), oracle.Table(E.PATHS["iron"])]
+    prm = oracle.params()
+    run_per_step(Simulation(ctx_till, ic), oracle, ic, tabs, prm, 2, fp64_check(tabs, prm))
+
+
 def test_eos_unknown_material(ctx64):
     from fdps_sph_amd import SphGpuError
     with pytest.raises(SphGpuError) as e:

@@ -29,7 +29,10 @@ of the oracle's code, and is chosen so that a plausible slip fails it:
   predicted energy that the step's EoS pass reads;
 * InternalEnergy (src/interpolation.h:107-335): exact inverse on a table whose
   entropy is linear in (rho, u), and the piecewise-linear inverse inside the
-  right bracket on a curved table.
+  right bracket on a curved table;
+* analytic Tillotson (src/EoS.h:91-113): its limits at rho0, under cold
+  compression and for a hot dilute gas, continuity of the regime blend, and
+  c^2 against central differences of P.
 """
 import math
 
@@ -410,3 +413,52 @@ def test_internal_energy_brackets_on_curved_table(oracle):
         for abs_int in (True, False):
             got = t.internal_energy(rho, S, abs_int)
             assert abs(got - expect) <= 1e-9 * expect, (rho, S, got, expect, abs_int)
+
+
+# ------------------------------------------------------------------ analytic Tillotson
+GRANITE = E.GRANITE_TILLOTSON  # rho0, u0, u_iv, u_cv, A, B, a, b, alpha, beta (src/EoS.h:446)
+
+
+def test_tillotson_closed_forms(oracle):
+    """Limits the Tillotson form fixes: at rho0 P = (a + b/(u/u0 + 1)) rho0 u;
+    cold compression (u = 0) P = A mu + B mu^2; a hot dilute gas P = a rho u
+    (both exponentials vanish); the blend between u_iv and u_cv meets each
+    branch at its end."""
+    rho0, u0, uiv, ucv, A, B, a, b, alpha, beta = GRANITE
+    t = oracle.Table(tillotson=GRANITE)
+    for u in (1e3, 1e6, 3e7):
+        expect = (a + b / (u / u0 + 1.0)) * rho0 * u
+        assert abs(t.pressure(rho0, u) - expect) <= 1e-15 * expect
+    for eta in (1.05, 1.3, 2.0):
+        mu = eta - 1.0
+        assert abs(t.pressure(eta * rho0, 0.0) - (A * mu + B * mu * mu)) <= 1e-15 * (A * mu + B * mu * mu)
+        # compressed states keep that cold part at any u, hot ones included:
+        # the thermal part is (a + b / (u / (u0 eta^2) + 1)) rho u
+        for u in (1e6, 3e7, 1e8):
+            th = (a + b / (u / u0 / eta ** 2 + 1.0)) * eta * rho0 * u
+            got = t.pressure(eta * rho0, u) - (A * mu + B * mu * mu)
+            assert abs(got - th) <= 1e-14 * (th + A * mu), (eta, u, got, th)
+    rho, u = 1e-3 * rho0, 2.0 * ucv
+    assert t.pressure(rho, u) == a * rho * u
+    # in expansion, P_co below u_iv and P_ex above u_cv, continuous across both
+    rho = 0.8 * rho0
+    for edge in (uiv, ucv):
+        lo, hi = t.pressure(rho, edge * (1 - 1e-12)), t.pressure(rho, edge * (1 + 1e-12))
+        assert abs(hi - lo) <= 1e-9 * abs(lo), (edge, lo, hi)
+
+
+@pytest.mark.parametrize("rho_f,u", [(1.3, 1e6), (1.0, 5e6), (0.8, 1e6), (0.8, 1e7), (0.8, 4e7), (0.5, 2.5e7),
+                                     (0.2, 6e7)])
+def test_tillotson_sound_speed_is_thermodynamic(oracle, rho_f, u):
+    """c^2 = dP/drho|_u + P/rho^2 dP/du|_rho against central differences of
+    the pressure itself, in every regime."""
+    rho0 = GRANITE[0]
+    t = oracle.Table(tillotson=GRANITE)
+    rho = rho_f * rho0
+    P = t.pressure(rho, u)
+    dr = (t.pressure(rho * (1 + 1e-6), u) - t.pressure(rho * (1 - 1e-6), u)) / (2e-6 * rho)
+    du = (t.pressure(rho, u * (1 + 1e-6)) - t.pressure(rho, u * (1 - 1e-6))) / (2e-6 * u)
+    c2 = dr + P / rho ** 2 * du
+    assert c2 > 0
+    assert abs(t.sound_speed(rho, u) ** 2 - c2) <= 1e-6 * c2, (t.sound_speed(rho, u) ** 2, c2)
+    assert math.isnan(t.temperature(rho, u)) and math.isnan(t.internal_energy(rho, 100.0))
