@@ -69,7 +69,10 @@ def kernel_symbol(kernel, mixed):
     sym, always64 = KERNEL_SYMBOL[kernel]
     return f"{sym}_mixed" if mixed and not always64 else sym
 STAGES = ["kick_drift", "sort_tree", "density_x4", "eos", "derivative", "gravity", "hydro", "final_dt"]
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+# rocprofv3 counter summary of the interaction kernels (per launch): HBM
+# traffic from FETCH_SIZE / WRITE_SIZE (copy-calibrated), VALU issue rate,
+# counted FP32 / FP64 VALU flops; the commands that made it are in the file
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "counters_1M_iron_dunite.json")
 
 DEFAULTS = {  # workload: (warmup, steps)
     "impact1m": (10, 90), "granite10k": (5, 45), "theta-sweep": (10, 90), "batch64": (5, 20),
@@ -176,16 +179,19 @@ def kernel_roofline(stats, mixed):
     return out
 
 
-def measured_traffic(kernel, precision, workload):
-    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+def measured_counters(kernel, precision, workload):
+    """Counter figures per launch of a kernel from the committed rocprofv3
+    --pmc summary of the same workload: HBM bytes (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration), VALU instructions per CU per cycle
+    and counted VALU flops.  None where the summary has no such entry."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(COUNTERS_FILE) as f:
             d = json.load(f)
     except (OSError, ValueError):
         return None
-    e = d.get("workloads", {}).get(workload, {}).get(kernel_symbol(kernel, precision == "mixed"))
-    return None if e is None else e.get("hbm_bytes_per_launch")
+    if d.get("workload") != workload:
+        return None
+    return d.get("kernels", {}).get(kernel_symbol(kernel, precision == "mixed"))
 
 
 # ------------------------------------------------------------------ runs
@@ -275,6 +281,43 @@ def force_errors(m, f):
             "div_v": err_stats(m["DIVV"], f["DIVV"], ids_m, ids_f)}
 
 
+def du_term_scale(sim, ngroups=200, seed=7):
+    """Size of the summed terms of du/dt, sum_j m_j (|A_i| + |AV_ij|/2) |dv| |gradW_bar|
+    (src/force.h:126-145), for the particles of `ngroups` sampled i-groups of a
+    simulation's current state over its exported symmetric lists: the scale
+    on which a du/dt whose terms cancel is compared.  Returns (ids, scale)."""
+    st = sim.state()
+    L = sim.lists(1)
+    gs, joff, jidx = L["group_start"], L["j_off"], L["j_idx"]
+    G = len(gs) - 1
+    groups = np.random.default_rng(seed).choice(G, min(ngroups, G), replace=False)
+    x, v, h, rho, p, c = st["pos"], st["vel"], st["smth"], st["dens"], st["pres"], st["snds"]
+    f, bal, m = st["gradh"], st["bal"], st["mass"]
+    A = f * p / rho ** 2
+    norm3 = 1365.0 / (64.0 * np.pi)
+    ids, out = [], []
+
+    def dwdr(r, hh):  # |dW/dr| of the Wendland C6 kernel, support 2.5 h
+        H = 2.5 * hh
+        q = np.minimum(r / H, 1.0)
+        return norm3 / H ** 4 * 22.0 * q * (1 + 7 * q + 16 * q * q) * (1 - q) ** 7
+    for g in groups:
+        ii = np.arange(gs[g], gs[g + 1])
+        jj = jidx[joff[g]:joff[g + 1]]
+        dr = x[ii][:, None, :] - x[jj][None]
+        dv = v[ii][:, None, :] - v[jj][None]
+        r = np.linalg.norm(dr, axis=2)
+        dvdr = np.sum(dv * dr, axis=2)
+        w = np.where(dvdr < 0, dvdr / np.maximum(r, 1e-300), 0.0)
+        vsig = c[ii][:, None] + c[jj][None] - 3.0 * w
+        av = np.abs(0.5 * vsig * w / (0.5 * (rho[ii][:, None] + rho[jj][None])) * 0.5 * (bal[ii][:, None] + bal[jj][None]))
+        gbar = 0.5 * (f[ii][:, None] * dwdr(r, h[ii][:, None]) + f[jj][None] * dwdr(r, h[jj][None]))
+        sc = np.sum(m[jj][None] * (np.abs(A[ii])[:, None] + 0.5 * av) * np.linalg.norm(dv, axis=2) * gbar, axis=1)
+        ids.append(st["id"][ii])
+        out.append(sc)
+    return np.concatenate(ids), np.concatenate(out)
+
+
 def compare_precisions(ctx64, ctx32, ic, warmup, steps, **params):
     """FP64 and mixed runs of the same workload plus the mixed form's errors."""
     F = run(ctx64, ic, warmup, steps, "fp64", **params)
@@ -283,8 +326,19 @@ def compare_precisions(ctx64, ctx32, ic, warmup, steps, **params):
     # one force evaluation of each precision from the FP64 state after the last step
     state = F["state"]
     from fdps_sph_amd import Simulation
-    fe = force_fields(Simulation(ctx64, state, **params))
-    me = force_errors(force_fields(Simulation(ctx32, state, **params)), fe)
+    sf = Simulation(ctx64, state, **params)
+    fe = force_fields(sf)
+    sids, sscale = du_term_scale(sf)
+    mf = force_fields(Simulation(ctx32, state, **params))
+    me = force_errors(mf, fe)
+    # du/dt of the sampled groups' particles on the size of its summed terms
+    pf = {k: v for k, v in zip(fe["ID"], fe["ENGDOT"])}
+    pm = {k: v for k, v in zip(mf["ID"], mf["ENGDOT"])}
+    num = np.array([abs(pm[k] - pf[k]) for k in sids])
+    den = np.maximum(np.abs(np.array([pf[k] for k in sids])), sscale)
+    e = np.divide(num, den, out=np.zeros_like(num), where=num > 0)
+    me["du_dt"]["on_term_scale"] = {"particles": int(len(sids)), "max": float(e.max()),
+                                    "p99": float(np.percentile(e, 99)), "rms": float(np.sqrt(np.mean(e * e)))}
     drift = abs(M["E1"] - F["E1"]) / abs(F["E1"])
     log(f"[mixed vs fp64] speedup {M['steps_per_s'] / F['steps_per_s']:.2f}x; acc err at the evolved state "
         f"max {me['acc']['max']:.2e} p99 {me['acc']['p99']:.2e} norm {me['acc']['norm']:.2e}; energy drift over "
@@ -299,7 +353,15 @@ def stage_dict(stages):
 def precision_block(R, mixed, workload):
     roof = kernel_roofline(R["stats"], mixed)
     for k, v in roof.items():
-        v["traffic"] = measured_traffic(k, "mixed" if mixed else "fp64", workload)
+        c = measured_counters(k, "mixed" if mixed else "fp64", workload)
+        v["traffic"] = None if c is None else c.get("hbm_bytes_per_launch")
+        # the other roof: fraction of the CU's one-wave64-VALU-instruction-per-cycle issue rate
+        v["valu_issue_frac"] = None if c is None else c.get("valu_issue")
+        if k == "grav" and c is not None and c.get("valu_flops_per_launch"):
+            # flops the VALU counters count per launch (FP64 or FP32), over this run's kernel time
+            cf = c["valu_flops_per_launch"] / (v["avg_ms"] * 1e-3) / 1e12
+            v["counter_tflops"] = cf
+            v["counter_frac"] = cf / v["peak"]
     return {"steps_per_s": R["steps_per_s"], "ms_per_step": R["ms_per_step"], "median_ms_per_step_stats_on": R["median_ms"],
             "gather_builds_per_force_eval": R["gather_builds_per_force_eval"],
             "kernels": {k: {kk: vv for kk, vv in v.items() if kk != "per_launch"} for k, v in roof.items()},
@@ -307,9 +369,42 @@ def precision_block(R, mixed, workload):
             "stages_ms": stage_dict(R["stages_timed"]), "stages_ms_stats_on": stage_dict(R["stages"])}, roof
 
 
+def targets_block(mb, R0, theta, workload):
+    """Pass/fail of the precision form's targets on this run: >= 2x the FP64
+    steps/s, <= 1e-5 relative acceleration error against FP64, <= 1e-6
+    relative total-energy drift against FP64 over 100 steps."""
+    a = mb["err_vs_fp64"]["after_last_step"]["acc"]
+    du = mb["err_vs_fp64"]["after_last_step"]["du_dt"]
+    sp = mb["speedup_vs_fp64"]
+    nsteps = R0["fp64"]["steps_total"]
+    drift = mb["energy_drift_vs_fp64"]
+    return {
+        "workload": workload, "theta": theta,
+        "speedup_ge_2": {"value": sp, "pass": sp >= 2.0},
+        "acc_rel_err_le_1e-5": {"max": a["max"], "p99": a["p99"], "norm": a["norm"],
+                                "max_on_force_scale": a.get("max_on_force_scale"),
+                                "pass_max": a["max"] <= 1e-5, "pass_p99": a["p99"] <= 1e-5,
+                                "pass_norm": a["norm"] <= 1e-5},
+        "du_dt_rel_err_on_term_scale": du.get("on_term_scale"),
+        "energy_drift_le_1e-6": {"value": drift, "steps": nsteps, "pass": drift <= 1e-6 and nsteps >= 100},
+    }
+
+
 # ------------------------------------------------------------------ CPU baseline
-def cpu_baseline(ic, tables, theta, steps, nsys=1):
-    """The oracle on the same workload: initial force evaluation + `steps` steps."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(ic, tables, theta, steps, nsys=1, warm=0):
+    """The oracle on the same workload: the initial force evaluation and `warm`
+    untimed steps, then `steps` timed steps."""
     from fdps_sph_amd import eos_tables as E
     from oracle import oracle as O
     n = len(ic["mass"])
@@ -326,16 +421,20 @@ def cpu_baseline(ic, tables, theta, steps, nsys=1):
     sim = O.Sim(s, tabs, O.params(theta=theta, nsys=nsys), kind="fast")
     t0 = time.perf_counter()
     sim.init()
+    for _ in range(warm):
+        sim.step()
     t_init = time.perf_counter() - t0
     t0 = time.perf_counter()
     for _ in range(steps):
         sim.step()
     t_step = (time.perf_counter() - t0) / max(steps, 1)
-    log(f"[cpu oracle] {O.num_threads()} threads: init {t_init:.2f} s, {steps} step(s) {t_step:.3f} s/step")
-    return {"value": 1.0 / t_step, "unit": "steps/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"{steps} full step(s) of the same {n:,}-particle workload after the initial force evaluation "
-                      f"({t_step:.3f} s/step); CPU oracle = the reference's functors restated in C, -O3 -ffast-math "
-                      "-funroll-loops -fopenmp, OpenMP over i-groups"}
+    model = cpu_model()
+    log(f"[cpu oracle] {O.num_threads()} threads on {model}: init + {warm} warm-up step(s) {t_init:.2f} s, "
+        f"{steps} timed step(s) {t_step:.3f} s/step")
+    return {"value": 1.0 / t_step, "unit": "steps/s", "cores": O.num_threads(), "kind": "port", "cpu_model": model,
+            "sample": f"{steps} full step(s) of the same {n:,}-particle workload, timed after the initial force "
+                      f"evaluation and {warm} warm-up step(s) ({t_step:.3f} s/step); CPU oracle = the reference's "
+                      "functors restated in C, -O3 -ffast-math -funroll-loops -fopenmp, OpenMP over i-groups"}
 
 
 def cpu_force_eval(ic, tables, theta):
@@ -358,7 +457,7 @@ def cpu_force_eval(ic, tables, theta):
     sim.init()
     t = time.perf_counter() - t0
     log(f"[cpu oracle] {O.num_threads()} threads: one force evaluation {t:.2f} s")
-    return {"value": 1.0 / t, "unit": "steps/s", "cores": O.num_threads(), "kind": "port",
+    return {"value": 1.0 / t, "unit": "steps/s", "cores": O.num_threads(), "kind": "port", "cpu_model": cpu_model(),
             "sample": f"one force evaluation (the initial one: 4 density passes, EoS, derivative, hydro, gravity) "
                       f"of the same {n:,}-particle workload ({t:.2f} s), timed as one step"}
 
@@ -455,6 +554,9 @@ def main():
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--persistent", type=int, default=0, metavar="MASK",
+                    help="interaction kernels run as one persistent launch: bitmask 1 density, 2 derivative, "
+                         "4 hydro, 8 gravity (batch64 picks it by autotuning)")
     ap.add_argument("--eos-density-fp32", action="store_true",
                     help="precision form: run the last density pass (the EoS input) in FP32 too")
     ap.add_argument("--sensitivity", type=float, default=0.0, metavar="EPS",
@@ -498,16 +600,16 @@ def main():
         ctx[p] = Context(0, p)
         load_tables(ctx[p], tables)
 
-    params = {"theta": args.theta, "nsys": nsys, "eos_density_fp64": 0 if args.eos_density_fp32 else 1}
+    params = {"theta": args.theta, "nsys": nsys, "eos_density_fp64": 0 if args.eos_density_fp32 else 1,
+              "persistent": args.persistent}
     extra = {}
     if args.workload == "batch64":
         from fdps_sph_amd import autotune
         tctx = ctx.get("mixed", ctx.get("fp64"))
         tr = autotune.tune(tctx, ic, warmup=1, steps=3, log=log, **params)
-        params.update(n_group=tr.n_group, n_leaf=tr.n_leaf)
-        extra["autotune"] = {"precision": "mixed" if "mixed" in ctx else "fp64", "best": {
-            "n_group": tr.n_group, "n_leaf": tr.n_leaf, "ms_per_step": tr.ms_per_step},
-            "table": [{"n_group": a, "n_leaf": b, "ms_per_step": c} for a, b, c in tr.table]}
+        params.update(tr.params())
+        extra["autotune"] = {"precision": "mixed" if "mixed" in ctx else "fp64",
+                             "best": dict(tr.params(), ms_per_step=tr.ms_per_step), "table": tr.table}
 
     sweep = [0.3, 0.5, 0.7] if args.workload == "theta-sweep" else [args.theta]
     results = {}
@@ -529,7 +631,10 @@ def main():
     rl = roof[dominant]
     roofline = {"bound": rl["bound"], "achieved": rl["achieved"], "peak": rl["peak"], "unit": rl["unit"],
                 "frac": rl["frac"], "traffic": rl["traffic"], "kernel": kernel_symbol(dominant, main_prec == "mixed"),
-                "avg_ms": rl["avg_ms"], "per_launch": rl["per_launch"]}
+                "avg_ms": rl["avg_ms"], "per_launch": rl["per_launch"], "valu_issue_frac": rl["valu_issue_frac"]}
+    for k in ("counter_tflops", "counter_frac"):
+        if k in rl:
+            roofline[k] = rl[k]
     if rl["bound"] == "valu":  # gravity: FMAs and a reciprocal square root per pair on the vector ALU, no MFMA
         roofline["note"] = ("vector-ALU bound: peak = MI355X's published " + ("FP32" if main_prec == "mixed" else "FP64")
                             + " vector rate; achieved = 38 flops per interaction x interactions / kernel time")
@@ -548,7 +653,8 @@ def main():
         "data": "synthetic (seeded lattice bodies)",
         "config": {"workload": f"{args.workload}: {desc}", "particles": n, "systems": nsys, "theta": th0,
                    "precision": main_prec, "n_group": params.get("n_group", 64),
-                   "n_leaf": params.get("n_leaf", 16),
+                   "n_leaf": params.get("n_leaf", 16), "persistent": params.get("persistent", 0),
+                   "list_front_lds": params.get("list_front_lds", 512),
                    "eos_density": "fp32" if args.eos_density_fp32 else "fp64"},
         "roofline": roofline,
         main_prec: block,
@@ -578,6 +684,7 @@ def main():
         out["mixed"] = mb
         out["mixed_steps_per_s"] = mb["steps_per_s"]
         out["speedup_mixed_vs_fp64"] = mb["speedup_vs_fp64"]
+        out["targets"] = targets_block(mb, R0, th0, args.workload)
     if len(sweep) > 1:
         out["theta_sweep"] = []
         for th in sweep:
@@ -586,6 +693,7 @@ def main():
                  "mixed_steps_per_s": r["mixed"]["steps_per_s"],
                  "speedup": r["mixed"]["steps_per_s"] / r["fp64"]["steps_per_s"],
                  "energy_drift_vs_fp64": r["drift"], "steps": r["fp64"]["steps_total"],
+                 "energy_drift_le_1e-6": r["drift"] <= 1e-6,
                  "err_vs_fp64": r["err"]}
             out["theta_sweep"].append(e)
     if args.sensitivity > 0 and "fp64" in R0:
@@ -617,8 +725,8 @@ def main():
             out["cpu_baseline"] = cpu_baseline(ic, tables, th0, warmup + steps, nsys)
         elif args.workload == "impact16m":
             out["cpu_baseline"] = cpu_force_eval(ic, tables, th0)
-        else:
-            out["cpu_baseline"] = cpu_baseline(ic, tables, th0, 1, nsys)
+        else:  # 1M, batch: the initial force evaluation, one warm-up step, three timed steps
+            out["cpu_baseline"] = cpu_baseline(ic, tables, th0, 3, nsys, warm=1)
     print(json.dumps(out), flush=True)
     for c in ctx.values():
         c.close()

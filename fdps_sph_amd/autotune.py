@@ -1,16 +1,24 @@
-"""Pick the tree/group parameters of a device-resident simulation by timing.
+"""Pick the tree/group/launch parameters of a device-resident simulation by timing.
 
 FDPS exposes the leaf size (n_leaf, `src/main.cpp:86-87`) and the i-group
 size (n_group, `src/main.cpp:88-90`) as tuning knobs and leaves the choice to
-the user.  Here the same two knobs are timed on the workload itself: each
-candidate builds a Simulation from the same initial state, runs `warmup`
-steps untimed and `steps` timed steps, and the fastest mean step wins.
+the user.  Here they are timed on the workload itself, together with two
+knobs of this implementation:
+
+* persistent: which interaction kernels run as one persistent launch taking
+  the groups of every system from per-XCD counters (a bitmask: 1 density,
+  2 derivative, 4 hydro, 8 gravity), the others one workgroup per group;
+* list_front_lds: the entries of a tree walk's frontier kept in LDS before it
+  continues in global memory -- the LDS tile of the list construction.  The
+  interaction kernels' j-tile is one wave of j-particles (64 entries) by
+  construction: each lane stages one entry.
 
 On this device the i-group is one 64-lane wavefront (one lane per
-i-particle), so n_group ranges over divisors of the wave width (larger FDPS
-groups only amortise host-side list construction, which is not on this path);
-the j-tile staged through LDS is one wave of j-particles per tile by
-construction.
+i-particle), so n_group ranges over sizes up to the wave width.  The search
+is staged: every (n_group, n_leaf) pair first, then the launch form and the
+LDS frontier at the best pair.  Each candidate builds a Simulation from the
+same initial state, runs `warmup` steps untimed and `steps` timed steps, and
+the fastest mean step wins.
 """
 from __future__ import annotations
 
@@ -21,6 +29,9 @@ from .sph import Context, Simulation
 
 N_GROUP_CHOICES = (32, 48, 64)
 N_LEAF_CHOICES = (8, 16, 32)
+# persistent-launch bitmask: none, gravity only, every interaction kernel
+PERSISTENT_CHOICES = (0, 8, 15)
+FRONT_LDS_CHOICES = (128, 512)
 
 
 @dataclass
@@ -28,28 +39,53 @@ class TuneResult:
     n_group: int
     n_leaf: int
     ms_per_step: float
-    table: list = field(default_factory=list)  # (n_group, n_leaf, ms_per_step) per candidate
+    persistent: int = 0
+    list_front_lds: int = 512
+    table: list = field(default_factory=list)  # dict per candidate: the four knobs and ms_per_step
+
+    def params(self) -> dict:
+        return {"n_group": self.n_group, "n_leaf": self.n_leaf, "persistent": self.persistent,
+                "list_front_lds": self.list_front_lds}
 
 
-def tune(ctx: Context, ic: dict, *, n_groups=N_GROUP_CHOICES, n_leaves=N_LEAF_CHOICES, warmup: int = 1,
-         steps: int = 3, log=None, **params) -> TuneResult:
-    """Time every (n_group, n_leaf) pair on `ic` and return the fastest.
+def _time(ctx, ic, warmup, steps, **params) -> float:
+    sim = Simulation(ctx, ic, **params)
+    if warmup:
+        sim.step(warmup)
+    t0 = time.perf_counter()
+    sim.step(steps)  # returns after the context's stream has drained
+    return 1e3 * (time.perf_counter() - t0) / steps
+
+
+def tune(ctx: Context, ic: dict, *, n_groups=N_GROUP_CHOICES, n_leaves=N_LEAF_CHOICES,
+         persistents=PERSISTENT_CHOICES, front_lds=FRONT_LDS_CHOICES, warmup: int = 1, steps: int = 3, log=None,
+         **params) -> TuneResult:
+    """Time the candidates on `ic` and return the fastest.
 
     `params` are passed to every Simulation (theta, nsys, ...).  Each candidate
     starts from the same initial state, so all of them time the same steps.
     """
     table = []
+    params = {k: v for k, v in params.items() if k not in ("n_group", "n_leaf", "persistent", "list_front_lds")}
+
+    def run(**knobs):
+        ms = _time(ctx, ic, warmup, steps, **knobs, **params)
+        row = dict(knobs, ms_per_step=ms)
+        table.append(row)
+        if log:
+            log("[autotune] " + " ".join(f"{k}={v}" for k, v in knobs.items()) + f": {ms:8.3f} ms/step")
+        return row
+
+    base = {"persistent": persistents[0], "list_front_lds": front_lds[-1]}
     for ng in n_groups:
         for nl in n_leaves:
-            sim = Simulation(ctx, ic, n_group=ng, n_leaf=nl, **params)
-            if warmup:
-                sim.step(warmup)
-            t0 = time.perf_counter()
-            sim.step(steps)  # returns after the context's stream has drained
-            ms = 1e3 * (time.perf_counter() - t0) / steps
-            table.append((ng, nl, ms))
-            if log:
-                log(f"[autotune] n_group={ng:3d} n_leaf={nl:3d}: {ms:8.3f} ms/step")
-            del sim
-    best = min(table, key=lambda r: r[2])
-    return TuneResult(best[0], best[1], best[2], table)
+            run(n_group=ng, n_leaf=nl, **base)
+    best = min(table, key=lambda r: r["ms_per_step"])
+    for p in persistents:
+        for fl in front_lds:
+            if p == best["persistent"] and fl == best["list_front_lds"]:
+                continue
+            run(n_group=best["n_group"], n_leaf=best["n_leaf"], persistent=p, list_front_lds=fl)
+    best = min(table, key=lambda r: r["ms_per_step"])
+    return TuneResult(best["n_group"], best["n_leaf"], best["ms_per_step"], best["persistent"],
+                      best["list_front_lds"], table)

@@ -155,6 +155,7 @@ struct SimState {
     PArrays P{}, Palt{};
     Tree tree;
     Lists lsph, lgrav;
+    DevBuf<int32_t> sched;  // per-XCD group counters of persistent force-kernel launches
     DevBuf<double> dt_sys, time_sys, dt_part;
     hipEvent_t ev[kStages + 1] = {};
     hipEvent_t xev[4] = {};  // cross-stream order of a force evaluation (no timing)
@@ -606,6 +607,11 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
     double *hsave = S.hsave.ensure(n);
     SPH_HIP(hipMemcpyAsync(hsave, P.h, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     Lists &L = S.lsph, &Lg = S.lgrav;
+    // persistent launches of the force kernels named by the bitmask (1 density,
+    // 2 derivative, 4 hydro, 8 gravity); their counters live on the main stream,
+    // where every force kernel runs, one launch after another
+    int32_t *sched_buf = S.prm.persistent ? S.sched.ensure(8) : nullptr;
+    auto sched_for = [&](int bit) { return (S.prm.persistent & bit) ? sched_buf : nullptr; };
     ++S.force_evals;
     auto lref = [&]() { return ListRef{L.idx.get(), nullptr, L.cnt.get(), L.cap}; };
     for (int attempt = 0;; ++attempt) {
@@ -647,6 +653,7 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
                 enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s, L.flag.get());
             }
             DensArgs a{G, gs, P.x, P.y, P.z, P.m, P.h, P.m, P.x, P.y, P.z, lref(), P.rho, P.h, P.ngb, kpos};
+            a.sched = sched_for(1);
             // the density of the last pass feeds the EoS; the precision form
             // keeps it FP64 unless asked otherwise (the earlier passes only steer h)
             const bool d64 = !mixed || (loop == 3 && S.prm.eos_density_fp64);
@@ -663,6 +670,7 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
         {
             DrvtArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.cs, P.m, P.x, P.y, P.z, P.vx, P.vy, P.vz,
                        lref(), P.divv, P.rotx, P.roty, P.rotz, P.gs, P.bal, kpos, kvel};
+            a.sched = sched_for(2);
             kbegin(&L, false, 1);
             if (mixed) launch_drvt_mixed(a, s);
             else launch_drvt_fp64(a, s);
@@ -683,6 +691,7 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
                        t.comx.get(), t.comy.get(), t.comz.get(), t.mass.get(),
                        ListRef{Lg.sidx.get(), nullptr, Lg.scnt.get(), Lg.scap},
                        P.gx, P.gy, P.gz, P.pot};
+            a.sched = sched_for(8);
             kbegin(&Lg, true, 3);
             if (mixed) launch_grav_mixed(a, s);
             else launch_grav_fp64(a, s);
@@ -696,6 +705,7 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
             HydroArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal,
                         P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal, P.m,
                         lref(), P.ax, P.ay, P.az, P.du, P.dti, rec};
+            a.sched = sched_for(4);
             kbegin(&L, false, 2);
             if (mixed) launch_hydro_mixed(a, s);
             else launch_hydro_fp64(a, s);
@@ -1050,6 +1060,7 @@ void sphgpu_sim_default_params(sphgpu_sim_params *p) {
     p->list_cap_spj = 8192;
     p->list_cap_front = 4096;
     p->list_front_lds = kFrontLds;
+    p->persistent = 0;
 }
 
 int32_t sphgpu_sim_init(sphgpu_ctx *c, const sphgpu_sim_params *prm, int64_t n, const double *pos, const double *vel,

@@ -122,6 +122,9 @@ struct DensArgs {
     int32_t *ngb_out;
     // if set: the j side packed as (x, y, z, m), one 32-byte record per particle
     const double4 *jpos = nullptr;
+    // persistent scheduling: 8 per-XCD group counters, cleared before the launch
+    // (null: one workgroup per group)
+    int32_t *sched = nullptr;
 };
 
 struct DrvtArgs {
@@ -134,6 +137,9 @@ struct DrvtArgs {
     double *divv, *rotx, *roty, *rotz, *gs, *bal;
     // if set: the j side packed as (x, y, z, m) and (vx, vy, vz, 0)
     const double4 *jpos = nullptr, *jvel = nullptr;
+    // persistent scheduling: 8 per-XCD group counters, cleared before the launch
+    // (null: one workgroup per group)
+    int32_t *sched = nullptr;
 };
 
 // The j side of the hydro pass as one 128-byte record per particle (one
@@ -154,6 +160,9 @@ struct HydroArgs {
     ListRef L;
     double *ax, *ay, *az, *du, *dt;
     const HydroRec *jrec = nullptr;  // if set: the j side is read from here (list indices index it)
+    // persistent scheduling: 8 per-XCD group counters, cleared before the launch
+    // (null: one workgroup per group)
+    int32_t *sched = nullptr;
 };
 
 struct GravArgs {
@@ -167,11 +176,20 @@ struct GravArgs {
     const double *xs, *ys, *zs, *ms;  // monopoles (SPJ); may be null
     ListRef Ls;
     double *ax, *ay, *az, *pot;
+    // persistent scheduling: 8 per-XCD group counters, cleared before the launch
+    // (null: one workgroup per group)
+    int32_t *sched = nullptr;
 };
 
 // Pack the j records of the density and derivative passes for particles
 // [0, n): (x, y, z, m) and (vx, vy, vz, 0); positions, masses and velocities
 // are fixed during a force evaluation.
+// Persistent launches of the interaction kernels: the 8 per-XCD group
+// counters cleared by a small kernel on the launch's stream, and the grid of
+// workgroups the GPU holds at once for `kernel` (occupancy x compute units),
+// a multiple of 8, at most the group count rounded up to 8.
+void launch_sched_reset(int32_t *sched, hipStream_t s);
+int persistent_grid(const void *kernel, int ngroups);
 void launch_pack_kin(int64_t n, const double *x, const double *y, const double *z, const double *m, const double *vx,
                      const double *vy, const double *vz, double4 *pos, double4 *vel, hipStream_t s);
 // Pack the hydro records of particles [0, n) from the SoA fields.
@@ -312,6 +330,35 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
     const int x = b & 7, k = b >> 3;
     return x * q + (x < r ? x : r) + k;
 }
+
+// Work distribution of the interaction kernels over the i-groups.  Without a
+// scheduler each workgroup takes one group through xcd_remap.  With one (8
+// per-XCD counters, zero at the launch) a fixed grid of persistent workgroups,
+// a multiple of 8 so that each keeps one XCD, loops: a workgroup takes the
+// next group of its own XCD's contiguous range (the ranges of xcd_remap), and
+// when that is exhausted the next group of another XCD's range.
+struct GroupIter {
+    int32_t *sched;
+    int n, x;
+    __device__ GroupIter(int32_t *s, int ngroups) : sched(s), n(ngroups), x((int) (blockIdx.x & 7)) {}
+    __device__ int first() {
+        if (!sched) return (int) blockIdx.x < n ? xcd_remap((int) blockIdx.x, n) : -1;
+        return next();
+    }
+    __device__ int next() {
+        if (!sched) return -1;
+        const int q = n >> 3, r = n & 7;
+        for (int k = 0; k < 8; ++k) {
+            const int X = (x + k) & 7;
+            const int lo = X * q + (X < r ? X : r), cnt = q + (X < r ? 1 : 0);
+            int t = 0;
+            if (threadIdx.x == 0) t = atomicAdd(&sched[X], 1);
+            t = __shfl(t, 0, kWave);
+            if (t < cnt) return lo + t;
+        }
+        return -1;
+    }
+};
 
 // Tiling of a group's j-list: T = ceil(nj / 64) tiles, tile t holding the
 // entries t, t + T, t + 2T, ... (lane l loads entry t + T l).  Interleaving

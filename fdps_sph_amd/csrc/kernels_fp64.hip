@@ -13,6 +13,9 @@
 // terms.  Each lane sums its pairs sequentially in tile order (see Tiles).
 // Pairs outside the support are skipped: there W and gradW are exactly zero,
 // so the skip changes no sum.
+#include <map>
+#include <mutex>
+
 #include "internal.hpp"
 
 namespace sphgpu {
@@ -130,8 +133,7 @@ __device__ __forceinline__ float wave_maxf_(float v) {
 // measured faster than the compiler's own choice (4 waves); the kernel now fits
 // 71 registers (7 waves).  A cap for 8 waves (64 registers, two spilled)
 // measured slower: 0.508 -> 0.516 ms at 1M.
-__global__ __launch_bounds__(kWave, 5) void k_dens(DensArgs a) {
-    const int g = xcd_remap(blockIdx.x, a.ngroups);
+__device__ __forceinline__ void dens_group(const DensArgs &a, const int g) {
     const int lane = threadIdx.x;
     const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
     const bool act = lane < ni;
@@ -184,12 +186,20 @@ __global__ __launch_bounds__(kWave, 5) void k_dens(DensArgs a) {
         if (a.ngb_out) a.ngb_out[ip] = ngb;
     }
 }
+__global__ __launch_bounds__(kWave, 5) void k_dens(DensArgs a) { dens_group(a, xcd_remap((int) blockIdx.x, a.ngroups)); }
+// persistent form: a fixed grid taking groups from the per-XCD counters
+__global__ __launch_bounds__(kWave, 5) void k_dens_persistent(DensArgs a) {
+    GroupIter it(a.sched, a.ngroups);
+    for (int g = it.first(); g >= 0; g = it.next()) {
+        dens_group(a, g);
+        __syncthreads();  // the next group reuses the LDS tiles
+    }
+}
 
 // ---------------------------------------------------------------- derivative
 // CalcDerivative::operator() (src/force.h:89-106) + Balsara switch of
 // RealPtcl::copyFromForce (src/class.h:132-142) when csi is given.
-__global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
-    const int g = xcd_remap(blockIdx.x, a.ngroups);
+__device__ __forceinline__ void drvt_group(const DrvtArgs &a, const int g) {
     const int lane = threadIdx.x;
     const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
     const bool act = lane < ni;
@@ -281,6 +291,15 @@ __global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
             a.bal[ip] = fabs(divv) / (fabs(divv) + sqrt(rx * rx + ry * ry + rz * rz) + 1.0e-4 * a.csi[ip] / hi);
     }
 }
+__global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) { drvt_group(a, xcd_remap((int) blockIdx.x, a.ngroups)); }
+// persistent form: a fixed grid taking groups from the per-XCD counters
+__global__ __launch_bounds__(kWave) void k_drvt_persistent(DrvtArgs a) {
+    GroupIter it(a.sched, a.ngroups);
+    for (int g = it.first(); g >= 0; g = it.next()) {
+        drvt_group(a, g);
+        __syncthreads();  // the next group reuses the LDS tiles
+    }
+}
 
 // ---------------------------------------------------------------- hydro force
 // CalcHydroForce::operator() (src/force.h:112-147), `#if 1` branch.  v_sig_max
@@ -325,8 +344,7 @@ __global__ void k_pack_hydro(int64_t n, const double *x, const double *y, const 
 }
 // 4 waves per SIMD (128 registers, seven spilled outside the pair loops):
 // measured 2.09 ms at 1M against 2.14 ms at the compiler's 3 waves.
-__global__ __launch_bounds__(kWave, 4) void k_hydro(HydroArgs a) {
-    const int g = xcd_remap(blockIdx.x, a.ngroups);
+__device__ __forceinline__ void hydro_group(const HydroArgs &a, const int g) {
     const int lane = threadIdx.x;
     const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
     const bool act = lane < ni;
@@ -474,6 +492,15 @@ __global__ __launch_bounds__(kWave, 4) void k_hydro(HydroArgs a) {
         a.dt[ip] = kCFL * 2.0 * hi / vmax;
     }
 }
+__global__ __launch_bounds__(kWave, 4) void k_hydro(HydroArgs a) { hydro_group(a, xcd_remap((int) blockIdx.x, a.ngroups)); }
+// persistent form: a fixed grid taking groups from the per-XCD counters
+__global__ __launch_bounds__(kWave, 4) void k_hydro_persistent(HydroArgs a) {
+    GroupIter it(a.sched, a.ngroups);
+    for (int g = it.first(); g >= 0; g = it.next()) {
+        hydro_group(a, g);
+        __syncthreads();  // the next group reuses the LDS tiles
+    }
+}
 
 // ---------------------------------------------------------------- gravity
 // CalcGravityForce<EPJ::Grav> then CalcGravityForce<SPJMonopole>
@@ -501,8 +528,7 @@ __device__ __forceinline__ void grav_tile_(const double4 *sp, int nt, double xi,
     }
 }
 
-__global__ __launch_bounds__(kWave) void k_grav(GravArgs a) {
-    const int g = xcd_remap(blockIdx.x, a.ngroups);
+__device__ __forceinline__ void grav_group(const GravArgs &a, const int g) {
     const int lane = threadIdx.x;
     const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
     const bool act = lane < ni;
@@ -538,6 +564,15 @@ __global__ __launch_bounds__(kWave) void k_grav(GravArgs a) {
         a.pot[ip] = pot;
     }
 }
+__global__ __launch_bounds__(kWave) void k_grav(GravArgs a) { grav_group(a, xcd_remap((int) blockIdx.x, a.ngroups)); }
+// persistent form: a fixed grid taking groups from the per-XCD counters
+__global__ __launch_bounds__(kWave) void k_grav_persistent(GravArgs a) {
+    GroupIter it(a.sched, a.ngroups);
+    for (int g = it.first(); g >= 0; g = it.next()) {
+        grav_group(a, g);
+        __syncthreads();  // the next group reuses the LDS tiles
+    }
+}
 
 }  // namespace
 
@@ -557,25 +592,53 @@ void launch_pack_hydro(int64_t n, const double *x, const double *y, const double
                        rho, cs, bal, gs, p, h, out);
     SPH_LAUNCH_CHECK();
 }
-void launch_dens_fp64(const DensArgs &a, hipStream_t s) {
-    if (a.ngroups <= 0) return;
-    hipLaunchKernelGGL(k_dens, dim3(a.ngroups), dim3(kWave), 0, s, a);
+namespace {
+__global__ void k_sched_reset(int32_t *sched) {
+    if (threadIdx.x < 8) sched[threadIdx.x] = 0;
+}
+}  // namespace
+
+void launch_sched_reset(int32_t *sched, hipStream_t s) {
+    hipLaunchKernelGGL(k_sched_reset, dim3(1), dim3(kWave), 0, s, sched);
     SPH_LAUNCH_CHECK();
 }
-void launch_drvt_fp64(const DrvtArgs &a, hipStream_t s) {
+
+int persistent_grid(const void *kernel, int ngroups) {
+    // resident workgroups per (device, kernel), looked up once
+    static std::mutex mu;
+    static std::map<std::pair<int, const void *>, int> cache;
+    int dev = 0;
+    SPH_HIP(hipGetDevice(&dev));
+    int grid;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cache.find({dev, kernel});
+        if (it == cache.end()) {
+            int ncu = 0, per_cu = 0;
+            SPH_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+            SPH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kWave, 0));
+            it = cache.emplace(std::make_pair(dev, kernel), std::max(8, (ncu * std::max(per_cu, 1)) & ~7)).first;
+        }
+        grid = it->second;
+    }
+    return std::min(grid, (ngroups + 7) & ~7);
+}
+
+template <class A>
+static void launch_groups(void (*k)(A), void (*kp)(A), const A &a, hipStream_t s) {
     if (a.ngroups <= 0) return;
-    hipLaunchKernelGGL(k_drvt, dim3(a.ngroups), dim3(kWave), 0, s, a);
+    if (a.sched) {
+        launch_sched_reset(a.sched, s);
+        hipLaunchKernelGGL(kp, dim3(persistent_grid((const void *) kp, a.ngroups)), dim3(kWave), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(k, dim3(a.ngroups), dim3(kWave), 0, s, a);
+    }
     SPH_LAUNCH_CHECK();
 }
-void launch_hydro_fp64(const HydroArgs &a, hipStream_t s) {
-    if (a.ngroups <= 0) return;
-    hipLaunchKernelGGL(k_hydro, dim3(a.ngroups), dim3(kWave), 0, s, a);
-    SPH_LAUNCH_CHECK();
-}
-void launch_grav_fp64(const GravArgs &a, hipStream_t s) {
-    if (a.ngroups <= 0) return;
-    hipLaunchKernelGGL(k_grav, dim3(a.ngroups), dim3(kWave), 0, s, a);
-    SPH_LAUNCH_CHECK();
-}
+
+void launch_dens_fp64(const DensArgs &a, hipStream_t s) { launch_groups(k_dens, k_dens_persistent, a, s); }
+void launch_drvt_fp64(const DrvtArgs &a, hipStream_t s) { launch_groups(k_drvt, k_drvt_persistent, a, s); }
+void launch_hydro_fp64(const HydroArgs &a, hipStream_t s) { launch_groups(k_hydro, k_hydro_persistent, a, s); }
+void launch_grav_fp64(const GravArgs &a, hipStream_t s) { launch_groups(k_grav, k_grav_persistent, a, s); }
 
 }  // namespace sphgpu

@@ -87,8 +87,7 @@ __device__ __forceinline__ void load_xyzm(const DensArgs &a, int q, double &x, d
 
 // ---------------------------------------------------------------- density
 // At least 5 waves per SIMD (no spills; measured faster than 4).
-__global__ __launch_bounds__(kWave, 5) void k_dens_mixed(DensArgs a) {
-    const int g = xcd_remap(blockIdx.x, a.ngroups);
+__device__ __forceinline__ void dens_mixed_group(const DensArgs &a, const int g) {
     const int lane = threadIdx.x;
     const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
     const bool act = lane < ni;
@@ -163,10 +162,18 @@ __global__ __launch_bounds__(kWave, 5) void k_dens_mixed(DensArgs a) {
         if (a.ngb_out) a.ngb_out[ip] = ngb;
     }
 }
+__global__ __launch_bounds__(kWave, 5) void k_dens_mixed(DensArgs a) { dens_mixed_group(a, xcd_remap((int) blockIdx.x, a.ngroups)); }
+// persistent form: a fixed grid taking groups from the per-XCD counters
+__global__ __launch_bounds__(kWave, 5) void k_dens_mixed_persistent(DensArgs a) {
+    GroupIter it(a.sched, a.ngroups);
+    for (int g = it.first(); g >= 0; g = it.next()) {
+        dens_mixed_group(a, g);
+        __syncthreads();  // the next group reuses the LDS tiles
+    }
+}
 
 // ---------------------------------------------------------------- derivative
-__global__ __launch_bounds__(kWave) void k_drvt_mixed(DrvtArgs a) {
-    const int g = xcd_remap(blockIdx.x, a.ngroups);
+__device__ __forceinline__ void drvt_mixed_group(const DrvtArgs &a, const int g) {
     const int lane = threadIdx.x;
     const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
     const bool act = lane < ni;
@@ -258,6 +265,15 @@ __global__ __launch_bounds__(kWave) void k_drvt_mixed(DrvtArgs a) {
         if (a.csi) a.bal[ip] = fabs(dv) / (fabs(dv) + sqrt(ox * ox + oy * oy + oz * oz) + 1.0e-4 * a.csi[ip] / hid);
     }
 }
+__global__ __launch_bounds__(kWave) void k_drvt_mixed(DrvtArgs a) { drvt_mixed_group(a, xcd_remap((int) blockIdx.x, a.ngroups)); }
+// persistent form: a fixed grid taking groups from the per-XCD counters
+__global__ __launch_bounds__(kWave) void k_drvt_mixed_persistent(DrvtArgs a) {
+    GroupIter it(a.sched, a.ngroups);
+    for (int g = it.first(); g >= 0; g = it.next()) {
+        drvt_mixed_group(a, g);
+        __syncthreads();  // the next group reuses the LDS tiles
+    }
+}
 
 // ---------------------------------------------------------------- hydro force
 // One entry of a hydro tile in LDS, read by the pair loops as five 16-byte
@@ -274,8 +290,7 @@ static_assert(sizeof(HydroSlotF) == 80, "LDS record stride");
 
 // 4 waves per SIMD: at 5 waves (96 registers) 27 spill, and the kernel took
 // 1.33 ms at 1M against 1.20 ms at 4 waves (124 registers).
-__global__ __launch_bounds__(kWave, 4) void k_hydro_mixed(HydroArgs a) {
-    const int g = xcd_remap(blockIdx.x, a.ngroups);
+__device__ __forceinline__ void hydro_mixed_group(const HydroArgs &a, const int g) {
     const int lane = threadIdx.x;
     const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
     const bool act = lane < ni;
@@ -415,6 +430,15 @@ __global__ __launch_bounds__(kWave, 4) void k_hydro_mixed(HydroArgs a) {
         a.dt[ip] = kCFL * 2.0 * hid / (double) vmax;
     }
 }
+__global__ __launch_bounds__(kWave, 4) void k_hydro_mixed(HydroArgs a) { hydro_mixed_group(a, xcd_remap((int) blockIdx.x, a.ngroups)); }
+// persistent form: a fixed grid taking groups from the per-XCD counters
+__global__ __launch_bounds__(kWave, 4) void k_hydro_mixed_persistent(HydroArgs a) {
+    GroupIter it(a.sched, a.ngroups);
+    for (int g = it.first(); g >= 0; g = it.next()) {
+        hydro_mixed_group(a, g);
+        __syncthreads();  // the next group reuses the LDS tiles
+    }
+}
 
 // ---------------------------------------------------------------- gravity
 __device__ __forceinline__ void grav_tile_mixed(const float *sx, const float *sy, const float *sz, const float *sgm,
@@ -434,8 +458,7 @@ __device__ __forceinline__ void grav_tile_mixed(const float *sx, const float *sy
     }
 }
 
-__global__ __launch_bounds__(kWave) void k_grav_mixed(GravArgs a) {
-    const int g = xcd_remap(blockIdx.x, a.ngroups);
+__device__ __forceinline__ void grav_mixed_group(const GravArgs &a, const int g) {
     const int lane = threadIdx.x;
     const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
     const bool act = lane < ni;
@@ -482,28 +505,41 @@ __global__ __launch_bounds__(kWave) void k_grav_mixed(GravArgs a) {
         a.pot[ip] = pot.value();
     }
 }
+__global__ __launch_bounds__(kWave) void k_grav_mixed(GravArgs a) { grav_mixed_group(a, xcd_remap((int) blockIdx.x, a.ngroups)); }
+// persistent form: a fixed grid taking groups from the per-XCD counters
+__global__ __launch_bounds__(kWave) void k_grav_mixed_persistent(GravArgs a) {
+    GroupIter it(a.sched, a.ngroups);
+    for (int g = it.first(); g >= 0; g = it.next()) {
+        grav_mixed_group(a, g);
+        __syncthreads();  // the next group reuses the LDS tiles
+    }
+}
 
 }  // namespace
 
-void launch_dens_mixed(const DensArgs &a, hipStream_t s) {
+template <class A>
+static void launch_groups(void (*k)(A), void (*kp)(A), const A &a, hipStream_t s) {
     if (a.ngroups <= 0) return;
-    hipLaunchKernelGGL(k_dens_mixed, dim3(a.ngroups), dim3(kWave), 0, s, a);
+    if (a.sched) {
+        launch_sched_reset(a.sched, s);
+        hipLaunchKernelGGL(kp, dim3(persistent_grid((const void *) kp, a.ngroups)), dim3(kWave), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(k, dim3(a.ngroups), dim3(kWave), 0, s, a);
+    }
     SPH_LAUNCH_CHECK();
+}
+
+void launch_dens_mixed(const DensArgs &a, hipStream_t s) {
+    launch_groups(k_dens_mixed, k_dens_mixed_persistent, a, s);
 }
 void launch_drvt_mixed(const DrvtArgs &a, hipStream_t s) {
-    if (a.ngroups <= 0) return;
-    hipLaunchKernelGGL(k_drvt_mixed, dim3(a.ngroups), dim3(kWave), 0, s, a);
-    SPH_LAUNCH_CHECK();
+    launch_groups(k_drvt_mixed, k_drvt_mixed_persistent, a, s);
 }
 void launch_hydro_mixed(const HydroArgs &a, hipStream_t s) {
-    if (a.ngroups <= 0) return;
-    hipLaunchKernelGGL(k_hydro_mixed, dim3(a.ngroups), dim3(kWave), 0, s, a);
-    SPH_LAUNCH_CHECK();
+    launch_groups(k_hydro_mixed, k_hydro_mixed_persistent, a, s);
 }
 void launch_grav_mixed(const GravArgs &a, hipStream_t s) {
-    if (a.ngroups <= 0) return;
-    hipLaunchKernelGGL(k_grav_mixed, dim3(a.ngroups), dim3(kWave), 0, s, a);
-    SPH_LAUNCH_CHECK();
+    launch_groups(k_grav_mixed, k_grav_mixed_persistent, a, s);
 }
 
 }  // namespace sphgpu

@@ -180,6 +180,11 @@ typedef struct {
                                 default 4096 (grown on overflow like the list capacities) */
     int32_t list_front_lds;  /* frontier entries a tree walk keeps in LDS before it spills
                                 to global memory (1..512), default 512 */
+    int32_t persistent;      /* bitmask of interaction kernels run as one persistent launch of
+                                the workgroups the GPU holds at once, taking the groups of
+                                all systems from per-XCD counters: 1 density, 2 derivative,
+                                4 hydro, 8 gravity; 0 (default): one workgroup per group.
+                                Results are identical either way. */
 } sphgpu_sim_params;
 
 void sphgpu_sim_default_params(sphgpu_sim_params *p);

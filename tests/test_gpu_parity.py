@@ -386,6 +386,21 @@ def test_second_stream_matches_serial(ctx, request):
         assert np.array_equal(over[f], serial[f]), f
 
 
+@pytest.mark.parametrize("ctx", ["ctx64", "ctxmix"])
+def test_persistent_launches_match(ctx, request):
+    """The interaction kernels as persistent launches over per-XCD group
+    counters give the one-workgroup-per-group launches' results bit for bit,
+    on a batch of several impacts (groups of every system in one launch)."""
+    c = request.getfixturevalue(ctx)
+    ic = IC.batch_of_impacts(n_systems=6, n_min=600, n_max=4000, seed=9)
+    ref, rb = _run(c, ic, 2, nsys=6)
+    per, pb = _run(c, ic, 2, nsys=6, persistent=15)
+    _print(f"[persistent launches] builds {rb} vs {pb}")
+    assert rb == pb
+    for f in _FIELDS:
+        assert np.array_equal(ref[f], per[f]), f
+
+
 def test_replaced_simulation_raises(ctx64):
     ic = IC.granite_impact(450, 50, seed=5)
     a = Simulation(ctx64, ic)
