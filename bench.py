@@ -195,11 +195,13 @@ def measured_counters(kernel, precision, workload):
 
 
 # ------------------------------------------------------------------ runs
-def run(ctx, ic, warmup, steps, label, **params):
-    """Simulation from `ic`: warm-up steps, then `steps` steps timed as one call
-    (the library's stream drains at its end), then a diagnostics pass of a few
-    more steps timed one by one with the per-kernel statistics on."""
+def run(ctx, ic, warmup, steps, label, precision, **params):
+    """Simulation from `ic` in the given precision form: warm-up steps, then
+    `steps` steps timed as one call (the library's stream drains at its end),
+    then a diagnostics pass of a few more steps timed one by one with the
+    per-kernel statistics on."""
     from fdps_sph_amd import Simulation
+    ctx.set_precision(precision)
     t0 = time.perf_counter()
     sim = Simulation(ctx, ic, **params)
     t_init = time.perf_counter() - t0
@@ -318,18 +320,23 @@ def du_term_scale(sim, ngroups=200, seed=7):
     return np.concatenate(ids), np.concatenate(out)
 
 
-def compare_precisions(ctx64, ctx32, ic, warmup, steps, **params):
-    """FP64 and mixed runs of the same workload plus the mixed form's errors."""
-    F = run(ctx64, ic, warmup, steps, "fp64", **params)
-    M = run(ctx32, ic, warmup, steps, "mixed", **params)
+def compare_precisions(ctx, ic, warmup, steps, **params):
+    """FP64 and mixed runs of the same workload plus the mixed form's errors.
+    One context runs both forms: a process has few hardware queues, and a
+    second context's two streams can end up sharing one (measured: the list
+    walks then no longer overlap the force kernels)."""
+    F = run(ctx, ic, warmup, steps, "fp64", "fp64", **params)
+    M = run(ctx, ic, warmup, steps, "mixed", "mixed", **params)
     err0 = force_errors(M["init"], F["init"])
     # one force evaluation of each precision from the FP64 state after the last step
     state = F["state"]
     from fdps_sph_amd import Simulation
-    sf = Simulation(ctx64, state, **params)
+    ctx.set_precision("fp64")
+    sf = Simulation(ctx, state, **params)
     fe = force_fields(sf)
     sids, sscale = du_term_scale(sf)
-    mf = force_fields(Simulation(ctx32, state, **params))
+    ctx.set_precision("mixed")
+    mf = force_fields(Simulation(ctx, state, **params))
     me = force_errors(mf, fe)
     # du/dt of the sampled groups' particles on the size of its summed terms
     pf = {k: v for k, v in zip(fe["ID"], fe["ENGDOT"])}
@@ -595,20 +602,19 @@ def main():
     n = len(ic["mass"])
     nsys = int(ic["sys"].max()) + 1 if "sys" in ic else 1
     precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
-    ctx = {}
-    for p in precs:
-        ctx[p] = Context(0, p)
-        load_tables(ctx[p], tables)
+    C = Context(0, precs[0])
+    load_tables(C, tables)
 
     params = {"theta": args.theta, "nsys": nsys, "eos_density_fp64": 0 if args.eos_density_fp32 else 1,
               "persistent": args.persistent}
     extra = {}
     if args.workload == "batch64":
         from fdps_sph_amd import autotune
-        tctx = ctx.get("mixed", ctx.get("fp64"))
-        tr = autotune.tune(tctx, ic, warmup=1, steps=3, log=log, **params)
+        tprec = "mixed" if "mixed" in precs else "fp64"
+        C.set_precision(tprec)
+        tr = autotune.tune(C, ic, warmup=1, steps=3, log=log, **params)
         params.update(tr.params())
-        extra["autotune"] = {"precision": "mixed" if "mixed" in ctx else "fp64",
+        extra["autotune"] = {"precision": tprec,
                              "best": dict(tr.params(), ms_per_step=tr.ms_per_step), "table": tr.table}
 
     sweep = [0.3, 0.5, 0.7] if args.workload == "theta-sweep" else [args.theta]
@@ -616,10 +622,10 @@ def main():
     for th in sweep:
         params["theta"] = th
         if len(precs) == 2:
-            F, M, err, drift = compare_precisions(ctx["fp64"], ctx["mixed"], ic, warmup, steps, **params)
+            F, M, err, drift = compare_precisions(C, ic, warmup, steps, **params)
             results[th] = {"fp64": F, "mixed": M, "err": err, "drift": drift}
         else:
-            R = run(ctx[precs[0]], ic, warmup, steps, precs[0], **params)
+            R = run(C, ic, warmup, steps, precs[0], precs[0], **params)
             results[th] = {precs[0]: R}
 
     th0 = 0.5 if 0.5 in results else sweep[0]
@@ -704,7 +710,7 @@ def main():
         sens = []
         for th in sweep:
             params["theta"] = th
-            P = run(ctx["fp64"], pic, warmup, steps, f"fp64 theta {th}, positions perturbed by {args.sensitivity:g}",
+            P = run(C, pic, warmup, steps, f"fp64 theta {th}, positions perturbed by {args.sensitivity:g}", "fp64",
                     **params)
             F0 = results[th]["fp64"]
             e = {"theta": th, "perturbation": args.sensitivity,
@@ -728,8 +734,7 @@ def main():
         else:  # 1M, batch: the initial force evaluation, one warm-up step, three timed steps
             out["cpu_baseline"] = cpu_baseline(ic, tables, th0, 3, nsys, warm=1)
     print(json.dumps(out), flush=True)
-    for c in ctx.values():
-        c.close()
+    C.close()
 
 
 if __name__ == "__main__":

@@ -72,6 +72,7 @@ def lib() -> C.CDLL:
     L.sphgpu_version.restype = i32
     L.sphgpu_create.argtypes = [C.POINTER(C.c_void_p), i32, i32]
     L.sphgpu_destroy.argtypes = [P]
+    L.sphgpu_set_precision.argtypes = [P, i32]
     for k in ("dens", "drvt", "hydro"):
         getattr(L, f"sphgpu_{k}_dispatch").argtypes = [P, i32, i32, PP, PI, PP, PI]
         getattr(L, f"sphgpu_{k}_retrieve").argtypes = [P, i32, i32, PI, PP]

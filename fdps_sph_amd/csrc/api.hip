@@ -772,6 +772,20 @@ int32_t sphgpu_create(sphgpu_ctx **ctx, int32_t device, int32_t precision) {
     });
 }
 
+int32_t sphgpu_set_precision(sphgpu_ctx *c, int32_t precision) {
+    return guard([&] {
+        check_ctx(c);
+        if (precision != SPHGPU_PREC_FP64 && precision != SPHGPU_PREC_MIXED)
+            throw Error(SPHGPU_ERR_ARG, "unknown precision");
+        for (const Batch &b : c->batches)
+            if (b.pending) throw Error(SPHGPU_ERR_STATE, "a multiwalk batch is pending; retrieve it first");
+        // work already queued keeps the precision it was launched with
+        SPH_HIP(hipStreamSynchronize(c->stream));
+        SPH_HIP(hipStreamSynchronize(c->stream2));
+        c->precision = precision;
+    });
+}
+
 int32_t sphgpu_destroy(sphgpu_ctx *ctx) {
     return guard([&] {
         if (!ctx) return;

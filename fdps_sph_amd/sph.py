@@ -53,6 +53,11 @@ class Context:
     def handle(self):
         return self._h
 
+    def set_precision(self, precision: str):
+        """Switch the form of the work launched from now on ("fp64" or "mixed")."""
+        check(self._L.sphgpu_set_precision(self._h, {"fp64": PREC_FP64, "mixed": PREC_MIXED}[precision]))
+        self.precision = precision
+
     # ------------------------------------------------------------ EoS
     def load_table(self, material: int, path: str):
         check(self._L.sphgpu_eos_load_table(self._h, int(material), path.encode()))

@@ -74,6 +74,13 @@ typedef struct sphgpu_ctx sphgpu_ctx;
 /* ---- context ---- */
 int32_t sphgpu_create(sphgpu_ctx **ctx, int32_t device, int32_t precision);
 int32_t sphgpu_destroy(sphgpu_ctx *ctx);
+/* Switches a context between SPHGPU_PREC_FP64 and SPHGPU_PREC_MIXED for the
+ * work it launches from then on (a simulation's state is the same in both).
+ * A process is given a few hardware queues (4 by default) and a context runs
+ * on two streams, so a second context may share a queue with the first and
+ * lose the overlap of its two streams: one context switched between the
+ * precision forms keeps both forms on the same two queues. */
+int32_t sphgpu_set_precision(sphgpu_ctx *ctx, int32_t precision);
 const char *sphgpu_last_error(void);
 int32_t sphgpu_version(void);
 

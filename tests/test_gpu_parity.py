@@ -401,6 +401,29 @@ def test_persistent_launches_match(ctx, request):
         assert np.array_equal(ref[f], per[f]), f
 
 
+def test_set_precision_matches_dedicated_context(ctxmix):
+    """A context switched to the precision form runs it exactly as a context
+    created in that form, and switched back runs FP64 again."""
+    from fdps_sph_amd import Context
+    ic = IC.granite_impact(9000, 1000, seed=1005)
+    ref, _ = _run(ctxmix, ic, 2)
+    c = Context(0, "fp64")
+    try:
+        for m, k in enumerate(("granite", "iron", "dunite")):
+            c.load_table(m, E.PATHS[k])
+        fp, _ = _run(c, ic, 2)
+        c.set_precision("mixed")
+        sw, _ = _run(c, ic, 2)
+        c.set_precision("fp64")
+        back, _ = _run(c, ic, 2)
+    finally:
+        c.close()
+    for f in _FIELDS:
+        assert np.array_equal(ref[f], sw[f]), f
+        assert np.array_equal(fp[f], back[f]), f
+    assert not np.array_equal(fp["ACC"], sw["ACC"])
+
+
 def test_replaced_simulation_raises(ctx64):
     ic = IC.granite_impact(450, 50, seed=5)
     a = Simulation(ctx64, ic)
