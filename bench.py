@@ -393,7 +393,10 @@ def targets_block(mb, R0, theta, workload):
                                 "pass_max": a["max"] <= 1e-5, "pass_p99": a["p99"] <= 1e-5,
                                 "pass_norm": a["norm"] <= 1e-5},
         "du_dt_rel_err_on_term_scale": du.get("on_term_scale"),
-        "energy_drift_le_1e-6": {"value": drift, "steps": nsteps, "pass": drift <= 1e-6 and nsteps >= 100},
+        # the target is stated over 100 steps: a shorter run reports its drift
+        # without a verdict
+        "energy_drift_le_1e-6": {"value": drift, "steps": nsteps,
+                                 "pass": (drift <= 1e-6) if nsteps >= 100 else None},
     }
 
 
