@@ -28,7 +28,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall", "-Wno
 MFMA_VGPR = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
 SOURCES = {
     "kernels_fp64.hip": ["-ffp-contract=off", *MFMA_VGPR],
-    "tree.hip": ["-ffp-contract=off"],
+    "tree.hip": ["-ffp-contract=off", *MFMA_VGPR],
     "eos.hip": ["-ffp-contract=off"],
     "integrate.hip": ["-ffp-contract=off"],
     "api.hip": ["-ffp-contract=off"],

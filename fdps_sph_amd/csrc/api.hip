@@ -639,6 +639,7 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
         lists_reset(L, s);
         lists_reset(Lg, s);
         enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s);
+        enqueue_gather_compact(S.tree, P, L, kGatherMargin, s, nullptr, kpos);
         signal(S.xev[0], s);  // tree, h restored, counters cleared, first gather lists
         await(s2, S.xev[0]);
         enqueue_lists(S.tree, S.tp, P, 2, Lg, 1.0, s2);
@@ -651,6 +652,7 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
             if (loop > 0) {
                 enqueue_gather_check(S.tree, P, L, s);
                 enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s, L.flag.get());
+                enqueue_gather_compact(S.tree, P, L, kGatherMargin, s, L.flag.get(), kpos);
             }
             DensArgs a{G, gs, P.x, P.y, P.z, P.m, P.h, P.m, P.x, P.y, P.z, lref(), P.rho, P.h, P.ngb, kpos};
             a.sched = sched_for(1);
@@ -667,6 +669,7 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
         rec_ev(4);
         enqueue_gather_check(S.tree, P, L, s);
         enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s, L.flag.get());
+        enqueue_gather_compact(S.tree, P, L, kGatherMargin, s, L.flag.get(), kpos);
         {
             DrvtArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.cs, P.m, P.x, P.y, P.z, P.vx, P.vy, P.vz,
                        lref(), P.divv, P.rotx, P.roty, P.rotz, P.gs, P.bal, kpos, kvel};
@@ -1330,7 +1333,7 @@ int32_t sphgpu_sim_lists(sphgpu_ctx *c, int32_t kind, int64_t *n_groups, int64_t
     return guard([&] {
         check_ctx(c);
         SimState &S = sim_of(c);
-        if (kind < 0 || kind > 2) throw Error(SPHGPU_ERR_ARG, "kind must be 0, 1 or 2");
+        if (kind < 0 || kind > 3) throw Error(SPHGPU_ERR_ARG, "kind must be 0, 1, 2 or 3");
         if (!n_groups || !n_j || !n_s) throw Error(SPHGPU_ERR_ARG, "null size output");
         hipStream_t s = c->stream;
         // The particles are in key order after every force evaluation (init and
@@ -1340,7 +1343,10 @@ int32_t sphgpu_sim_lists(sphgpu_ctx *c, int32_t kind, int64_t *n_groups, int64_t
         tree_build_only(S.tree, S.tp, S.n, S.P, s);
         Lists &L = (kind == 2) ? S.lgrav : S.lsph;
         if (kind == 2) build_lists_grav(S.tree, S.tp, S.P, L, s);
-        else {
+        else if (kind == 3) {
+            build_lists_sph(S.tree, S.tp, S.P, 0, L, s, kGatherMargin);
+            enqueue_gather_compact(S.tree, S.P, L, kGatherMargin, s);
+        } else {
             if (kind == 1) tree_update_maxH(S.tree, S.P.h, s);
             build_lists_sph(S.tree, S.tp, S.P, kind, L, s);
         }

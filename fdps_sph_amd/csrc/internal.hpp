@@ -280,7 +280,9 @@ struct Lists {
     DevBuf<int32_t> flag;      // gather-list validity: 1 when some group outgrew its list
     PinnedBuf<int32_t> hflag;  // host copy of the overflow bits and the construction counter
     DevBuf<double> sbox;     // gather lists: per-group search box (lo[3], hi[3])
+    DevBuf<double> crad;     // compacted gather lists: per-group radius R_g they cover
     int kind = -1;           // kind of the lists held (-1 none, 0 gather, 1 symmetric, 2 gravity)
+    bool compact = false;    // the gather lists were compacted (enqueue_gather_compact)
 };
 // Search-radius factor of the gather lists that the density passes reuse:
 // 8 % on h covers the change of h over the density iterations of nearly every
@@ -304,8 +306,15 @@ void enqueue_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int ki
 void lists_fetch(Lists &L, hipStream_t s, int slot = 0);
 bool lists_grow(Lists &L);
 // Sets L.flag (device) to 1 when some group's current search box (same tree,
-// positions unchanged, h possibly changed) has outgrown its gather list.
+// positions unchanged, h possibly changed) has outgrown its gather list, or,
+// for compacted lists, when some 2.5 h_i of the group exceeds its R_g.
 void enqueue_gather_check(const Tree &t, const PArrays &P, Lists &L, hipStream_t s);
+// Compacts each group's gather list in place to the entries within R_g of
+// some particle of the group, R_g = hfac * max_i 2.5 h_i (h as now), keeping
+// their order; when `need` is given, only if that device flag is nonzero (as
+// the construction it follows).  pos: optional packed (x, y, z, m) records.
+void enqueue_gather_compact(const Tree &t, const PArrays &P, Lists &L, double hfac, hipStream_t s,
+                            const int32_t *need = nullptr, const double4 *pos = nullptr);
 // Synchronous forms: capacities grow until the lists fit.
 bool build_lists_sph(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, hipStream_t s,
                      double hfac = 1.0);
@@ -489,6 +498,7 @@ __device__ __forceinline__ uint64_t mfma_dot_mask(const float4 *q, int nt, float
                                                   float px, float py, float pz, float pc) {
     return mfma_dot_mask_f([q](int e) { return q[e]; }, nt, tx, ty, tz, negc, px, py, pz, pc);
 }
+
 
 // Symmetric form from the matrix cores (inside either support): the bit is
 // set when t <= c_i or t + w_i <= Q_j, t = |o_j|^2 - 2 o_i.o_j as above, c_i

@@ -153,16 +153,30 @@ __device__ __forceinline__ void dens_group(const DensArgs &a, const int g) {
     const int64_t lb = list_begin(a.L, g);
     const Tiles tl(a.L.cnt[g]);
     __shared__ DensSlot sj[kWave];
+    // Software-pipelined tiles: the j record of tile t+1 and the list index of
+    // tile t+2 are in flight while tile t is evaluated (a lane without an entry
+    // loads particle 0, which exists whenever the list is not empty).
+    auto load = [&](int q) { return a.jpos ? a.jpos[q] : make_double4(a.xj[q], a.yj[q], a.zj[q], a.mj[q]); };
+    double4 pq = make_double4(0.0, 0.0, 0.0, 0.0);
+    int qn = 0;
+    if (tl.T > 0) {
+        const int k0 = tl.entry(0, lane), k1 = tl.entry(1, lane);
+        pq = load(k0 < tl.nj ? a.L.idx[lb + k0] : 0);
+        qn = (tl.T > 1 && k1 < tl.nj) ? a.L.idx[lb + k1] : 0;
+    }
     for (int t = 0; t < tl.T; ++t) {
         const int k = tl.entry(t, lane);
         float nj1 = 0.f;
         if (k < tl.nj) {
-            const int q = a.L.idx[lb + k];
-            const double4 pq = a.jpos ? a.jpos[q] : make_double4(a.xj[q], a.yj[q], a.zj[q], a.mj[q]);
             const double x = pq.x, y = pq.y, z = pq.z;
             const float ox = (float) (x - fr.cx), oy = (float) (y - fr.cy), oz = (float) (z - fr.cz);
             sj[lane] = DensSlot{x, y, z, pq.w, make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)))};
             nj1 = norm1_(ox, oy, oz);
+        }
+        if (t + 1 < tl.T) {
+            pq = load(qn);
+            const int k2 = tl.entry(t + 2, lane);
+            qn = (t + 2 < tl.T && k2 < tl.nj) ? a.L.idx[lb + k2] : 0;
         }
         const float negc = act ? -dot_cut_(Hc, wi, n1i, wave_maxf_(nj1)) : INFINITY;
         const float pnegc = __shfl_xor(negc, 32, kWave);
@@ -222,24 +236,37 @@ __device__ __forceinline__ void drvt_group(const DrvtArgs &a, const int g) {
     const int64_t lb = list_begin(a.L, g);
     const Tiles tl(a.L.cnt[g]);
     __shared__ DrvtSlot sj[kWave];
+    // software-pipelined tiles, as in the density pass
+    auto load = [&](int q, double4 &pq, double4 &vq) {
+        if (a.jpos) {
+            pq = a.jpos[q];
+            vq = a.jvel[q];
+        } else {
+            pq = make_double4(a.xj[q], a.yj[q], a.zj[q], a.mj[q]);
+            vq = make_double4(a.vxj[q], a.vyj[q], a.vzj[q], 0.0);
+        }
+    };
+    double4 pq = make_double4(0.0, 0.0, 0.0, 0.0), vq = pq;
+    int qn = 0;
+    if (tl.T > 0) {
+        const int k0 = tl.entry(0, lane), k1 = tl.entry(1, lane);
+        load(k0 < tl.nj ? a.L.idx[lb + k0] : 0, pq, vq);
+        qn = (tl.T > 1 && k1 < tl.nj) ? a.L.idx[lb + k1] : 0;
+    }
     for (int t = 0; t < tl.T; ++t) {
         const int k = tl.entry(t, lane);
         float nj1 = 0.f;
         if (k < tl.nj) {
-            const int q = a.L.idx[lb + k];
-            double4 pq, vq;
-            if (a.jpos) {
-                pq = a.jpos[q];
-                vq = a.jvel[q];
-            } else {
-                pq = make_double4(a.xj[q], a.yj[q], a.zj[q], a.mj[q]);
-                vq = make_double4(a.vxj[q], a.vyj[q], a.vzj[q], 0.0);
-            }
             const double x = pq.x, y = pq.y, z = pq.z;
             const float ox = (float) (x - fr.cx), oy = (float) (y - fr.cy), oz = (float) (z - fr.cz);
             sj[lane] = DrvtSlot{x, y, z, pq.w, vq.x, vq.y, vq.z, 0.0,
                                 make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)))};
             nj1 = norm1_(ox, oy, oz);
+        }
+        if (t + 1 < tl.T) {
+            load(qn, pq, vq);
+            const int k2 = tl.entry(t + 2, lane);
+            qn = (t + 2 < tl.T && k2 < tl.nj) ? a.L.idx[lb + k2] : 0;
         }
         const float negc = act ? -dot_cut_(Hc, wi, n1i, wave_maxf_(nj1)) : INFINITY;
         const float pnegc = __shfl_xor(negc, 32, kWave);
@@ -378,12 +405,16 @@ __device__ __forceinline__ void hydro_group(const HydroArgs &a, const int g) {
     double ax = 0.0, ay = 0.0, az = 0.0, du = 0.0, vmax = 0.0;
     __shared__ HydroSlot sj[kWave];
     __shared__ float sQ[kWave];  // threshold of j's support
+    // the list index of tile t+1 is in flight while tile t is evaluated (the
+    // j record itself would hold 32 more registers through the pair loops)
+    int qn = (tl.T > 0 && tl.entry(0, lane) < tl.nj) ? a.L.idx[lb + tl.entry(0, lane)] : 0;
     for (int t = 0; t < tl.T; ++t) {
         const int k = tl.entry(t, lane);
         float nj1 = 0.f;
         double bj = 0.0;
+        const int q = qn;
+        if (t + 1 < tl.T && tl.entry(t + 1, lane) < tl.nj) qn = a.L.idx[lb + tl.entry(t + 1, lane)];
         if (k < tl.nj) {
-            const int q = a.L.idx[lb + k];
             const HydroJ r = a.jrec ? a.jrec[q]
                                     : make_hydro_rec(a.xj[q], a.yj[q], a.zj[q], a.vxj[q], a.vyj[q], a.vzj[q], a.mj[q],
                                                      a.rhoj[q], a.csj[q], a.balj[q], a.gsj[q], a.pj[q], a.hj[q]);

@@ -811,9 +811,10 @@ __global__ __launch_bounds__(kWave) void k_list_grav(ListArgs a) {
 // indices inside that box, so it is then a superset of the exact list with the
 // in-support particles in the same order.  Sets *flag when some group's box
 // has outgrown its list.
+// A compacted list (k_list_compact) also needs every 2.5 h_i <= R_g = crad[g].
 __global__ __launch_bounds__(kWave) void k_list_box_check(int ngroups, const int32_t *gstart, const double *x,
                                                           const double *y, const double *z, const double *h,
-                                                          const double *sbox, int32_t *flag) {
+                                                          const double *sbox, const double *crad, int32_t *flag) {
     const int g = blockIdx.x;
     if (g >= ngroups) return;
     const int lane = threadIdx.x;
@@ -829,7 +830,128 @@ __global__ __launch_bounds__(kWave) void k_list_box_check(int ngroups, const int
         const double hi = wave_max(act ? c[k] + H : -INFINITY);
         bad = bad || (lo < sbox[(int64_t) g * 6 + k]) || (hi > sbox[(int64_t) g * 6 + 3 + k]);
     }
+    if (crad) bad = bad || (wave_max(act ? H : 0.0) > crad[g]);
     if (lane == 0 && bad) atomicOr(flag, 1);
+}
+
+__device__ __forceinline__ float wave_maxf(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// Which entries of a tile lie inside the cut of some lane: the dot form of
+// mfma_dot_mask_f with the roles of the operands swapped -- rows are the
+// group's particles, columns the tile's entries -- so that each lane holds
+// one entry's column and the OR over the particles is a minimum over the
+// lane's own accumulators (and its partner half's, through one ballot).
+// Lane l supplies A[l%32][l/32] (particle 32 ib + l%32: its own when its half
+// is ib, else its partner's) and B[l/32][l%32] (entry 32 jb + l%32); register
+// r of lane half h holds row 8(r/4) + 4h + r%4 of the lane's column.  K = 6
+// in three steps: particle side (-2 o_x, -2 o_y | -2 o_z, 1 | -c, 0), entry
+// side (o_x, o_y | o_z, w | 1, 0).  Returns the wave-uniform mask of entries
+// < nt with t - c < 0 for some lane (negc = -c; +inf admits nothing).
+__device__ __forceinline__ uint64_t mfma_dot_any_(const float4 *q, int nt, float tx, float ty, float tz, float negc,
+                                                  float px, float py, float pz, float pc) {
+    const int lane = threadIdx.x, h = lane >> 5, m = lane & 31;
+    uint64_t any = 0;
+#pragma unroll
+    for (int jb = 0; jb < 2; ++jb) {
+        const float4 o = q[32 * jb + m];
+        const float b0 = h ? o.y : o.x, b1 = h ? o.w : o.z, b2 = h ? 0.f : 1.f;
+        float mn = INFINITY;
+#pragma unroll
+        for (int ib = 0; ib < 2; ++ib) {
+            const bool own = (ib == h);  // particle 32 ib + m is this lane's own, else its partner's
+            const float a0 = h ? (own ? ty : py) : (own ? tx : px);
+            const float a1 = h ? 1.f : (own ? tz : pz);
+            const float a2 = h ? 0.f : (own ? negc : pc);
+            f32x16_ c = f32x16_{};
+            c = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, b2, c, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) mn = fminf(mn, c[r]);
+        }
+        const uint64_t b = __ballot(mn < 0.f);
+        any |= (uint64_t) ((uint32_t) b | (uint32_t) (b >> 32)) << (32 * jb);
+    }
+    if (nt < kWave) any &= (1ULL << nt) - 1;
+    return any;
+}
+
+// Compaction of the gather lists.  A gather list holds every particle inside
+// the group's search box, most of it beyond every support (the box's corners
+// and the margin on h): the density and derivative kernels stage, mask and
+// skip those entries in every pass.  This keeps, in list order, the entries j
+// with |x_j - x_i| <= R_g for some i of the group, R_g = hfac max_i 2.5 h_i
+// (h as at construction), and records R_g; while every 2.5 h_i <= R_g
+// (checked with the box in k_list_box_check) the kept entries include every
+// particle inside each i's support, in the same order as before.  The test is
+// the matrix-core dot form of the interaction kernels with the lane threshold
+// of R_g (dot_cut_: room for the FP32 offsets' rounding), so it admits a
+// superset of the FP64 condition; an entry is kept when some lane's test
+// admits it (mfma_dot_any_).  In place: chunk c reads entries [64c, 64c + 64) into
+// registers before it writes at most as many entries below 64c + 64, and the
+// chunks after it start at 64c + 64.
+__global__ __launch_bounds__(kWave) void k_list_compact(int ngroups, const int32_t *gstart, const double *x,
+                                                        const double *y, const double *z, const double *h,
+                                                        const double4 *pos, double hfac, int32_t *idx, int32_t *cnt,
+                                                        int64_t cap, double *crad, const int32_t *need) {
+    if (need && *need == 0) return;
+    const int g = xcd_remap((int) blockIdx.x, ngroups);
+    const int lane = threadIdx.x;
+    const int i0 = gstart[g], ni = gstart[g + 1] - i0;
+    const bool act = lane < ni;
+    const int p = i0 + (act ? lane : 0);
+    const double xi = x[p], yi = y[p], zi = z[p];
+    const double R = hfac * wave_max(act ? kSupport * h[p] : 0.0);
+    if (lane == 0) crad[g] = R;
+    const double cx = 0.5 * (wave_min(act ? xi : INFINITY) + wave_max(act ? xi : -INFINITY));
+    const double cy = 0.5 * (wave_min(act ? yi : INFINITY) + wave_max(act ? yi : -INFINITY));
+    const double cz = 0.5 * (wave_min(act ? zi : INFINITY) + wave_max(act ? zi : -INFINITY));
+    const float oxi = (float) (xi - cx), oyi = (float) (yi - cy), ozi = (float) (zi - cz);
+    const float wi = fmaf(oxi, oxi, fmaf(oyi, oyi, ozi * ozi)), n1i = norm1_(oxi, oyi, ozi);
+    const float Rc = (float) R * 1.0001f;
+    const float tx = -2.f * oxi, ty = -2.f * oyi, tz = -2.f * ozi;
+    const float ptx = __shfl_xor(tx, 32, kWave), pty = __shfl_xor(ty, 32, kWave), ptz = __shfl_xor(tz, 32, kWave);
+    int32_t *list = idx + (int64_t) g * cap;
+    const int nj = cnt[g];
+    const uint64_t lt = (1ULL << lane) - 1;
+    __shared__ float4 so[kWave];
+    // software-pipelined chunks: the record of chunk c+1 and the list index of
+    // chunk c+2 are loaded before chunk c is written back (its writes stay below
+    // 64c + 64); a lane past the end loads particle 0
+    auto load = [&](int q) { return pos ? pos[q] : make_double4(x[q], y[q], z[q], 0.0); };
+    int q = (lane < nj) ? list[lane] : 0;
+    double4 r = load(q);
+    int qn = (kWave + lane < nj) ? list[kWave + lane] : 0;
+    int count = 0;
+    for (int base = 0; base < nj; base += kWave) {
+        const int k = base + lane;
+        float n1 = 0.f;
+        if (k < nj) {
+            const float ox = (float) (r.x - cx), oy = (float) (r.y - cy), oz = (float) (r.z - cz);
+            so[lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
+            n1 = norm1_(ox, oy, oz);
+        }
+        const int qk = q;
+        if (base + kWave < nj) {
+            q = qn;
+            r = load(q);
+            qn = (base + 2 * kWave + lane < nj) ? list[base + 2 * kWave + lane] : 0;
+        }
+        const float negc = act ? -dot_cut_(Rc, wi, n1i, wave_maxf(n1)) : INFINITY;
+        const float pnegc = __shfl_xor(negc, 32, kWave);
+        __syncthreads();
+        const uint64_t kept = mfma_dot_any_(so, min(kWave, nj - base), tx, ty, tz, negc, ptx, pty, ptz, pnegc);
+        const bool keep = (k < nj) && ((kept >> lane) & 1);
+        const uint64_t b = __ballot(keep);
+        if (keep) list[count + __popcll(b & lt)] = qk;
+        count += __popcll(b);
+        __syncthreads();
+    }
+    if (lane == 0) cnt[g] = count;
 }
 
 inline int nblk(int64_t n, int b) { return (int) ((n + b - 1) / b); }
@@ -1088,6 +1210,19 @@ void enqueue_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int ki
     else hipLaunchKernelGGL(k_list_grav, dim3(grid), dim3(kWave), 0, s, a);
     SPH_LAUNCH_CHECK();
     L.kind = kind;
+    L.compact = false;
+}
+
+void enqueue_gather_compact(const Tree &t, const PArrays &P, Lists &L, double hfac, hipStream_t s,
+                            const int32_t *need, const double4 *pos) {
+    if (L.kind != 0) throw Error(-1, "enqueue_gather_compact: the lists held are not gather lists");
+    const int64_t G = t.ngroups;
+    L.crad.ensure((size_t) std::max<int64_t>(G, 1));
+    L.compact = true;
+    if (G <= 0) return;
+    hipLaunchKernelGGL(k_list_compact, dim3((unsigned) G), dim3(kWave), 0, s, (int) G, t.gstart.get(), P.x, P.y,
+                       P.z, P.h, pos, hfac, L.idx.get(), L.cnt.get(), L.cap, L.crad.get(), need);
+    SPH_LAUNCH_CHECK();
 }
 
 void lists_fetch(Lists &L, hipStream_t s, int slot) {
@@ -1110,7 +1245,8 @@ void enqueue_gather_check(const Tree &t, const PArrays &P, Lists &L, hipStream_t
     SPH_LAUNCH_CHECK();
     if (t.ngroups <= 0) return;
     hipLaunchKernelGGL(k_list_box_check, dim3((unsigned) t.ngroups), dim3(kWave), 0, s, (int) t.ngroups,
-                       t.gstart.get(), P.x, P.y, P.z, P.h, L.sbox.get(), L.flag.get());
+                       t.gstart.get(), P.x, P.y, P.z, P.h, L.sbox.get(), L.compact ? L.crad.get() : nullptr,
+                       L.flag.get());
     SPH_LAUNCH_CHECK();
 }
 

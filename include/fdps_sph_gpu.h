@@ -252,7 +252,10 @@ int32_t sphgpu_sim_kernel_stats(sphgpu_ctx *ctx, sphgpu_kernel_stat *out, int32_
 
 /* Interaction-list export for the current particle state (sorts and builds
  * the tree first).  kind: 0 gather (density/derivative), 1 symmetric
- * (hydro), 2 gravity.  First call with NULL arrays to get the sizes. */
+ * (hydro), 2 gravity, 3 the gather lists as a force evaluation uses them:
+ * built with the 8 % margin on h and compacted to the entries within R_g of
+ * some particle of the group, R_g = 1.08 max_i 2.5 h_i.  First call with NULL
+ * arrays to get the sizes. */
 int32_t sphgpu_sim_lists(sphgpu_ctx *ctx, int32_t kind, int64_t *n_groups, int64_t *n_j, int64_t *n_s,
                          int32_t *group_start, int64_t *j_off, int32_t *j_idx, int64_t *s_off, int32_t *s_idx);
 /* Tree-node moments after the last tree build: per node mass, com[3],

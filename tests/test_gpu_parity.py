@@ -225,6 +225,40 @@ def test_lists_match_bruteforce(ctx64, oracle, small_impact):
         assert np.array_equal(L["s_idx"][L["s_off"][g]:L["s_off"][g + 1]], es), g
 
 
+def test_compacted_gather_lists(ctx64, small_impact):
+    """The gather lists a force evaluation uses (export kind 3): built with the
+    8 % margin on h, then compacted on the matrix cores to the entries within
+    R_g = 1.08 max_i 2.5 h_i of some particle of the group.  Each group's list
+    must stay in ascending index order, keep every particle inside some
+    member's margin sphere (1.08 * 2.5 h_i, FP64 distances), and hold nothing
+    beyond R_g plus the FP32 test's room or outside the search box."""
+    _print("[compacted gather lists n=2000]")
+    sim = Simulation(ctx64, small_impact)
+    L = sim.lists(3)
+    st = sim.state()
+    x, h = st["pos"], st["smth"]
+    gs = L["group_start"]
+    kept_total = box_total = 0
+    for g in range(len(gs) - 1):
+        i = np.arange(gs[g], gs[g + 1])
+        got = L["j_idx"][L["j_off"][g]:L["j_off"][g + 1]]
+        assert np.all(np.diff(got) > 0), g
+        d = np.sqrt(((x[None, :, :] - x[i][:, None, :]) ** 2).sum(axis=2))  # (n_i, n)
+        Hm = (2.5 * h[i]) * 1.08  # the margin radius, rounded as the list construction rounds it
+        need = np.nonzero(np.any(d <= (Hm * (1 - 1e-9))[:, None], axis=0))[0]
+        assert np.all(np.isin(need, got)), g
+        R = 1.08 * np.max(2.5 * h[i])
+        assert np.all(d[:, got].min(axis=0) <= R * (1 + 1e-4)), g
+        lo = (x[i] - Hm[:, None]).min(axis=0)
+        hi = (x[i] + Hm[:, None]).max(axis=0)
+        inbox = np.all((x >= lo) & (x <= hi), axis=1)
+        assert np.all(inbox[got]), g
+        kept_total += len(got)
+        box_total += int(inbox.sum())
+    _print(f"  kept {kept_total} of {box_total} box entries")
+    assert kept_total < box_total
+
+
 # ------------------------------------------------------------------ multiwalk boundary
 def test_two_tags_in_flight_and_empty_lists(ctx64, oracle):
     """FDPS double-buffers multiwalk batches (tag_max > 1): batches of
