@@ -477,6 +477,7 @@ __device__ __forceinline__ double wave_max(double v) {
 
 struct GBox {
     double slo[3], shi[3], glo[3], ghi[3];
+    double r2;  // gather lists: (largest search radius)^2 (1 + 1e-12), the reach beyond the group's box
 };
 
 __device__ __forceinline__ void group_box(const ListArgs &a, int i0, int ni, GBox &b) {
@@ -492,6 +493,19 @@ __device__ __forceinline__ void group_box(const ListArgs &a, int i0, int ni, GBo
         b.slo[k] = wave_min(act ? c[k] - H : INFINITY);
         b.shi[k] = wave_max(act ? c[k] + H : -INFINITY);
     }
+    const double R = wave_max(act ? H : 0.0);
+    b.r2 = (R * R) * (1.0 + 1.0e-12);
+}
+
+// squared distance from a point to the group's box (0 inside)
+__device__ __forceinline__ double box_dist2(const GBox &b, const double c[3]) {
+    double d2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double d = (c[k] < b.glo[k]) ? (b.glo[k] - c[k]) : ((c[k] > b.ghi[k]) ? (c[k] - b.ghi[k]) : 0.0);
+        d2 = d2 + d * d;
+    }
+    return d2;
 }
 
 template <int KIND>
@@ -500,7 +514,16 @@ __device__ __forceinline__ bool node_test(const ListArgs &a, int id, const GBox 
     bool ov = true;
 #pragma unroll
     for (int k = 0; k < 3; ++k) ov = ov & (lo[k] <= b.shi[k]) & (hi[k] >= b.slo[k]);
-    if (ov || KIND == 0) return ov;
+    if (KIND == 0) {  // and within the search radius of the group's box
+        double d2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double d = (lo[k] > b.ghi[k]) ? (lo[k] - b.ghi[k]) : ((hi[k] < b.glo[k]) ? (b.glo[k] - hi[k]) : 0.0);
+            d2 = d2 + d * d;
+        }
+        return ov && d2 <= b.r2;
+    }
+    if (ov) return ov;
     const double mh = a.maxH[id];
     bool ov2 = true;
 #pragma unroll
@@ -514,7 +537,8 @@ __device__ __forceinline__ bool part_test(const ListArgs &a, int p, const GBox &
     bool in = true;
 #pragma unroll
     for (int k = 0; k < 3; ++k) in = in & (b.slo[k] <= c[k]) & (c[k] <= b.shi[k]);
-    if (in || KIND == 0) return in;
+    if (KIND == 0) return in && box_dist2(b, c) <= b.r2;
+    if (in) return in;
     const double H = kSupport * a.h[p];
     bool ov = true;
 #pragma unroll

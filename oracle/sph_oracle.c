@@ -924,11 +924,13 @@ void orc_tree_update_maxH(orc_tree *t, const orc_state *s)
     }
 }
 
-typedef struct { double slo[3], shi[3], glo[3], ghi[3]; } gbox_;
+typedef struct { double slo[3], shi[3], glo[3], ghi[3], r2; } gbox_;
 
-/* group box (positions) and search box (positions +- 2.5 h) */
+/* group box (positions), search box (positions +- 2.5 h) and, for the gather
+ * lists, the squared reach beyond the group box: (max 2.5 h)^2 (1 + 1e-12) */
 static void group_boxes_(const orc_state *s, int64_t i0, int64_t i1, gbox_ *b)
 {
+    double R = 0.0;
     for (int k = 0; k < 3; ++k) { b->slo[k] = b->glo[k] = INFINITY; b->shi[k] = b->ghi[k] = -INFINITY; }
     for (int64_t p = i0; p < i1; ++p) {
         const double c[3] = {s->x[p], s->y[p], s->z[p]};
@@ -939,7 +941,22 @@ static void group_boxes_(const orc_state *s, int64_t i0, int64_t i1, gbox_ *b)
             b->slo[k] = stdmin_(b->slo[k], c[k] - H);
             b->shi[k] = stdmax_(b->shi[k], c[k] + H);
         }
+        R = stdmax_(R, H);
     }
+    b->r2 = (R * R) * (1.0 + 1.0e-12);
+}
+
+/* a gather list holds the particles inside the search box that are also
+ * within the largest search radius of the group box (the others lie beyond
+ * every member's support) */
+static inline double box_dist2_(const gbox_ *b, const double c[3])
+{
+    double d2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        const double d = (c[k] < b->glo[k]) ? (b->glo[k] - c[k]) : ((c[k] > b->ghi[k]) ? (c[k] - b->ghi[k]) : 0.0);
+        d2 = d2 + d * d;
+    }
+    return d2;
 }
 
 static inline int ptest_(const gbox_ *b, int kind, double x, double y, double z, double H)
@@ -947,7 +964,8 @@ static inline int ptest_(const gbox_ *b, int kind, double x, double y, double z,
     const double c[3] = {x, y, z};
     int in = 1;
     for (int k = 0; k < 3; ++k) in &= (b->slo[k] <= c[k]) & (c[k] <= b->shi[k]);
-    if (in || kind == 0) return in;
+    if (kind == 0) return in && box_dist2_(b, c) <= b->r2;
+    if (in) return in;
     int ov = 1;
     for (int k = 0; k < 3; ++k) ov &= (c[k] - H <= b->ghi[k]) & (c[k] + H >= b->glo[k]);
     return ov;
@@ -958,7 +976,15 @@ static inline int ntest_(const orc_tree *t, int64_t id, const gbox_ *b, int kind
     const double *lo = t->lo + 3 * id, *hi = t->hi + 3 * id;
     int ov = 1;
     for (int k = 0; k < 3; ++k) ov &= (lo[k] <= b->shi[k]) & (hi[k] >= b->slo[k]);
-    if (ov || kind == 0) return ov;
+    if (kind == 0) {
+        double d2 = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const double d = (lo[k] > b->ghi[k]) ? (lo[k] - b->ghi[k]) : ((hi[k] < b->glo[k]) ? (b->glo[k] - hi[k]) : 0.0);
+            d2 = d2 + d * d;
+        }
+        return ov && d2 <= b->r2;
+    }
+    if (ov) return ov;
     const double mh = t->maxH[id];
     int ov2 = 1;
     for (int k = 0; k < 3; ++k) ov2 &= (lo[k] - mh <= b->ghi[k]) & (hi[k] + mh >= b->glo[k]);
