@@ -383,6 +383,65 @@ struct Tiles {
     __device__ __forceinline__ int count(int t) const { return (nj - t + T - 1) / T; }
 };
 
+// Cross-lane operations on the VALU's data-parallel-primitive and lane-swap
+// paths instead of an LDS permute (a ds_bpermute round trip per step).  The
+// reductions take six steps -- swaps within quads (1,0,3,2 and 2,3,0,1), row
+// rotations by 4 and 8 (every lane of a 16-lane row then holds its row's
+// result), lane 15's value into rows 1 and 3, lane 31's into rows 2 and 3 --
+// after which lane 63 holds the wave's, read back as a uniform value.  Lanes
+// of rows a step does not write keep their value, which min and max absorb.
+// Every lane of the wave must take part (uniform control flow: a lane
+// switched off would neither pass its value on nor receive the result).
+// Lane layouts checked on the device against the plain loop.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_i_(int v) { return __builtin_amdgcn_update_dpp(v, v, CTRL, ROWS, 0xF, false); }
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_f_(float v) { return __int_as_float(dpp_i_<CTRL, ROWS>(__float_as_int(v))); }
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_d_(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_i_<CTRL, ROWS>((int) b), hi = dpp_i_<CTRL, ROWS>((int) (b >> 32));
+    return __longlong_as_double((long long) (((unsigned long long) (unsigned) hi << 32) | (unsigned) lo));
+}
+__device__ __forceinline__ float lane63_f_(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
+__device__ __forceinline__ double lane63_d_(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int) b, 63), hi = __builtin_amdgcn_readlane((int) (b >> 32), 63);
+    return __longlong_as_double((long long) (((unsigned long long) (unsigned) hi << 32) | (unsigned) lo));
+}
+#define SPHGPU_WAVE_REDUCE_(v, OP, DPP)  \
+    v = OP(v, DPP<0xB1, 0xF>(v));        \
+    v = OP(v, DPP<0x4E, 0xF>(v));        \
+    v = OP(v, DPP<0x124, 0xF>(v));       \
+    v = OP(v, DPP<0x128, 0xF>(v));       \
+    v = OP(v, DPP<0x142, 0xA>(v));       \
+    v = OP(v, DPP<0x143, 0xC>(v))
+__device__ __forceinline__ double min_d_(double a, double b) { return (b < a) ? b : a; }  // std::min
+__device__ __forceinline__ double max_d_(double a, double b) { return (a < b) ? b : a; }  // std::max
+__device__ __forceinline__ float dpp_wave_maxf(float v) {
+    SPHGPU_WAVE_REDUCE_(v, fmaxf, dpp_f_);
+    return lane63_f_(v);
+}
+__device__ __forceinline__ float dpp_wave_minf(float v) {
+    SPHGPU_WAVE_REDUCE_(v, fminf, dpp_f_);
+    return lane63_f_(v);
+}
+__device__ __forceinline__ double dpp_wave_min(double v) {
+    SPHGPU_WAVE_REDUCE_(v, min_d_, dpp_d_);
+    return lane63_d_(v);
+}
+__device__ __forceinline__ double dpp_wave_max(double v) {
+    SPHGPU_WAVE_REDUCE_(v, max_d_, dpp_d_);
+    return lane63_d_(v);
+}
+#undef SPHGPU_WAVE_REDUCE_
+// The value of lane l ^ 32, through one swap of the two wave halves.
+__device__ __forceinline__ int xor32_i(int v) {
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (threadIdx.x & 32) ? (int) r[0] : (int) r[1];
+}
+__device__ __forceinline__ float xor32_f(float v) { return __int_as_float(xor32_i(__float_as_int(v))); }
+
 // Dot-form mask threshold of one lane for one tile (see dot_mask): the
 // in-support test |x_j - x_i|^2 <= H^2 evaluated through FP32 offsets needs
 // room for the offsets' rounding (radius + 2^-22 (|o_i|_1 + N)) and for the

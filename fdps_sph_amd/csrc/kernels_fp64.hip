@@ -68,6 +68,8 @@ __device__ __forceinline__ Dist dist_(double r2) {
 }
 __device__ __forceinline__ double norm2_(double x, double y, double z) { return fma(z, z, fma(y, y, x * x)); }
 
+// group-level reductions (once per group) through LDS permutes; the per-tile
+// ones take the VALU's lane paths (dpp_wave_*, xor32_f)
 __device__ __forceinline__ double wave_min_(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = dmin(v, __shfl_xor(v, o, kWave));
@@ -121,11 +123,7 @@ __device__ __forceinline__ Frame group_frame(double x, double y, double z, bool 
     const double lz = wave_min_(act ? z : INFINITY), hz = wave_max_(act ? z : -INFINITY);
     return {0.5 * (lx + hx), 0.5 * (ly + hy), 0.5 * (lz + hz)};
 }
-__device__ __forceinline__ float wave_maxf_(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
-    return v;
-}
+__device__ __forceinline__ float wave_maxf_(float v) { return dpp_wave_maxf(v); }
 
 // ---------------------------------------------------------------- density
 // CalcDensity::operator() (src/force.h:9-23)
@@ -178,8 +176,9 @@ __device__ __forceinline__ void dens_group(const DensArgs &a, const int g) {
             const int k2 = tl.entry(t + 2, lane);
             qn = (t + 2 < tl.T && k2 < tl.nj) ? a.L.idx[lb + k2] : 0;
         }
-        const float negc = act ? -dot_cut_(Hc, wi, n1i, wave_maxf_(nj1)) : INFINITY;
-        const float pnegc = __shfl_xor(negc, 32, kWave);
+        const float nj1max = wave_maxf_(nj1);  // every lane takes part: outside the condition
+        const float negc = act ? -dot_cut_(Hc, wi, n1i, nj1max) : INFINITY;
+        const float pnegc = xor32_f(negc);
         __syncthreads();
         const uint64_t mask = mfma_dot_mask_f([&](int e) { return sj[e].o; }, tl.count(t), tx, ty, tz, negc, ptx, pty,
                                               ptz, pnegc);
@@ -268,8 +267,9 @@ __device__ __forceinline__ void drvt_group(const DrvtArgs &a, const int g) {
             const int k2 = tl.entry(t + 2, lane);
             qn = (t + 2 < tl.T && k2 < tl.nj) ? a.L.idx[lb + k2] : 0;
         }
-        const float negc = act ? -dot_cut_(Hc, wi, n1i, wave_maxf_(nj1)) : INFINITY;
-        const float pnegc = __shfl_xor(negc, 32, kWave);
+        const float nj1max = wave_maxf_(nj1);  // every lane takes part: outside the condition
+        const float negc = act ? -dot_cut_(Hc, wi, n1i, nj1max) : INFINITY;
+        const float pnegc = xor32_f(negc);
         __syncthreads();
         const uint64_t mask = mfma_dot_mask_f([&](int e) { return sj[e].o; }, tl.count(t), tx, ty, tz, negc, ptx, pty,
                                               ptz, pnegc);
@@ -426,15 +426,16 @@ __device__ __forceinline__ void hydro_group(const HydroArgs &a, const int g) {
             nj1 = norm1_(ox, oy, oz);
             sQ[lane] = dot_cut_j_((float) r.H * 1.0001f, nj1, n1max);
         }
-        const float negc = act ? -dot_cut_(Hci, wi, n1i, wave_maxf_(nj1)) : INFINITY;
-        const float pnegc = __shfl_xor(negc, 32, kWave);
-        const float dsym = act ? wi - negc : 0.f, pdsym = __shfl_xor(dsym, 32, kWave);
+        const float nj1max = wave_maxf_(nj1);  // every lane takes part: outside the condition
+        const float negc = act ? -dot_cut_(Hci, wi, n1i, nj1max) : INFINITY;
+        const float pnegc = xor32_f(negc);
+        const float dsym = act ? wi - negc : 0.f, pdsym = xor32_f(dsym);
         // the entries that can raise some lane's maximum: b_j >= T, T the least
         // over the lanes of vmax_i (1 - 1e-10) - b_i.  For every other entry
         // b_i + b_j stays below each lane's maximum by a margin far above the
         // rounding of either side, so pass 1 visits only these (none: the tile
         // is skipped)
-        const double T = wave_min_(act ? vmax * (1.0 - 1.0e-10) - bi : INFINITY);
+        const double T = dpp_wave_min(act ? vmax * (1.0 - 1.0e-10) - bi : INFINITY);
         const uint64_t cand = __ballot(k < tl.nj && !(bj < T));
         __syncthreads();
         const int nt = tl.count(t);

@@ -35,6 +35,8 @@ struct Kahan {
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// group-level reductions (once per group) through LDS permutes; the per-tile
+// ones take the VALU's lane paths (dpp_wave_*, xor32_f)
 __device__ __forceinline__ double wave_min(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = dmin(v, __shfl_xor(v, o, kWave));
@@ -45,7 +47,6 @@ __device__ __forceinline__ double wave_max(double v) {
     for (int o = 32; o > 0; o >>= 1) v = dmax(v, __shfl_xor(v, o, kWave));
     return v;
 }
-// centre of the box spanned by the active lanes' values
 __device__ __forceinline__ float wave_maxf(float v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
@@ -56,6 +57,7 @@ __device__ __forceinline__ float wave_minf(float v) {
     for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, kWave));
     return v;
 }
+// centre of the box spanned by the active lanes' values
 __device__ __forceinline__ double wave_centre(double v, bool act) {
     const double lo = wave_min(act ? v : INFINITY);
     const double hi = wave_max(act ? v : -INFINITY);
@@ -356,9 +358,10 @@ __device__ __forceinline__ void hydro_mixed_group(const HydroArgs &a, const int 
             e.a = make_float4((float) Aj, 0.f, 0.f, 0.f);
             slot[lane] = e;
         }
-        const float negc = act ? -dot_cut_(Hci, wi, n1i, wave_maxf(nj1)) : INFINITY;
-        const float pnegc = __shfl_xor(negc, 32, kWave);
-        const float dsym = act ? wi - negc : 0.f, pdsym = __shfl_xor(dsym, 32, kWave);
+        const float nj1max = dpp_wave_maxf(nj1);  // every lane takes part: outside the condition
+        const float negc = act ? -dot_cut_(Hci, wi, n1i, nj1max) : INFINITY;
+        const float pnegc = xor32_f(negc);
+        const float dsym = act ? wi - negc : 0.f, pdsym = xor32_f(dsym);
         // the entries that can raise some lane's v_sig maximum: bound_j >= T,
         // T the least over the lanes of vmax_i (1 - 1e-5) - b_i (a margin far
         // above the FP32 rounding); no other entry can change any maximum

@@ -858,11 +858,7 @@ __global__ __launch_bounds__(kWave) void k_list_box_check(int ngroups, const int
     if (lane == 0 && bad) atomicOr(flag, 1);
 }
 
-__device__ __forceinline__ float wave_maxf(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
-    return v;
-}
+__device__ __forceinline__ float wave_maxf(float v) { return dpp_wave_maxf(v); }
 
 // Which entries of a tile lie inside the cut of some lane: the dot form of
 // mfma_dot_mask_f with the roles of the operands swapped -- rows are the
@@ -967,8 +963,9 @@ __global__ __launch_bounds__(kWave, 8) void k_list_compact(int ngroups, const in
             r = load(q);
             qn = (base + 2 * kWave + lane < nj) ? list[base + 2 * kWave + lane] : 0;
         }
-        const float negc = act ? -dot_cut_(Rc, wi, n1i, wave_maxf(n1)) : INFINITY;
-        const float pnegc = __shfl_xor(negc, 32, kWave);
+        const float n1max = wave_maxf(n1);  // every lane takes part: outside the condition
+        const float negc = act ? -dot_cut_(Rc, wi, n1i, n1max) : INFINITY;
+        const float pnegc = xor32_f(negc);
         __syncthreads();
         const uint64_t kept = mfma_dot_any_(so, min(kWave, nj - base), tx, ty, tz, negc, ptx, pty, ptz, pnegc);
         const bool keep = (k < nj) && ((kept >> lane) & 1);
