@@ -212,9 +212,12 @@ def run(ctx, ic, warmup, steps, label, precision, **params):
     for _ in range(warmup):
         sim.step(1)
         dts.append(float(np.min(sim.time()[0])))
+    g0 = sim.graph_stats()
     t_all = time.perf_counter()
     sim.step(steps)  # returns after the context's stream has drained
     wall = time.perf_counter() - t_all
+    g1 = sim.graph_stats()
+    graph = {"replayed_steps_timed": g1[0] - g0[0], "captures_timed": g1[1] - g0[1], "captures_total": g1[1]}
     stages_timed = sim.stage_ms()  # the last timed step (list walks overlapping kernels)
     parts1 = energy_parts(sim)
     e1 = sum(parts1.values())
@@ -240,7 +243,7 @@ def run(ctx, ic, warmup, steps, label, precision, **params):
             "steps_per_s": steps / wall, "stats": stats, "stages": stages, "stages_timed": stages_timed,
             "init": init, "E0": e0, "E1": e1,
             "parts0": parts0, "parts1": parts1, "warmup_dts": dts, "gather_builds_per_force_eval": gb / max(fe, 1),
-            "t": float(np.max(t)), "steps_total": warmup + steps}
+            "t": float(np.max(t)), "steps_total": warmup + steps, "graph": graph}
 
 
 def err_stats(a, b, ids_a, ids_b, scale=None):
@@ -370,6 +373,7 @@ def precision_block(R, mixed, workload):
             v["counter_tflops"] = cf
             v["counter_frac"] = cf / v["peak"]
     return {"steps_per_s": R["steps_per_s"], "ms_per_step": R["ms_per_step"], "median_ms_per_step_stats_on": R["median_ms"],
+            "graph": R["graph"],
             "gather_builds_per_force_eval": R["gather_builds_per_force_eval"],
             "kernels": {k: {kk: vv for kk, vv in v.items() if kk != "per_launch"} for k, v in roof.items()},
             "per_launch": {k: v["per_launch"] for k, v in roof.items()},
@@ -567,6 +571,11 @@ def main():
     ap.add_argument("--persistent", type=int, default=0, metavar="MASK",
                     help="interaction kernels run as one persistent launch: bitmask 1 density, 2 derivative, "
                          "4 hydro, 8 gravity (batch64 picks it by autotuning)")
+    ap.add_argument("--graph", type=int, choices=[0, 1], default=1,
+                    help="1 (default): every step after the first replays a recorded HIP graph of the whole step; "
+                         "0: steps launched kernel by kernel")
+    ap.add_argument("--walk-waves", type=int, default=0,
+                    help="persistent waves of each interaction-list walk (0: the library's default, 4096)")
     ap.add_argument("--eos-density-fp32", action="store_true",
                     help="precision form: run the last density pass (the EoS input) in FP32 too")
     ap.add_argument("--sensitivity", type=float, default=0.0, metavar="EPS",
@@ -609,7 +618,7 @@ def main():
     load_tables(C, tables)
 
     params = {"theta": args.theta, "nsys": nsys, "eos_density_fp64": 0 if args.eos_density_fp32 else 1,
-              "persistent": args.persistent}
+              "persistent": args.persistent, "graph": args.graph, "walk_waves": args.walk_waves}
     extra = {}
     if args.workload == "batch64":
         from fdps_sph_amd import autotune
@@ -663,7 +672,8 @@ def main():
         "config": {"workload": f"{args.workload}: {desc}", "particles": n, "systems": nsys, "theta": th0,
                    "precision": main_prec, "n_group": params.get("n_group", 64),
                    "n_leaf": params.get("n_leaf", 16), "persistent": params.get("persistent", 0),
-                   "list_front_lds": params.get("list_front_lds", 512),
+                   "list_front_lds": params.get("list_front_lds", 512), "graph": args.graph,
+                   "walk_waves": params.get("walk_waves", 0) or 4096,
                    "eos_density": "fp32" if args.eos_density_fp32 else "fp64"},
         "roofline": roofline,
         main_prec: block,
@@ -727,7 +737,8 @@ def main():
         out["fp64_sensitivity"] = sens if len(sens) > 1 else sens[0]
         params["theta"] = th0
     out.update(extra)
-    out["graph_captured"] = False
+    # every timed step a replay of a recorded graph (the warm-up steps record them)
+    out["graph_captured"] = bool(args.graph) and R["graph"]["replayed_steps_timed"] == steps
 
     if not args.no_cpu_baseline:
         if args.workload == "granite10k":

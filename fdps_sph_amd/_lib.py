@@ -38,7 +38,8 @@ class SimParams(C.Structure):
     _fields_ = [("n_leaf", C.c_int32), ("n_group", C.c_int32), ("theta", C.c_double), ("damping", C.c_double),
                 ("ext_force_until", C.c_double), ("nsys", C.c_int32), ("list_cap_sph", C.c_int32),
                 ("list_cap_epj", C.c_int32), ("list_cap_spj", C.c_int32), ("eos_density_fp64", C.c_int32),
-                ("list_cap_front", C.c_int32), ("list_front_lds", C.c_int32), ("persistent", C.c_int32)]
+                ("list_cap_front", C.c_int32), ("list_front_lds", C.c_int32), ("persistent", C.c_int32),
+                ("graph", C.c_int32), ("walk_waves", C.c_int32)]
 
 
 class KernelStat(C.Structure):
@@ -94,6 +95,8 @@ def lib() -> C.CDLL:
     L.sphgpu_sim_get_time.argtypes = [P, P, P]
     L.sphgpu_sim_stage_ms.argtypes = [P, P, i32]
     L.sphgpu_sim_list_builds.argtypes = [P, P, P]
+    L.sphgpu_sim_graph_stats.argtypes = [P, P, P]
+    L.sphgpu_sim_set_capacities.argtypes = [P, i32, i32, i32]
     L.sphgpu_sim_lists.argtypes = [P, i32, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64), P, P, P, P, P]
     L.sphgpu_sim_nodes.argtypes = [P, C.POINTER(i64), P]
     L.sphgpu_sim_set_stats.argtypes = [P, i32]

@@ -11,14 +11,16 @@ knobs of this implementation:
 * list_front_lds: the entries of a tree walk's frontier kept in LDS before it
   continues in global memory -- the LDS tile of the list construction.  The
   interaction kernels' j-tile is one wave of j-particles (64 entries) by
-  construction: each lane stages one entry.
+  construction: each lane stages one entry;
+* walk_waves: the persistent waves of each list walk, which runs beside the
+  force kernels and shares the GPU with them.
 
 On this device the i-group is one 64-lane wavefront (one lane per
 i-particle), so n_group ranges over sizes up to the wave width.  The search
 is staged: every (n_group, n_leaf) pair first, then the launch form and the
-LDS frontier at the best pair.  Each candidate builds a Simulation from the
-same initial state, runs `warmup` steps untimed and `steps` timed steps, and
-the fastest mean step wins.
+LDS frontier at the best pair, then the walks' waves.  Each candidate builds
+a Simulation from the same initial state, runs `warmup` steps untimed and
+`steps` timed steps, and the fastest mean step wins.
 """
 from __future__ import annotations
 
@@ -32,6 +34,7 @@ N_LEAF_CHOICES = (8, 16, 32)
 # persistent-launch bitmask: none, gravity only, every interaction kernel
 PERSISTENT_CHOICES = (0, 8, 15)
 FRONT_LDS_CHOICES = (128, 512)
+WALK_WAVES_CHOICES = (2048, 4096, 8192)
 
 
 @dataclass
@@ -41,11 +44,12 @@ class TuneResult:
     ms_per_step: float
     persistent: int = 0
     list_front_lds: int = 512
-    table: list = field(default_factory=list)  # dict per candidate: the four knobs and ms_per_step
+    walk_waves: int = 4096
+    table: list = field(default_factory=list)  # dict per candidate: the five knobs and ms_per_step
 
     def params(self) -> dict:
         return {"n_group": self.n_group, "n_leaf": self.n_leaf, "persistent": self.persistent,
-                "list_front_lds": self.list_front_lds}
+                "list_front_lds": self.list_front_lds, "walk_waves": self.walk_waves}
 
 
 def _time(ctx, ic, warmup, steps, **params) -> float:
@@ -58,15 +62,16 @@ def _time(ctx, ic, warmup, steps, **params) -> float:
 
 
 def tune(ctx: Context, ic: dict, *, n_groups=N_GROUP_CHOICES, n_leaves=N_LEAF_CHOICES,
-         persistents=PERSISTENT_CHOICES, front_lds=FRONT_LDS_CHOICES, warmup: int = 1, steps: int = 3, log=None,
-         **params) -> TuneResult:
+         persistents=PERSISTENT_CHOICES, front_lds=FRONT_LDS_CHOICES, wave_counts=WALK_WAVES_CHOICES,
+         warmup: int = 1, steps: int = 3, log=None, **params) -> TuneResult:
     """Time the candidates on `ic` and return the fastest.
 
     `params` are passed to every Simulation (theta, nsys, ...).  Each candidate
     starts from the same initial state, so all of them time the same steps.
     """
     table = []
-    params = {k: v for k, v in params.items() if k not in ("n_group", "n_leaf", "persistent", "list_front_lds")}
+    knobs = ("n_group", "n_leaf", "persistent", "list_front_lds", "walk_waves")
+    params = {k: v for k, v in params.items() if k not in knobs}
 
     def run(**knobs):
         ms = _time(ctx, ic, warmup, steps, **knobs, **params)
@@ -76,7 +81,8 @@ def tune(ctx: Context, ic: dict, *, n_groups=N_GROUP_CHOICES, n_leaves=N_LEAF_CH
             log("[autotune] " + " ".join(f"{k}={v}" for k, v in knobs.items()) + f": {ms:8.3f} ms/step")
         return row
 
-    base = {"persistent": persistents[0], "list_front_lds": front_lds[-1]}
+    ww0 = 4096 if 4096 in wave_counts else wave_counts[0]
+    base = {"persistent": persistents[0], "list_front_lds": front_lds[-1], "walk_waves": ww0}
     for ng in n_groups:
         for nl in n_leaves:
             run(n_group=ng, n_leaf=nl, **base)
@@ -85,7 +91,11 @@ def tune(ctx: Context, ic: dict, *, n_groups=N_GROUP_CHOICES, n_leaves=N_LEAF_CH
         for fl in front_lds:
             if p == best["persistent"] and fl == best["list_front_lds"]:
                 continue
-            run(n_group=best["n_group"], n_leaf=best["n_leaf"], persistent=p, list_front_lds=fl)
+            run(n_group=best["n_group"], n_leaf=best["n_leaf"], persistent=p, list_front_lds=fl, walk_waves=ww0)
+    best = min(table, key=lambda r: r["ms_per_step"])
+    for ww in wave_counts:
+        if ww != best["walk_waves"]:
+            run(**{k: best[k] for k in knobs if k != "walk_waves"}, walk_waves=ww)
     best = min(table, key=lambda r: r["ms_per_step"])
     return TuneResult(best["n_group"], best["n_leaf"], best["ms_per_step"], best["persistent"],
-                      best["list_front_lds"], table)
+                      best["list_front_lds"], best["walk_waves"], table)

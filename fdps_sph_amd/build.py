@@ -29,6 +29,7 @@ MFMA_VGPR = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
 SOURCES = {
     "kernels_fp64.hip": ["-ffp-contract=off", *MFMA_VGPR],
     "tree.hip": ["-ffp-contract=off", *MFMA_VGPR],
+    "sort.hip": [],
     "eos.hip": ["-ffp-contract=off"],
     "integrate.hip": ["-ffp-contract=off"],
     "api.hip": ["-ffp-contract=off"],

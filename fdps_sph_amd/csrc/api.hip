@@ -77,11 +77,11 @@ __global__ void k_iota(int64_t n, int32_t *a) {
 }
 
 // sums of n_i, n_j, n_s and n_i*(n_j+n_s) over the groups of one launch
-__global__ void k_list_counts(int ngroups, const int32_t *gstart, const int32_t *cnt, const int32_t *scnt,
+__global__ void k_list_counts(const int32_t *gcount, const int32_t *gstart, const int32_t *cnt, const int32_t *scnt,
                               long long *acc) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     long long v[4] = {0, 0, 0, 0};
-    if (g < ngroups) {
+    if (g < *gcount) {
         const long long ni = gstart[g + 1] - gstart[g];
         const long long nj = cnt[g];
         const long long ns = scnt ? scnt[g] : 0;
@@ -143,6 +143,21 @@ constexpr int kStages = 8;
 // ------------------------------------------------------------------ simulation state
 constexpr int kStatKinds = 5;
 
+// one recorded step (the graph of one parity of the particle buffer sets)
+struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<int64_t> sig;  // the signature it was recorded with (step_sig)
+    StepGraph() = default;
+    StepGraph(const StepGraph &) = delete;
+    StepGraph &operator=(const StepGraph &) = delete;
+    ~StepGraph() { reset(); }
+    void reset() {
+        if (exec) (void) hipGraphExecDestroy(exec);
+        exec = nullptr;
+        sig.clear();
+    }
+};
+
 struct SimState {
     sphgpu_sim_params prm{};
     TreeParams tp{};
@@ -175,8 +190,16 @@ struct SimState {
     DevBuf<HydroRec> hrec;     // j records of the hydro pass
     DevBuf<double4> kpos, kvel; // j records of the density and derivative passes
     int64_t gather_builds = 0; // gather-list constructions since init
-    int64_t force_evals = 0;   // force evaluations (sim_forces calls) since init
+    int64_t force_evals = 0;   // force evaluations completed since init
+    int kslot = 0;             // statistics slots used by the evaluation in progress
+    PinnedBuf<int32_t> stash;  // error words and counts of the last evaluation (kSt*)
+    // recorded steps, one per parity of the particle buffer sets (prm.graph)
+    StepGraph graphs[2];
+    std::vector<int64_t> direct_sig;  // signature of the last step enqueued directly
+    int64_t graph_steps = 0, graph_captures = 0;
     ~SimState() {
+        graphs[0].reset();
+        graphs[1].reset();
         for (auto &e : ev)
             if (e) (void) hipEventDestroy(e);
         for (auto &e : xev)
@@ -566,13 +589,82 @@ void sim_bind(SimState &S) {
     bind(1, S.Palt);
 }
 
-// One force evaluation (src/main.cpp:125-146).  The list constructions are
-// enqueued without waiting for them: each ORs capacity overflows into its
-// Lists' error bits, and whether a gather list must be rebuilt is decided on
-// the device (enqueue_gather_check / the `need` flag of enqueue_lists).  The
-// host waits once, at the end; when some list overflowed, the capacities grow
-// and the evaluation runs again from the saved h -- the only input it
-// overwrites (the density passes update h).
+// The state a recorded step depends on beyond its buffers' contents: the
+// allocation epoch (every buffer address), precision, kernel options and
+// every capacity.  A recorded step is replayed only while this is unchanged.
+std::vector<int64_t> step_sig(const sphgpu_ctx *c, const SimState &S) {
+    std::vector<int64_t> v;
+    v.push_back((int64_t) alloc_epoch());
+    v.push_back(c->precision);
+    v.push_back(S.prm.persistent);
+    v.push_back(S.prm.eos_density_fp64);
+    v.push_back(S.tree.gcap);
+    for (int l = 0; l < S.tree.nlevels; ++l) v.push_back(S.tree.cap_lv[l]);
+    for (const Lists *L : {&S.lsph, &S.lgrav}) {
+        v.push_back(L->cap);
+        v.push_back(L->scap);
+        v.push_back(L->fcap);
+        v.push_back(L->flds);
+        v.push_back(L->waves);
+    }
+    return v;
+}
+
+// Words the host reads after a force evaluation (SimState::stash): the
+// gather lists' overflow bits and construction counter (taken before the
+// symmetric lists reuse their buffers), the symmetric and the gravity lists'
+// overflow words, and the tree's counts (kInfo*).
+constexpr int kStGather = 0, kStSph = 2, kStGrav = 4, kStInfo = 6, kStWords = kStInfo + kInfoWords;
+
+// Copies up to four word ranges, one after another, into host memory: an
+// ordinary kernel, so that a recorded step carries it.
+struct StashArgs {
+    const int32_t *src[4];
+    int n[4];
+    int32_t *dst;
+};
+__global__ void k_stash(StashArgs a) {
+    int o = 0;
+    for (int k = 0; k < 4; ++k) {
+        if (a.src[k])
+            for (int i = threadIdx.x; i < a.n[k]; i += blockDim.x) a.dst[o + i] = a.src[k][i];
+        o += a.n[k];
+    }
+}
+void launch_stash(StashArgs a, hipStream_t s) {
+    hipLaunchKernelGGL(k_stash, dim3(1), dim3(64), 0, s, a);
+    SPH_LAUNCH_CHECK();
+}
+
+__global__ void k_copy_d(int64_t n, const double *src, double *dst) {
+    const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[k];
+}
+void launch_copy_d(int64_t n, const double *src, double *dst, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_copy_d, dim3(nblk(n, 256)), dim3(256), 0, s, n, src, dst);
+    SPH_LAUNCH_CHECK();
+}
+
+// The error words the last kernels of a step check (StepGuard): the tree's
+// and both list kinds' overflow bits.
+StepGuard step_guard(SimState &S) {
+    StepGuard g;
+    g.w[0] = S.tree.info.get() + kInfoErr;
+    g.w[1] = S.lsph.err.get();
+    g.w[2] = S.lgrav.err.get();
+    return g;
+}
+
+// One force evaluation (src/main.cpp:125-146), enqueued without any wait for
+// the host: the sort and tree build (launches sized by capacities, counts on
+// the device), the lists (each ORs capacity overflows into its error bits;
+// whether a gather list must be rebuilt is decided on the device), the
+// density passes, EoS, derivative, gravity and hydro, and finally the error
+// words and counts copied to S.stash.  `restore_h` first puts back the h that
+// a failed evaluation of the same state started from (the only input an
+// evaluation overwrites: the density passes update h); the sort of the
+// sorted state is then the identity.
 //
 // Two list walks run on a second stream, beside force kernels they do not feed:
 // the walks wait on dependent loads and leave the VALU idle, the kernels are
@@ -583,7 +675,7 @@ void sim_bind(SimState &S) {
 // the derivative pass, while the gravity kernel runs.  With kernel statistics
 // on, everything runs in this order on one stream, so that each kernel's timing
 // is its own.
-void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
+void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
     hipStream_t s = c->stream;
     hipStream_t s2 = S.stats ? s : c->stream2;
     auto signal = [&](hipEvent_t e, hipStream_t from) {  // order later work on the other stream after `from`'s so far
@@ -597,148 +689,270 @@ void sim_forces(sphgpu_ctx *c, SimState &S, bool timed) {
     auto rec_ev = [&](int k) {
         if (timed) SPH_HIP(hipEventRecord(S.ev[k], s));
     };
+    double *hsave = S.hsave.ensure(n);
+    if (restore_h) launch_copy_d(n, hsave, S.P.h, s);
     tree_sort_and_build(S.tree, S.tp, n, S.P, S.Palt, s);
     rec_ev(2);
     PArrays &P = S.P;
-    const int G = (int) S.tree.ngroups;
+    const int G = (int) S.tree.gcap;  // launches are sized by the capacity, the kernels read the count
     const int32_t *gs = S.tree.gstart.get();
+    const int32_t *gcount = S.tree.info.get() + kInfoG;
     double4 *kpos = S.kpos.ensure(n), *kvel = S.kvel.ensure(n);  // j records of the density / derivative passes
     launch_pack_kin(n, P.x, P.y, P.z, P.m, P.vx, P.vy, P.vz, kpos, kvel, s);
-    double *hsave = S.hsave.ensure(n);
-    SPH_HIP(hipMemcpyAsync(hsave, P.h, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    launch_copy_d(n, P.h, hsave, s);
     Lists &L = S.lsph, &Lg = S.lgrav;
+    int32_t *stash = S.stash.p;
     // persistent launches of the force kernels named by the bitmask (1 density,
     // 2 derivative, 4 hydro, 8 gravity); their counters live on the main stream,
     // where every force kernel runs, one launch after another
     int32_t *sched_buf = S.prm.persistent ? S.sched.ensure(8) : nullptr;
     auto sched_for = [&](int bit) { return (S.prm.persistent & bit) ? sched_buf : nullptr; };
-    ++S.force_evals;
     auto lref = [&]() { return ListRef{L.idx.get(), nullptr, L.cnt.get(), L.cap}; };
+    S.kslot = 0;
+    if (S.stats) {
+        S.kcount_try.ensure(4 * kStatKinds);
+        hipLaunchKernelGGL(k_fill_ll, dim3(1), dim3(64), 0, s, (int64_t) 4 * kStatKinds, S.kcount_try.get());
+        SPH_LAUNCH_CHECK();
+    }
+    auto kbegin = [&](const Lists *Lp, bool spj, int kind) {
+        if (!S.stats) return;
+        S.kkind[S.kslot] = kind;
+        hipLaunchKernelGGL(k_list_counts, dim3(nblk(G, 256)), dim3(256), 0, s, gcount, gs, Lp->cnt.get(),
+                           spj ? Lp->scnt.get() : nullptr, S.kcount_try.get() + 4 * kind);
+        SPH_LAUNCH_CHECK();
+        SPH_HIP(hipEventRecord(S.kev[S.kslot][0], s));
+    };
+    auto kend = [&]() {
+        if (!S.stats) return;
+        SPH_HIP(hipEventRecord(S.kev[S.kslot][1], s));
+        ++S.kslot;
+    };
+    lists_reset(L, s);
+    lists_reset(Lg, s);
+    enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s);
+    enqueue_gather_compact(S.tree, P, L, kGatherMargin, s, nullptr, kpos);
+    signal(S.xev[0], s);  // tree, h restored, counters cleared, first gather lists
+    await(s2, S.xev[0]);
+    enqueue_lists(S.tree, S.tp, P, 2, Lg, 1.0, s2);
+    signal(S.xev[1], s2);  // gravity lists
+    // 4 density passes, each over lists that cover the current h
+    // (src/main.cpp:138-140).  One gather list, built with a margin on h,
+    // serves every pass (and the derivative pass) while each group's
+    // search box still fits inside the box it was built for.
+    for (int loop = 0; loop <= 3; ++loop) {
+        if (loop > 0) {
+            enqueue_gather_check(S.tree, P, L, s);
+            enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s, L.flag.get());
+            enqueue_gather_compact(S.tree, P, L, kGatherMargin, s, L.flag.get(), kpos);
+        }
+        DensArgs a{G, gs, P.x, P.y, P.z, P.m, P.h, P.m, P.x, P.y, P.z, lref(), P.rho, P.h, P.ngb, kpos};
+        a.sched = sched_for(1);
+        a.gcount = gcount;
+        // the density of the last pass feeds the EoS; the precision form
+        // keeps it FP64 unless asked otherwise (the earlier passes only steer h)
+        const bool d64 = !mixed || (loop == 3 && S.prm.eos_density_fp64);
+        kbegin(&L, false, (mixed && d64) ? 4 : 0);
+        if (d64) launch_dens_fp64(a, s);
+        else launch_dens_mixed(a, s);
+        kend();
+    }
+    rec_ev(3);
+    launch_eos_pressure(n, P.mat, P.rho, P.u, c->d_mats.get(), SPHGPU_MAX_MATERIALS, P.p, P.cs, s);
+    rec_ev(4);
+    enqueue_gather_check(S.tree, P, L, s);
+    enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s, L.flag.get());
+    enqueue_gather_compact(S.tree, P, L, kGatherMargin, s, L.flag.get(), kpos);
+    {
+        DrvtArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.cs, P.m, P.x, P.y, P.z, P.vx, P.vy, P.vz,
+                   lref(), P.divv, P.rotx, P.roty, P.rotz, P.gs, P.bal, kpos, kvel};
+        a.sched = sched_for(2);
+        a.gcount = gcount;
+        kbegin(&L, false, 1);
+        if (mixed) launch_drvt_mixed(a, s);
+        else launch_drvt_fp64(a, s);
+        kend();
+    }
+    rec_ev(5);
+    signal(S.xev[2], s);  // final h; the derivative pass is done with the gather lists
+    await(s2, S.xev[2]);
+    tree_update_maxH(S.tree, P.h, s2);
+    // gather-list bits and construction counter, before the symmetric lists reuse L
+    launch_stash(StashArgs{{L.err.get(), nullptr, nullptr, nullptr}, {2, 0, 0, 0}, stash + kStGather}, s2);
+    enqueue_lists(S.tree, S.tp, P, 1, L, 1.0, s2);
+    signal(S.xev[3], s2);  // symmetric lists
+    await(s, S.xev[1]);
+    {
+        const Tree &t = S.tree;
+        GravArgs a{G, gs, P.x, P.y, P.z, nullptr, P.h, P.x, P.y, P.z, P.m,
+                   ListRef{Lg.idx.get(), nullptr, Lg.cnt.get(), Lg.cap},
+                   t.comx.get(), t.comy.get(), t.comz.get(), t.mass.get(),
+                   ListRef{Lg.sidx.get(), nullptr, Lg.scnt.get(), Lg.scap},
+                   P.gx, P.gy, P.gz, P.pot};
+        a.sched = sched_for(8);
+        a.gcount = gcount;
+        kbegin(&Lg, true, 3);
+        if (mixed) launch_grav_mixed(a, s);
+        else launch_grav_fp64(a, s);
+        kend();
+    }
+    rec_ev(6);
+    {
+        HydroRec *rec = S.hrec.ensure(n);
+        launch_pack_hydro(n, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.m, P.rho, P.cs, P.bal, P.gs, P.p, P.h, rec, s);
+        await(s, S.xev[3]);
+        HydroArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal,
+                    P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal, P.m,
+                    lref(), P.ax, P.ay, P.az, P.du, P.dti, rec};
+        a.sched = sched_for(4);
+        a.gcount = gcount;
+        kbegin(&L, false, 2);
+        if (mixed) launch_hydro_mixed(a, s);
+        else launch_hydro_fp64(a, s);
+        kend();
+    }
+    rec_ev(7);
+    launch_stash(StashArgs{{L.err.get(), Lg.err.get(), S.tree.info.get(), nullptr}, {2, 2, kInfoWords, 0},
+                           stash + kStSph},
+                 s);
+    S.direct_sig = step_sig(c, S);
+}
+
+// After the stream has drained: reads the stash of the evaluation just run,
+// grows every capacity it exceeded (and the tree's that it came close to) and
+// returns true when nothing overflowed.  On success the tree's counts are
+// applied on the host and the evaluation is counted.
+bool forces_check(SimState &S) {
+    const int32_t *h = S.stash.p;
+    const int32_t *info = h + kStInfo;
+    tree_grow_caps(S.tree, info);
+    if (info[kInfoErr]) return false;  // no groups were formed: the lists hold nothing to learn from
+    const bool ok_s = lists_grow_bits(S.lsph, h[kStGather] | h[kStSph]);
+    const bool ok_g = lists_grow_bits(S.lgrav, h[kStGrav]);
+    if (!(ok_s && ok_g)) return false;
+    tree_apply_info(S.tree, info);
+    S.gather_builds += h[kStGather + 1];
+    ++S.force_evals;
+    return true;
+}
+
+// kernel statistics of a successful evaluation (statistics on)
+void forces_stats(sphgpu_ctx *c, SimState &S) {
+    if (!S.stats) return;
+    hipStream_t s = c->stream;
+    hipLaunchKernelGGL(k_add_ll, dim3(1), dim3(64), 0, s, (int64_t) 4 * kStatKinds, S.kcount_try.get(),
+                       S.kcount.get());
+    SPH_LAUNCH_CHECK();
+    for (int k = 0; k < S.kslot; ++k) {
+        float ms = 0;
+        SPH_HIP(hipEventElapsedTime(&ms, S.kev[k][0], S.kev[k][1]));
+        S.kms[S.kkind[k]] += ms;
+        S.klaunch[S.kkind[k]] += 1;
+    }
+}
+
+// A force evaluation run to completion: enqueued, waited for once, and run
+// again with grown capacities while something overflows.  `restore_h`: the
+// state is that of a failed evaluation (see forces_enqueue).
+void sim_forces(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h = false) {
     for (int attempt = 0;; ++attempt) {
         if (attempt >= 12) throw Error(-2, "interaction list capacity could not be grown enough");
-        if (attempt > 0) SPH_HIP(hipMemcpyAsync(P.h, hsave, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-        int kslot = 0;
-        if (S.stats) {
-            S.kcount_try.ensure(4 * kStatKinds);
-            hipLaunchKernelGGL(k_fill_ll, dim3(1), dim3(64), 0, s, (int64_t) 4 * kStatKinds, S.kcount_try.get());
-            SPH_LAUNCH_CHECK();
-        }
-        auto kbegin = [&](const Lists *Lp, bool spj, int kind) {
-            if (!S.stats) return;
-            S.kkind[kslot] = kind;
-            hipLaunchKernelGGL(k_list_counts, dim3(nblk(G, 256)), dim3(256), 0, s, G, gs, Lp->cnt.get(),
-                               spj ? Lp->scnt.get() : nullptr, S.kcount_try.get() + 4 * kind);
-            SPH_LAUNCH_CHECK();
-            SPH_HIP(hipEventRecord(S.kev[kslot][0], s));
-        };
-        auto kend = [&]() {
-            if (!S.stats) return;
-            SPH_HIP(hipEventRecord(S.kev[kslot][1], s));
-            ++kslot;
-        };
-        lists_reset(L, s);
-        lists_reset(Lg, s);
-        enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s);
-        enqueue_gather_compact(S.tree, P, L, kGatherMargin, s, nullptr, kpos);
-        signal(S.xev[0], s);  // tree, h restored, counters cleared, first gather lists
-        await(s2, S.xev[0]);
-        enqueue_lists(S.tree, S.tp, P, 2, Lg, 1.0, s2);
-        signal(S.xev[1], s2);  // gravity lists
-        // 4 density passes, each over lists that cover the current h
-        // (src/main.cpp:138-140).  One gather list, built with a margin on h,
-        // serves every pass (and the derivative pass) while each group's
-        // search box still fits inside the box it was built for.
-        for (int loop = 0; loop <= 3; ++loop) {
-            if (loop > 0) {
-                enqueue_gather_check(S.tree, P, L, s);
-                enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s, L.flag.get());
-                enqueue_gather_compact(S.tree, P, L, kGatherMargin, s, L.flag.get(), kpos);
-            }
-            DensArgs a{G, gs, P.x, P.y, P.z, P.m, P.h, P.m, P.x, P.y, P.z, lref(), P.rho, P.h, P.ngb, kpos};
-            a.sched = sched_for(1);
-            // the density of the last pass feeds the EoS; the precision form
-            // keeps it FP64 unless asked otherwise (the earlier passes only steer h)
-            const bool d64 = !mixed || (loop == 3 && S.prm.eos_density_fp64);
-            kbegin(&L, false, (mixed && d64) ? 4 : 0);
-            if (d64) launch_dens_fp64(a, s);
-            else launch_dens_mixed(a, s);
-            kend();
-        }
-        rec_ev(3);
-        launch_eos_pressure(n, P.mat, P.rho, P.u, c->d_mats.get(), SPHGPU_MAX_MATERIALS, P.p, P.cs, s);
-        rec_ev(4);
-        enqueue_gather_check(S.tree, P, L, s);
-        enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s, L.flag.get());
-        enqueue_gather_compact(S.tree, P, L, kGatherMargin, s, L.flag.get(), kpos);
-        {
-            DrvtArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.cs, P.m, P.x, P.y, P.z, P.vx, P.vy, P.vz,
-                       lref(), P.divv, P.rotx, P.roty, P.rotz, P.gs, P.bal, kpos, kvel};
-            a.sched = sched_for(2);
-            kbegin(&L, false, 1);
-            if (mixed) launch_drvt_mixed(a, s);
-            else launch_drvt_fp64(a, s);
-            kend();
-        }
-        rec_ev(5);
-        signal(S.xev[2], s);  // final h; the derivative pass is done with the gather lists
-        await(s2, S.xev[2]);
-        tree_update_maxH(S.tree, P.h, s2);
-        lists_fetch(L, s2, 1);  // gather-list bits and construction counter, before the symmetric lists reuse L
-        enqueue_lists(S.tree, S.tp, P, 1, L, 1.0, s2);
-        signal(S.xev[3], s2);  // symmetric lists
-        await(s, S.xev[1]);
-        {
-            const Tree &t = S.tree;
-            GravArgs a{G, gs, P.x, P.y, P.z, nullptr, P.h, P.x, P.y, P.z, P.m,
-                       ListRef{Lg.idx.get(), nullptr, Lg.cnt.get(), Lg.cap},
-                       t.comx.get(), t.comy.get(), t.comz.get(), t.mass.get(),
-                       ListRef{Lg.sidx.get(), nullptr, Lg.scnt.get(), Lg.scap},
-                       P.gx, P.gy, P.gz, P.pot};
-            a.sched = sched_for(8);
-            kbegin(&Lg, true, 3);
-            if (mixed) launch_grav_mixed(a, s);
-            else launch_grav_fp64(a, s);
-            kend();
-        }
-        rec_ev(6);
-        {
-            HydroRec *rec = S.hrec.ensure(n);
-            launch_pack_hydro(n, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.m, P.rho, P.cs, P.bal, P.gs, P.p, P.h, rec, s);
-            await(s, S.xev[3]);
-            HydroArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal,
-                        P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal, P.m,
-                        lref(), P.ax, P.ay, P.az, P.du, P.dti, rec};
-            a.sched = sched_for(4);
-            kbegin(&L, false, 2);
-            if (mixed) launch_hydro_mixed(a, s);
-            else launch_hydro_fp64(a, s);
-            kend();
-        }
-        rec_ev(7);
-        lists_fetch(L, s, 0);
-        lists_fetch(Lg, s, 0);
-        SPH_HIP(hipStreamSynchronize(s));  // the one wait of the evaluation
-        const bool ok_s = lists_grow(L), ok_g = lists_grow(Lg);
-        if (!(ok_s && ok_g)) continue;
-        S.gather_builds += L.hflag.p[3];
-        if (S.stats) {
-            hipLaunchKernelGGL(k_add_ll, dim3(1), dim3(64), 0, s, (int64_t) 4 * kStatKinds, S.kcount_try.get(),
-                               S.kcount.get());
-            SPH_LAUNCH_CHECK();
-            for (int k = 0; k < kslot; ++k) {
-                float ms = 0;
-                SPH_HIP(hipEventElapsedTime(&ms, S.kev[k][0], S.kev[k][1]));
-                S.kms[S.kkind[k]] += ms;
-                S.klaunch[S.kkind[k]] += 1;
-            }
-        }
-        break;
+        forces_enqueue(c, S, timed, restore_h || attempt > 0);
+        SPH_HIP(hipStreamSynchronize(c->stream));  // the one wait of the evaluation
+        if (forces_check(S)) break;
     }
+    forces_stats(c, S);
 }
 
 void sim_dt(sphgpu_ctx *c, SimState &S) {
     launch_dt_reduce(S.nsys, S.sys_start_h.data(), S.tree.sys_start.get(), S.n, S.P.dti, S.dt_part.get(),
-                     S.dt_sys.get(), c->stream);
+                     S.dt_sys.get(), step_guard(S), c->stream);
+}
+
+int sim_parity(const SimState &S) { return S.P.x == S.f[0][0].get() ? 0 : 1; }
+
+// the work of one step (src/main.cpp:125-173) on the context's streams
+void step_enqueue(sphgpu_ctx *c, SimState &S, bool timed) {
+    hipStream_t s = c->stream;
+    if (timed) SPH_HIP(hipEventRecord(S.ev[0], s));
+    launch_kick_drift(S.n, S.P, S.dt_sys.get(), s);
+    launch_time_advance(S.nsys, S.time_sys.get(), S.dt_sys.get(), s);
+    if (timed) SPH_HIP(hipEventRecord(S.ev[1], s));
+    forces_enqueue(c, S, timed, false);
+}
+
+void step_finish(sphgpu_ctx *c, SimState &S, bool timed) {
+    hipStream_t s = c->stream;
+    launch_finalize(S.n, S.P, S.dt_sys.get(), S.time_sys.get(), S.prm.ext_force_until, S.prm.damping, step_guard(S),
+                    s);
+    sim_dt(c, S);
+    if (timed) SPH_HIP(hipEventRecord(S.ev[8], s));
+}
+
+// Records one step of the current parity into a graph.  The capture runs no
+// work; the host-side exchange of the particle buffer sets that the sort makes
+// is undone afterwards (the launch of the graph makes it).  False when the
+// step would have allocated (its buffers are then not those of the graph).
+bool capture_step(sphgpu_ctx *c, SimState &S, StepGraph &g, const std::vector<int64_t> &sig) {
+    hipStream_t s = c->stream;
+    const PArrays P0 = S.P, A0 = S.Palt;
+    const uint64_t ep = alloc_epoch();
+    hipGraph_t graph = nullptr;
+    SPH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    try {
+        step_enqueue(c, S, false);
+        step_finish(c, S, false);
+    } catch (...) {
+        hipGraph_t tmp = nullptr;
+        (void) hipStreamEndCapture(s, &tmp);
+        if (tmp) (void) hipGraphDestroy(tmp);
+        S.P = P0;
+        S.Palt = A0;
+        throw;
+    }
+    SPH_HIP(hipStreamEndCapture(s, &graph));
+    S.P = P0;
+    S.Palt = A0;
+    if (alloc_epoch() != ep) {
+        (void) hipGraphDestroy(graph);
+        return false;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void) hipGraphDestroy(graph);
+    SPH_HIP(e);
+    g.reset();
+    g.exec = exec;
+    g.sig = sig;
+    ++S.graph_captures;
+    return true;
+}
+
+// One step as a replay of its recorded graph, when the simulation records its
+// steps and the graph of this parity is (or can now be) recorded with the
+// current signature; false when the step must run directly.  A recorded step
+// is captured only after a direct step has run with the same signature, so
+// that every buffer it uses is allocated.  A replay whose evaluation overflowed
+// left the state as the evaluation found it (StepGuard); the evaluation is
+// then run directly with grown capacities, and the step finished.
+bool step_graph(sphgpu_ctx *c, SimState &S) {
+    if (!S.prm.graph || S.stats) return false;
+    const std::vector<int64_t> sig = step_sig(c, S);
+    StepGraph &g = S.graphs[sim_parity(S)];
+    if (g.exec && g.sig != sig) g.reset();
+    if (!g.exec) {
+        if (sig != S.direct_sig) return false;
+        if (!capture_step(c, S, g, sig)) return false;
+    }
+    hipStream_t s = c->stream;
+    SPH_HIP(hipGraphLaunch(g.exec, s));
+    std::swap(S.P, S.Palt);  // the recorded sort exchanged the buffer sets
+    SPH_HIP(hipStreamSynchronize(s));
+    ++S.graph_steps;
+    if (!forces_check(S)) {
+        sim_forces(c, S, false, true);
+        step_finish(c, S, false);
+    }
+    return true;
 }
 
 SimState &sim_of(sphgpu_ctx *c) {
@@ -1077,7 +1291,9 @@ void sphgpu_sim_default_params(sphgpu_sim_params *p) {
     p->list_cap_spj = 8192;
     p->list_cap_front = 4096;
     p->list_front_lds = kFrontLds;
+    p->walk_waves = 0;
     p->persistent = 0;
+    p->graph = 0;
 }
 
 int32_t sphgpu_sim_init(sphgpu_ctx *c, const sphgpu_sim_params *prm, int64_t n, const double *pos, const double *vel,
@@ -1137,6 +1353,7 @@ int32_t sphgpu_sim_init(sphgpu_ctx *c, const sphgpu_sim_params *prm, int64_t n, 
         for (Lists *L : {&S->lsph, &S->lgrav}) {
             L->fcap = prm->list_cap_front > 0 ? prm->list_cap_front : 4096;
             L->flds = (prm->list_front_lds > 0 && prm->list_front_lds < kFrontLds) ? prm->list_front_lds : kFrontLds;
+            L->waves = prm->walk_waves > 0 ? prm->walk_waves : 0;
         }
         sim_bind(*S);
         hipStream_t s = c->stream;
@@ -1166,6 +1383,8 @@ int32_t sphgpu_sim_init(sphgpu_ctx *c, const sphgpu_sim_params *prm, int64_t n, 
         SPH_HIP(hipMemsetAsync(S->time_sys.get(), 0, sizeof(double) * nsys, s));
         const int nchunk = (int) std::max<int64_t>(1, (maxsys + 4095) / 4096);
         S->dt_part.ensure((size_t) nsys * nchunk);
+        S->stash.flags = hipHostMallocCoherent;
+        S->stash.ensure(kStWords);
         for (auto &e : S->ev) SPH_HIP(hipEventCreate(&e));
         for (auto &e : S->xev) SPH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         // src/main.cpp:69-104: initialize, forces, dt, external force
@@ -1185,18 +1404,18 @@ int32_t sphgpu_sim_step(sphgpu_ctx *c, int32_t nsteps) {
         SimState &S = sim_of(c);
         if (nsteps < 0) throw Error(SPHGPU_ERR_ARG, "negative step count");
         hipStream_t s = c->stream;
+        bool direct = false;
         for (int k = 0; k < nsteps; ++k) {
-            SPH_HIP(hipEventRecord(S.ev[0], s));
-            launch_kick_drift(S.n, S.P, S.dt_sys.get(), s);
-            launch_time_advance(S.nsys, S.time_sys.get(), S.dt_sys.get(), s);
-            SPH_HIP(hipEventRecord(S.ev[1], s));
-            sim_forces(c, S, true);
-            launch_finalize(S.n, S.P, S.dt_sys.get(), S.time_sys.get(), S.prm.ext_force_until, S.prm.damping, s);
-            sim_dt(c, S);
-            SPH_HIP(hipEventRecord(S.ev[8], s));
+            if (step_graph(c, S)) continue;
+            step_enqueue(c, S, true);
+            SPH_HIP(hipStreamSynchronize(s));  // the one wait of the evaluation
+            if (forces_check(S)) forces_stats(c, S);
+            else sim_forces(c, S, true, true);
+            step_finish(c, S, true);
+            direct = true;
         }
         SPH_HIP(hipStreamSynchronize(s));
-        if (nsteps > 0)
+        if (direct)
             for (int k = 0; k < kStages; ++k) {
                 float ms = 0;
                 SPH_HIP(hipEventElapsedTime(&ms, S.ev[k], S.ev[k + 1]));
@@ -1292,6 +1511,33 @@ int32_t sphgpu_sim_list_builds(sphgpu_ctx *c, int64_t *gather_builds, int64_t *f
     });
 }
 
+int32_t sphgpu_sim_set_capacities(sphgpu_ctx *c, int32_t list_cap_sph, int32_t list_cap_epj,
+                                  int32_t list_cap_spj) {
+    return guard([&] {
+        check_ctx(c);
+        SimState &S = sim_of(c);
+        if (list_cap_sph < 1 || list_cap_epj < 1 || list_cap_spj < 1)
+            throw Error(SPHGPU_ERR_ARG, "list capacities must be >= 1");
+        SPH_HIP(hipStreamSynchronize(c->stream));
+        const bool shrink = list_cap_sph <= S.lsph.cap && list_cap_epj <= S.lgrav.cap && list_cap_spj <= S.lgrav.scap;
+        S.lsph.cap = list_cap_sph;
+        S.lgrav.cap = list_cap_epj;
+        S.lgrav.scap = list_cap_spj;
+        // smaller capacities fit the buffers allocated: a step may be recorded at once
+        if (shrink) S.direct_sig = step_sig(c, S);
+    });
+}
+
+int32_t sphgpu_sim_graph_stats(sphgpu_ctx *c, int64_t *replayed_steps, int64_t *captures) {
+    return guard([&] {
+        check_ctx(c);
+        SimState &S = sim_of(c);
+        if (!replayed_steps || !captures) throw Error(SPHGPU_ERR_ARG, "null output");
+        *replayed_steps = S.graph_steps;
+        *captures = S.graph_captures;
+    });
+}
+
 int32_t sphgpu_sim_set_stats(sphgpu_ctx *c, int32_t on) {
     return guard([&] {
         check_ctx(c);
@@ -1341,6 +1587,7 @@ int32_t sphgpu_sim_lists(sphgpu_ctx *c, int32_t kind, int64_t *n_groups, int64_t
         // the tree is rebuilt in place: a re-sort would permute only the fields
         // the step carries and leave the others in the alternate buffers.
         tree_build_only(S.tree, S.tp, S.n, S.P, s);
+        tree_sync_counts(S.tree, s);
         Lists &L = (kind == 2) ? S.lgrav : S.lsph;
         if (kind == 2) build_lists_grav(S.tree, S.tp, S.P, L, s);
         else if (kind == 3) {

@@ -62,9 +62,9 @@ __global__ void k_ext_force(int64_t n, PArrays P, const double *time_sys, double
 }
 
 __global__ void k_finalize(int64_t n, PArrays P, const double *dt_sys, const double *time_sys, double until,
-                           double damping) {
+                           double damping, StepGuard guard) {
     const int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    if (p >= n || guard.failed()) return;
     ext_force(P, p, time_sys, until);
     const double ht = 0.5 * dt_sys[P.sys[p]];
     double vx = P.vhx[p] + ht * (P.ax[p] + P.gx[p]);
@@ -95,7 +95,8 @@ __global__ void k_dt_partial(const int32_t *sys_start, int nchunk, const double 
     if (threadIdx.x == 0) part[(int64_t) s * nchunk + c] = sh[0];
 }
 
-__global__ void k_dt_final(int nchunk, const double *part, double *dt_sys) {
+__global__ void k_dt_final(int nchunk, const double *part, double *dt_sys, StepGuard guard) {
+    if (guard.failed()) return;
     const int s = blockIdx.x;
     double v = 1.0e+30;
     for (int c = threadIdx.x; c < nchunk; c += blockDim.x) v = dmin(v, part[(int64_t) s * nchunk + c]);
@@ -135,21 +136,22 @@ void launch_ext_force(int64_t n, PArrays P, const double *time_sys, double until
 }
 
 void launch_finalize(int64_t n, PArrays P, const double *dt_sys, const double *time_sys, double until,
-                     double damping, hipStream_t s) {
+                     double damping, StepGuard guard, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_finalize, dim3(nblk(n, 256)), dim3(256), 0, s, n, P, dt_sys, time_sys, until, damping);
+    hipLaunchKernelGGL(k_finalize, dim3(nblk(n, 256)), dim3(256), 0, s, n, P, dt_sys, time_sys, until, damping,
+                       guard);
     SPH_LAUNCH_CHECK();
 }
 
 void launch_dt_reduce(int nsys, const int32_t *sys_start_host, const int32_t *sys_start, int64_t n,
-                      const double *dti, double *part, double *dt_sys, hipStream_t s) {
+                      const double *dti, double *part, double *dt_sys, StepGuard guard, hipStream_t s) {
     (void) n;
     int maxn = 0;
     for (int q = 0; q < nsys; ++q) maxn = std::max(maxn, sys_start_host[q + 1] - sys_start_host[q]);
     const int nchunk = std::max(1, (maxn + kChunk - 1) / kChunk);
     hipLaunchKernelGGL(k_dt_partial, dim3(nchunk, nsys), dim3(256), 0, s, sys_start, nchunk, dti, part);
     SPH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_dt_final, dim3(nsys), dim3(256), 0, s, nchunk, part, dt_sys);
+    hipLaunchKernelGGL(k_dt_final, dim3(nsys), dim3(256), 0, s, nchunk, part, dt_sys, guard);
     SPH_LAUNCH_CHECK();
 }
 

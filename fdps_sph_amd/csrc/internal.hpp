@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
@@ -29,6 +30,11 @@ struct Error : public std::runtime_error {
 #define SPH_LAUNCH_CHECK() SPH_HIP(hipGetLastError())
 
 // ------------------------------------------------------------------ device buffer (grow-only)
+// Counts every allocation of the buffers below: a recorded step (HIP graph)
+// holds buffer addresses and is replayed only while this is unchanged.
+inline std::atomic<uint64_t> g_alloc_epoch{0};
+inline uint64_t alloc_epoch() { return g_alloc_epoch.load(); }
+
 template <class T>
 struct DevBuf {
     T *p = nullptr;
@@ -52,6 +58,7 @@ struct DevBuf {
         release();
         SPH_HIP(hipMalloc(&p, c * sizeof(T)));
         n = c;
+        g_alloc_epoch.fetch_add(1);
         return p;
     }
     T *get() const { return p; }
@@ -61,6 +68,9 @@ template <class T>
 struct PinnedBuf {
     T *p = nullptr;
     size_t n = 0;
+    // hipHostMallocCoherent for words that kernels write and the host reads
+    // after the stream drains
+    unsigned flags = hipHostMallocDefault;
     PinnedBuf() = default;
     PinnedBuf(const PinnedBuf &) = delete;
     PinnedBuf &operator=(const PinnedBuf &) = delete;
@@ -72,8 +82,9 @@ struct PinnedBuf {
         if (p) (void) hipHostFree(p);
         p = nullptr;
         size_t c = count ? count : 1;
-        SPH_HIP(hipHostMalloc(&p, c * sizeof(T), hipHostMallocDefault));
+        SPH_HIP(hipHostMalloc(&p, c * sizeof(T), flags));
         n = c;
+        g_alloc_epoch.fetch_add(1);
         return p;
     }
 };
@@ -125,6 +136,9 @@ struct DensArgs {
     // persistent scheduling: 8 per-XCD group counters, cleared before the launch
     // (null: one workgroup per group)
     int32_t *sched = nullptr;
+    // if set: the group count is read here (a tree build's device count, at
+    // most ngroups, which then only sizes the launch)
+    const int32_t *gcount = nullptr;
 };
 
 struct DrvtArgs {
@@ -140,6 +154,9 @@ struct DrvtArgs {
     // persistent scheduling: 8 per-XCD group counters, cleared before the launch
     // (null: one workgroup per group)
     int32_t *sched = nullptr;
+    // if set: the group count is read here (a tree build's device count, at
+    // most ngroups, which then only sizes the launch)
+    const int32_t *gcount = nullptr;
 };
 
 // The j side of the hydro pass as one 128-byte record per particle (one
@@ -163,6 +180,9 @@ struct HydroArgs {
     // persistent scheduling: 8 per-XCD group counters, cleared before the launch
     // (null: one workgroup per group)
     int32_t *sched = nullptr;
+    // if set: the group count is read here (a tree build's device count, at
+    // most ngroups, which then only sizes the launch)
+    const int32_t *gcount = nullptr;
 };
 
 struct GravArgs {
@@ -179,6 +199,9 @@ struct GravArgs {
     // persistent scheduling: 8 per-XCD group counters, cleared before the launch
     // (null: one workgroup per group)
     int32_t *sched = nullptr;
+    // if set: the group count is read here (a tree build's device count, at
+    // most ngroups, which then only sizes the launch)
+    const int32_t *gcount = nullptr;
 };
 
 // Pack the j records of the density and derivative passes for particles
@@ -234,24 +257,44 @@ struct TreeParams {
     double theta;
 };
 
+// Device-side counts of a tree build (Tree::info, int32 words): the node
+// count of each level and the levels' offsets in the flattened node array
+// (every count 0 when some level exceeds its capacity), the group count (0
+// when the nodes or the groups exceed their capacities), error bits (1 nodes,
+// 2 groups beyond capacity) and the counts as found.
+constexpr int kInfoCnt = 0;
+constexpr int kInfoOff = kInfoCnt + kMaxLevels;  // kMaxLevels + 1 words
+constexpr int kInfoG = kInfoOff + kMaxLevels + 1;
+constexpr int kInfoErr = kInfoG + 1;
+constexpr int kInfoRawCnt = kInfoErr + 1;
+constexpr int kInfoRawG = kInfoRawCnt + kMaxLevels;
+constexpr int kInfoWords = kInfoRawG + 1;
+
 struct Tree {
     // per-system
     DevBuf<int32_t> sys_start;  // nsys+1 (host-known, fixed)
     std::vector<int32_t> sys_start_h;  // the same offsets on the host
-    PinnedBuf<int32_t> hcnt;    // level / group counts read back per build
     DevBuf<double> sys_lo, sys_scale;  // 3*nsys, nsys
     DevBuf<double> part;           // bbox partials
     DevBuf<int32_t> sys_root;      // nsys
     // keys and sort
     DevBuf<uint64_t> key, key_alt;
-    DevBuf<int32_t> perm, iota;
-    DevBuf<uint8_t> sort_tmp, sel_tmp;
-    size_t sort_tmp_bytes = 0, sel_tmp_bytes = 0;
+    DevBuf<int32_t> perm, vtmp;    // the sorted permutation; the sort's other value buffer
+    DevBuf<int32_t> rs_bh, rs_tot; // the sort's per-tile digit counts and digit totals
     DevBuf<int64_t> cnt;
-    DevBuf<uint8_t> flags;
-    DevBuf<int32_t> sel_out, counts;
+    DevBuf<int32_t> sel_out;
     DevBuf<int32_t> sel_blk;  // per-block node-start counts of every level, then their offsets
-    // nodes (levels flattened, level 0 = leaves)
+    // counts of the last build (see kInfo*) and their host copy
+    DevBuf<int32_t> info;
+    PinnedBuf<int32_t> hinfo;
+    // capacities the launches and buffers are sized by; chosen from the counts
+    // of the first build (which waits for them) and grown when counts come near
+    bool caps_known = false;
+    int32_t cap_lv[kMaxLevels] = {0};
+    int64_t cap_nodes = 0;
+    int64_t gcap = 0;
+    // nodes (levels flattened, level 0 = leaves); the counts on the host are
+    // those of the last tree_apply_info
     int nlevels = 0;
     int64_t level_off[kMaxLevels + 1] = {0};
     int64_t nnodes = 0;
@@ -260,12 +303,29 @@ struct Tree {
     // i-groups
     int64_t ngroups = 0;
     DevBuf<int32_t> gstart;
+    DevBuf<int32_t> gcnt;  // groups of each packing-level node, then their offsets
     int64_t n = 0;  // particles of the last build
 };
+
+// Stable radix sort of n (key, value) pairs on the low `keybits` key bits:
+// keys from k0, values the input indices; k1 and v0, v1 are the other
+// buffers, bh holds radix_sort_tiles(n) * 256 counts and tot 256.  The sorted
+// keys and values end in *kres and *vres (one of k0/k1, v0/v1).
+int radix_sort_tiles(int64_t n);
+void radix_sort_pairs(int64_t n, int keybits, uint64_t *k0, uint64_t *k1, int32_t *v0, int32_t *v1, int32_t *bh,
+                      int32_t *tot, uint64_t **kres, int32_t **vres, hipStream_t s);
 
 void tree_sort_and_build(Tree &t, const TreeParams &tp, int64_t n, PArrays &P, PArrays &Palt, hipStream_t s);
 void tree_build_only(Tree &t, const TreeParams &tp, int64_t n, const PArrays &P, hipStream_t s);
 void tree_update_maxH(Tree &t, const double *h, hipStream_t s);
+// Host copies of the counts from a host copy of Tree::info.
+void tree_apply_info(Tree &t, const int32_t *info_host);
+// Grows the capacities that the counts of info_host exceed or come close to;
+// true if any grew.
+bool tree_grow_caps(Tree &t, const int32_t *info_host);
+// Waits for the stream, reads the counts and applies them (an error if the
+// last build exceeded its capacities).
+void tree_sync_counts(Tree &t, hipStream_t s);
 
 // BFS frontier entries a list-walk wave keeps in LDS (two buffers of this many)
 constexpr int kFrontLds = 512;
@@ -276,6 +336,7 @@ struct Lists {
     DevBuf<int32_t> frontier;
     int64_t fcap = 0;        // per-workgroup BFS frontier capacity in global memory
     int flds = kFrontLds;    // frontier entries kept in LDS before the global part
+    int waves = 0;           // persistent waves of a walk (0: the default, 8192)
     DevBuf<int32_t> err;       // overflow bits, construction counter, per-XCD group counters
     DevBuf<int32_t> flag;      // gather-list validity: 1 when some group outgrew its list
     PinnedBuf<int32_t> hflag;  // host copy of the overflow bits and the construction counter
@@ -305,6 +366,7 @@ void enqueue_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int ki
                    hipStream_t s, const int32_t *need = nullptr);
 void lists_fetch(Lists &L, hipStream_t s, int slot = 0);
 bool lists_grow(Lists &L);
+bool lists_grow_bits(Lists &L, int32_t err);
 // Sets L.flag (device) to 1 when some group's current search box (same tree,
 // positions unchanged, h possibly changed) has outgrown its gather list, or,
 // for compacted lists, when some 2.5 h_i of the group exceeds its R_g.
@@ -319,17 +381,29 @@ void enqueue_gather_compact(const Tree &t, const PArrays &P, Lists &L, double hf
 bool build_lists_sph(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, hipStream_t s,
                      double hfac = 1.0);
 bool build_lists_grav(const Tree &t, const TreeParams &tp, const PArrays &P, Lists &L, hipStream_t s);
-int64_t lists_max_count(const Lists &L, int64_t ngroups, bool spj, hipStream_t s);
 
 // ------------------------------------------------------------------ integrator -- integrate.hip
+// Error words a step's last kernels read before they change the state (the
+// final kick, the time step): any nonzero word means that the force
+// evaluation ran out of a capacity and is to be run again, so they leave the
+// state as it was.  Null words are not read.
+struct StepGuard {
+    const int32_t *w[3] = {nullptr, nullptr, nullptr};
+    __device__ __forceinline__ bool failed() const {
+        int32_t e = 0;
+        for (int k = 0; k < 3; ++k)
+            if (w[k]) e |= *w[k];
+        return e != 0;
+    }
+};
 void launch_init_h(int64_t n, PArrays P, hipStream_t s);
 void launch_kick_drift(int64_t n, PArrays P, const double *dt_sys, hipStream_t s);
 void launch_time_advance(int nsys, double *time_sys, const double *dt_sys, hipStream_t s);
 void launch_finalize(int64_t n, PArrays P, const double *dt_sys, const double *time_sys, double until,
-                     double damping, hipStream_t s);
+                     double damping, StepGuard guard, hipStream_t s);
 void launch_ext_force(int64_t n, PArrays P, const double *time_sys, double until, hipStream_t s);
 void launch_dt_reduce(int nsys, const int32_t *sys_start_host, const int32_t *sys_start, int64_t n,
-                      const double *dti, double *part, double *dt_sys, hipStream_t s);
+                      const double *dti, double *part, double *dt_sys, StepGuard guard, hipStream_t s);
 
 // ------------------------------------------------------------------ device helpers
 __device__ __forceinline__ int xcd_remap(int b, int n) {
@@ -339,6 +413,11 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
     const int x = b & 7, k = b >> 3;
     return x * q + (x < r ? x : r) + k;
 }
+
+// The group count of an interaction launch: a device count when the args
+// carry one (the grid is then sized by a capacity), else the launch's own.
+template <class A>
+__device__ __forceinline__ int group_count(const A &a) { return a.gcount ? *a.gcount : a.ngroups; }
 
 // Work distribution of the interaction kernels over the i-groups.  Without a
 // scheduler each workgroup takes one group through xcd_remap.  With one (8

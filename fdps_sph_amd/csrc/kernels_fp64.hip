@@ -199,10 +199,13 @@ __device__ __forceinline__ void dens_group(const DensArgs &a, const int g) {
         if (a.ngb_out) a.ngb_out[ip] = ngb;
     }
 }
-__global__ __launch_bounds__(kWave, 5) void k_dens(DensArgs a) { dens_group(a, xcd_remap((int) blockIdx.x, a.ngroups)); }
+__global__ __launch_bounds__(kWave, 5) void k_dens(DensArgs a) {
+    const int G = group_count(a);
+    if ((int) blockIdx.x < G) dens_group(a, xcd_remap((int) blockIdx.x, G));
+}
 // persistent form: a fixed grid taking groups from the per-XCD counters
 __global__ __launch_bounds__(kWave, 5) void k_dens_persistent(DensArgs a) {
-    GroupIter it(a.sched, a.ngroups);
+    GroupIter it(a.sched, group_count(a));
     for (int g = it.first(); g >= 0; g = it.next()) {
         dens_group(a, g);
         __syncthreads();  // the next group reuses the LDS tiles
@@ -318,10 +321,13 @@ __device__ __forceinline__ void drvt_group(const DrvtArgs &a, const int g) {
             a.bal[ip] = fabs(divv) / (fabs(divv) + sqrt(rx * rx + ry * ry + rz * rz) + 1.0e-4 * a.csi[ip] / hi);
     }
 }
-__global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) { drvt_group(a, xcd_remap((int) blockIdx.x, a.ngroups)); }
+__global__ __launch_bounds__(kWave) void k_drvt(DrvtArgs a) {
+    const int G = group_count(a);
+    if ((int) blockIdx.x < G) drvt_group(a, xcd_remap((int) blockIdx.x, G));
+}
 // persistent form: a fixed grid taking groups from the per-XCD counters
 __global__ __launch_bounds__(kWave) void k_drvt_persistent(DrvtArgs a) {
-    GroupIter it(a.sched, a.ngroups);
+    GroupIter it(a.sched, group_count(a));
     for (int g = it.first(); g >= 0; g = it.next()) {
         drvt_group(a, g);
         __syncthreads();  // the next group reuses the LDS tiles
@@ -524,10 +530,13 @@ __device__ __forceinline__ void hydro_group(const HydroArgs &a, const int g) {
         a.dt[ip] = kCFL * 2.0 * hi / vmax;
     }
 }
-__global__ __launch_bounds__(kWave, 4) void k_hydro(HydroArgs a) { hydro_group(a, xcd_remap((int) blockIdx.x, a.ngroups)); }
+__global__ __launch_bounds__(kWave, 4) void k_hydro(HydroArgs a) {
+    const int G = group_count(a);
+    if ((int) blockIdx.x < G) hydro_group(a, xcd_remap((int) blockIdx.x, G));
+}
 // persistent form: a fixed grid taking groups from the per-XCD counters
 __global__ __launch_bounds__(kWave, 4) void k_hydro_persistent(HydroArgs a) {
-    GroupIter it(a.sched, a.ngroups);
+    GroupIter it(a.sched, group_count(a));
     for (int g = it.first(); g >= 0; g = it.next()) {
         hydro_group(a, g);
         __syncthreads();  // the next group reuses the LDS tiles
@@ -596,10 +605,13 @@ __device__ __forceinline__ void grav_group(const GravArgs &a, const int g) {
         a.pot[ip] = pot;
     }
 }
-__global__ __launch_bounds__(kWave) void k_grav(GravArgs a) { grav_group(a, xcd_remap((int) blockIdx.x, a.ngroups)); }
+__global__ __launch_bounds__(kWave) void k_grav(GravArgs a) {
+    const int G = group_count(a);
+    if ((int) blockIdx.x < G) grav_group(a, xcd_remap((int) blockIdx.x, G));
+}
 // persistent form: a fixed grid taking groups from the per-XCD counters
 __global__ __launch_bounds__(kWave) void k_grav_persistent(GravArgs a) {
-    GroupIter it(a.sched, a.ngroups);
+    GroupIter it(a.sched, group_count(a));
     for (int g = it.first(); g >= 0; g = it.next()) {
         grav_group(a, g);
         __syncthreads();  // the next group reuses the LDS tiles

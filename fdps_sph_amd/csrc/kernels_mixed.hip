@@ -164,10 +164,13 @@ __device__ __forceinline__ void dens_mixed_group(const DensArgs &a, const int g)
         if (a.ngb_out) a.ngb_out[ip] = ngb;
     }
 }
-__global__ __launch_bounds__(kWave, 5) void k_dens_mixed(DensArgs a) { dens_mixed_group(a, xcd_remap((int) blockIdx.x, a.ngroups)); }
+__global__ __launch_bounds__(kWave, 5) void k_dens_mixed(DensArgs a) {
+    const int G = group_count(a);
+    if ((int) blockIdx.x < G) dens_mixed_group(a, xcd_remap((int) blockIdx.x, G));
+}
 // persistent form: a fixed grid taking groups from the per-XCD counters
 __global__ __launch_bounds__(kWave, 5) void k_dens_mixed_persistent(DensArgs a) {
-    GroupIter it(a.sched, a.ngroups);
+    GroupIter it(a.sched, group_count(a));
     for (int g = it.first(); g >= 0; g = it.next()) {
         dens_mixed_group(a, g);
         __syncthreads();  // the next group reuses the LDS tiles
@@ -267,10 +270,13 @@ __device__ __forceinline__ void drvt_mixed_group(const DrvtArgs &a, const int g)
         if (a.csi) a.bal[ip] = fabs(dv) / (fabs(dv) + sqrt(ox * ox + oy * oy + oz * oz) + 1.0e-4 * a.csi[ip] / hid);
     }
 }
-__global__ __launch_bounds__(kWave) void k_drvt_mixed(DrvtArgs a) { drvt_mixed_group(a, xcd_remap((int) blockIdx.x, a.ngroups)); }
+__global__ __launch_bounds__(kWave) void k_drvt_mixed(DrvtArgs a) {
+    const int G = group_count(a);
+    if ((int) blockIdx.x < G) drvt_mixed_group(a, xcd_remap((int) blockIdx.x, G));
+}
 // persistent form: a fixed grid taking groups from the per-XCD counters
 __global__ __launch_bounds__(kWave) void k_drvt_mixed_persistent(DrvtArgs a) {
-    GroupIter it(a.sched, a.ngroups);
+    GroupIter it(a.sched, group_count(a));
     for (int g = it.first(); g >= 0; g = it.next()) {
         drvt_mixed_group(a, g);
         __syncthreads();  // the next group reuses the LDS tiles
@@ -433,10 +439,13 @@ __device__ __forceinline__ void hydro_mixed_group(const HydroArgs &a, const int 
         a.dt[ip] = kCFL * 2.0 * hid / (double) vmax;
     }
 }
-__global__ __launch_bounds__(kWave, 4) void k_hydro_mixed(HydroArgs a) { hydro_mixed_group(a, xcd_remap((int) blockIdx.x, a.ngroups)); }
+__global__ __launch_bounds__(kWave, 4) void k_hydro_mixed(HydroArgs a) {
+    const int G = group_count(a);
+    if ((int) blockIdx.x < G) hydro_mixed_group(a, xcd_remap((int) blockIdx.x, G));
+}
 // persistent form: a fixed grid taking groups from the per-XCD counters
 __global__ __launch_bounds__(kWave, 4) void k_hydro_mixed_persistent(HydroArgs a) {
-    GroupIter it(a.sched, a.ngroups);
+    GroupIter it(a.sched, group_count(a));
     for (int g = it.first(); g >= 0; g = it.next()) {
         hydro_mixed_group(a, g);
         __syncthreads();  // the next group reuses the LDS tiles
@@ -508,10 +517,13 @@ __device__ __forceinline__ void grav_mixed_group(const GravArgs &a, const int g)
         a.pot[ip] = pot.value();
     }
 }
-__global__ __launch_bounds__(kWave) void k_grav_mixed(GravArgs a) { grav_mixed_group(a, xcd_remap((int) blockIdx.x, a.ngroups)); }
+__global__ __launch_bounds__(kWave) void k_grav_mixed(GravArgs a) {
+    const int G = group_count(a);
+    if ((int) blockIdx.x < G) grav_mixed_group(a, xcd_remap((int) blockIdx.x, G));
+}
 // persistent form: a fixed grid taking groups from the per-XCD counters
 __global__ __launch_bounds__(kWave) void k_grav_mixed_persistent(GravArgs a) {
-    GroupIter it(a.sched, a.ngroups);
+    GroupIter it(a.sched, group_count(a));
     for (int g = it.first(); g >= 0; g = it.next()) {
         grav_mixed_group(a, g);
         __syncthreads();  // the next group reuses the LDS tiles

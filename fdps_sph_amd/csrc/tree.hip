@@ -13,15 +13,26 @@
 // node, so a 64-lane wavefront holds ~57 compact i-particles on average
 // instead of the ~25 of one octree cell.  This file is compiled with
 // -ffp-contract=off.
-#include <hipcub/hipcub.hpp>
-
+//
+// Every count the build finds (nodes per level, groups) stays on the device
+// (Tree::info): the host sizes launches and buffers by capacities it chose
+// from earlier builds, and kernels past a count exit.  A build whose counts
+// exceed the capacities sets an error bit and leaves no groups, so nothing
+// downstream runs on it; the host reads the bits when it waits anyway, grows
+// the capacities and builds again.  No step of a build waits for the host,
+// so it can be recorded in a HIP graph.
 #include "internal.hpp"
 
 namespace sphgpu {
 namespace {
 
 constexpr int kChunk = 4096;       // bbox partial-reduction chunk
-constexpr int kListSlots = 8192;      // persistent list-builder workgroups (32 waves per CU)
+constexpr int kListSlots = 8192;      // most persistent list-builder workgroups (32 waves per CU)
+// default: 16 waves per CU.  A walk runs beside force kernels (second stream,
+// or the same queue of a recorded step); with all 8192 it took the slots the
+// density passes needed (1M step, recorded: FP64 7.33 -> 7.22 ms, mixed 5.35
+// -> 5.24 ms; 2048 waves: 7.40 / 5.56 ms)
+constexpr int kListWavesDefault = 4096;
 constexpr int64_t kFrontier0 = 4096;  // initial per-workgroup BFS frontier capacity (doubled on overflow)
 
 __device__ __forceinline__ uint64_t expand_bits(uint64_t v) {
@@ -96,7 +107,7 @@ __global__ void k_bbox_final(int nchunk, int bits, const double *part, double *l
 }
 
 __global__ void k_keys(int64_t n, const double *x, const double *y, const double *z, const int32_t *sys, int bits,
-                       const double *lo, const double *scale, uint64_t *key, int32_t *val) {
+                       const double *lo, const double *scale, uint64_t *key) {
     const int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int s = sys[p];
@@ -106,7 +117,6 @@ __global__ void k_keys(int64_t n, const double *x, const double *y, const double
     const uint64_t iy = quantise(y[p], lo[3 * s + 1], sc, qmax);
     const uint64_t iz = quantise(z[p], lo[3 * s + 2], sc, qmax);
     key[p] = ((uint64_t) s << (3 * bits)) | (expand_bits(ix) << 2) | (expand_bits(iy) << 1) | expand_bits(iz);
-    val[p] = (int32_t) p;
 }
 
 struct PermArgs {
@@ -216,8 +226,9 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_count(int64_t n, const int6
     }
 }
 
-// exclusive scan of the block counts of every level (one workgroup); totals to counts[0..L)
-__global__ __launch_bounds__(kSelThreads) void k_sel_scan(int nb, int L, int32_t *bcount, int32_t *counts) {
+// exclusive scan of the block counts of every level (one workgroup); the
+// level totals to info[kInfoRawCnt + l]
+__global__ __launch_bounds__(kSelThreads) void k_sel_scan(int nb, int L, int32_t *bcount, int32_t *info) {
     __shared__ int buf[kSelThreads];
     const int tid = threadIdx.x;
     for (int l = 0; l < L; ++l) {
@@ -237,7 +248,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_scan(int nb, int L, int32_t
             carry += buf[kSelThreads - 1];
             __syncthreads();
         }
-        if (tid == 0) counts[l] = carry;
+        if (tid == 0) info[kInfoRawCnt + l] = carry;
     }
 }
 
@@ -272,22 +283,43 @@ __global__ __launch_bounds__(kSelThreads) void k_sel_scatter(int64_t n, const in
     }
 }
 
-// node counts and offsets of every level, passed by value (level = blockIdx.y)
-struct LevelLayout {
-    int64_t cnt[kMaxLevels], off[kMaxLevels];
+// node capacity of each level, passed by value
+struct LevelCaps {
+    int32_t cap[kMaxLevels];
 };
 
-// node starts of every level into the flattened node array
-__global__ void k_copy_levels(int64_t n, LevelLayout lv, const int32_t *sel, int32_t *nstart) {
+// Level layout from the counts found: when every level fits its capacity, the
+// counts and their exclusive prefix (the flattened node array's level
+// offsets); otherwise every count 0 and error bit 1.
+__global__ void k_level_layout(int L, LevelCaps caps, int32_t *info) {
+    if (threadIdx.x != 0) return;
+    bool ok = true;
+    for (int l = 0; l < L; ++l) ok = ok && info[kInfoRawCnt + l] <= caps.cap[l];
+    int32_t off = 0;
+    for (int l = 0; l < L; ++l) {
+        const int32_t c = ok ? info[kInfoRawCnt + l] : 0;
+        info[kInfoOff + l] = off;
+        info[kInfoCnt + l] = c;
+        off += c;
+    }
+    info[kInfoOff + L] = off;
+    info[kInfoErr] = ok ? 0 : 1;
+    info[kInfoG] = 0;
+}
+
+// node starts of every level into the flattened node array (level = blockIdx.y)
+__global__ void k_copy_levels(int64_t n, const int32_t *info, const int32_t *sel, int32_t *nstart) {
     const int l = blockIdx.y;
-    for (int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; q < lv.cnt[l]; q += (int64_t) gridDim.x * blockDim.x)
-        nstart[lv.off[l] + q] = sel[(int64_t) l * n + q];
+    const int64_t cnt_l = info[kInfoCnt + l], off_l = info[kInfoOff + l];
+    for (int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; q < cnt_l; q += (int64_t) gridDim.x * blockDim.x)
+        nstart[off_l + q] = sel[(int64_t) l * n + q];
 }
 
 // node ranges and children of every level
-__global__ void k_node_ranges(int64_t n, LevelLayout lv, int32_t *nstart, int32_t *nend, int32_t *c0, int32_t *c1) {
+__global__ void k_node_ranges(int64_t n, const int32_t *info, int32_t *nstart, int32_t *nend, int32_t *c0,
+                              int32_t *c1) {
     const int l = blockIdx.y;
-    const int64_t cnt_l = lv.cnt[l], off_l = lv.off[l];
+    const int64_t cnt_l = info[kInfoCnt + l], off_l = info[kInfoOff + l];
     for (int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; q < cnt_l; q += (int64_t) gridDim.x * blockDim.x) {
         const int64_t id = off_l + q;
         const int32_t st = nstart[id];
@@ -297,10 +329,11 @@ __global__ void k_node_ranges(int64_t n, LevelLayout lv, int32_t *nstart, int32_
             c0[id] = -1;
             c1[id] = -1;
         } else {
-            const int64_t off_below = lv.off[l - 1];
+            const int64_t off_below = info[kInfoOff + l - 1];
             const int32_t *below = nstart + off_below;
-            c0[id] = (int32_t) (off_below + lb_i32(below, (int) lv.cnt[l - 1], st));
-            c1[id] = (int32_t) (off_below + lb_i32(below, (int) lv.cnt[l - 1], en));
+            const int cnt_below = info[kInfoCnt + l - 1];
+            c0[id] = (int32_t) (off_below + lb_i32(below, cnt_below, st));
+            c1[id] = (int32_t) (off_below + lb_i32(below, cnt_below, en));
         }
     }
 }
@@ -310,10 +343,10 @@ struct MomArgs {
     double *mass, *mx, *my, *mz, *comx, *comy, *comz, *lox, *loy, *loz, *hix, *hiy, *hiz, *maxH;
 };
 
-__global__ void k_moments_leaf(int64_t cnt_l, MomArgs a, const double *x, const double *y, const double *z,
+__global__ void k_moments_leaf(const int32_t *info, MomArgs a, const double *x, const double *y, const double *z,
                                const double *m, const double *h) {
     const int64_t id = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= cnt_l) return;
+    if (id >= info[kInfoCnt]) return;
     double M = 0, sx = 0, sy = 0, sz = 0, mh = 0;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int p = a.nstart[id]; p < a.nend[id]; ++p) {
@@ -334,10 +367,10 @@ __global__ void k_moments_leaf(int64_t cnt_l, MomArgs a, const double *x, const 
     a.maxH[id] = mh;
 }
 
-__global__ void k_moments_level(int64_t cnt_l, int64_t off_l, MomArgs a) {
+__global__ void k_moments_level(int l, const int32_t *info, MomArgs a) {
     const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= cnt_l) return;
-    const int64_t id = off_l + q;
+    if (q >= info[kInfoCnt + l]) return;
+    const int64_t id = info[kInfoOff + l] + q;
     double M = 0, sx = 0, sy = 0, sz = 0, mh = 0;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int c = a.c0[id]; c < a.c1[id]; ++c) {
@@ -357,60 +390,115 @@ __global__ void k_moments_level(int64_t cnt_l, int64_t off_l, MomArgs a) {
     a.maxH[id] = mh;
 }
 
-__global__ void k_maxH_leaf(int64_t cnt_l, const int32_t *nstart, const int32_t *nend, const double *h, double *maxH) {
+__global__ void k_maxH_leaf(const int32_t *info, const int32_t *nstart, const int32_t *nend, const double *h,
+                            double *maxH) {
     const int64_t id = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= cnt_l) return;
+    if (id >= info[kInfoCnt]) return;
     double mh = 0;
     for (int p = nstart[id]; p < nend[id]; ++p) mh = dmax(mh, kSupport * h[p]);
     maxH[id] = mh;
 }
 
-__global__ void k_maxH_level(int64_t cnt_l, int64_t off_l, const int32_t *c0, const int32_t *c1, double *maxH) {
+__global__ void k_maxH_level(int l, const int32_t *info, const int32_t *c0, const int32_t *c1, double *maxH) {
     const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= cnt_l) return;
-    const int64_t id = off_l + q;
+    if (q >= info[kInfoCnt + l]) return;
+    const int64_t id = info[kInfoOff + l] + q;
     double mh = 0;
     for (int c = c0[id]; c < c1[id]; ++c) mh = dmax(mh, maxH[c]);
     maxH[id] = mh;
 }
 
-__global__ void k_sys_root(int64_t cnt_top, int64_t off_top, const int32_t *nstart, const int32_t *sys,
-                           int32_t *sys_root) {
+__global__ void k_sys_root(int l, const int32_t *info, const int32_t *nstart, const int32_t *sys, int32_t *sys_root) {
     const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= cnt_top) return;
-    sys_root[sys[nstart[off_top + q]]] = (int32_t) (off_top + q);
+    if (q >= info[kInfoCnt + l]) return;
+    const int64_t id = info[kInfoOff + l] + q;
+    sys_root[sys[nstart[id]]] = (int32_t) id;
 }
 
 // i-groups: inside node q of the packing level, consecutive leaves are packed
-// greedily while a group stays within n_group particles; flags mark each
-// group's first particle
-__global__ void k_pack_groups(int64_t cnt_l, int64_t off_l, const int32_t *nstart, const int32_t *nend,
-                              const int32_t *c0, const int32_t *c1, int depth, int n_group, uint8_t *flags) {
-    const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= cnt_l) return;
-    // the node's leaves: down `depth` levels along the first and the last child
-    // (every node has at least one child); leaves are nodes 0 .. of level 0
-    int a = (int) (off_l + q), last = a;
+// greedily while a group stays within n_group particles.  The node's leaves
+// are found by descending `depth` levels along its first and its last child
+// (every node has at least one child; leaves are the nodes of level 0).
+template <bool WRITE>
+__device__ __forceinline__ int pack_node(const int32_t *nstart, const int32_t *nend, const int32_t *c0,
+                                         const int32_t *c1, int node, int depth, int n_group, int32_t *out) {
+    int a = node, last = node;
     for (int l = 0; l < depth; ++l) {
         a = c0[a];
         last = c1[last] - 1;
     }
     const int b = last + 1;
-    int run = 0;
+    int run = 0, ng = 0;
 #pragma unroll 8
     for (int c = a; c < b; ++c) {
         const int sz = nend[c] - nstart[c];
         if (c == a || run + sz > n_group) {
-            flags[nstart[c]] = 1;
+            if (WRITE) out[ng] = nstart[c];
+            ++ng;
             run = 0;
         }
         run += sz;
     }
+    return ng;
 }
 
-__global__ void k_fill_u8(int64_t n, uint8_t *a) {
-    const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) a[k] = 0;
+// groups of each node of level l
+__global__ void k_group_count(int l, const int32_t *info, const int32_t *nstart, const int32_t *nend,
+                              const int32_t *c0, const int32_t *c1, int n_group, int32_t *gcnt) {
+    const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= info[kInfoCnt + l]) return;
+    gcnt[q] = pack_node<false>(nstart, nend, c0, c1, (int) (info[kInfoOff + l] + q), l, n_group, nullptr);
+}
+
+// exclusive scan of the per-node group counts in place (one workgroup); the
+// total to info[kInfoRawG]
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void k_group_scan(int l, int32_t *info, int32_t *gcnt) {
+    __shared__ int wsum[kScanThreads / kWave];
+    const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
+    const int cnt = info[kInfoCnt + l];
+    int carry = 0;
+    for (int c0 = 0; c0 < cnt; c0 += kScanThreads) {
+        const int q = c0 + tid;
+        const int v = (q < cnt) ? gcnt[q] : 0;
+        int incl = v;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int t = __shfl_up(incl, o, kWave);
+            if (lane >= o) incl += t;
+        }
+        if (lane == kWave - 1) wsum[w] = incl;
+        __syncthreads();
+        int pre = carry, chunk = 0;
+        for (int u = 0; u < kScanThreads / kWave; ++u) {
+            pre += (u < w) ? wsum[u] : 0;
+            chunk += wsum[u];
+        }
+        if (q < cnt) gcnt[q] = pre + incl - v;
+        carry += chunk;
+        __syncthreads();
+    }
+    if (tid == 0) info[kInfoRawG] = carry;
+}
+
+// the group count (0 and error bit 2 beyond the capacity; 0 after a node
+// overflow) and the end of the last group
+__global__ void k_group_layout(int32_t *info, int64_t gcap, int64_t n, int32_t *gstart) {
+    if (threadIdx.x != 0) return;
+    const int32_t raw = info[kInfoRawG];
+    if (raw > gcap) info[kInfoErr] |= 2;
+    const int32_t G = (info[kInfoErr] == 0) ? raw : 0;
+    info[kInfoG] = G;
+    gstart[G] = (int32_t) n;
+}
+
+// each node's group starts at its offset
+__global__ void k_group_write(int l, const int32_t *info, const int32_t *nstart, const int32_t *nend,
+                              const int32_t *c0, const int32_t *c1, int n_group, const int32_t *goff,
+                              int32_t *gstart) {
+    const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= info[kInfoCnt + l] || info[kInfoG] == 0) return;
+    pack_node<true>(nstart, nend, c0, c1, (int) (info[kInfoOff + l] + q), l, n_group, gstart + goff[q]);
 }
 
 __global__ void k_fill_i32(int64_t n, int32_t *a, int32_t v) {
@@ -424,7 +512,7 @@ __global__ void k_iota(int64_t n, int32_t *a) {
 
 // ------------------------------------------------------------------ lists
 struct ListArgs {
-    int ngroups;
+    const int32_t *gcount;  // device: the group count of the tree build
     const int32_t *gstart, *sys;
     const double *x, *y, *z, *h;
     int nlevels;
@@ -718,7 +806,7 @@ __global__ __launch_bounds__(kWave) void k_list_sph(ListArgs a) {
     __shared__ int s_pref[kWave + 1];
     __shared__ int s_first[kWave];
     __shared__ int32_t s_front[2 * kFrontLds];
-    const GroupQueue Q = xcd_queue(a.ngroups, a.queue);
+    const GroupQueue Q = xcd_queue(*a.gcount, a.queue);
     for (int g = Q.next(); g < Q.end; g = Q.next()) {
         const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
         GBox b;
@@ -759,7 +847,7 @@ __global__ __launch_bounds__(kWave) void k_list_grav(ListArgs a) {
     __shared__ int s_first[kWave];
     __shared__ int32_t s_front[2 * kFrontLds];
     const int64_t fcap = front_cap(a);
-    const GroupQueue Q = xcd_queue(a.ngroups, a.queue);
+    const GroupQueue Q = xcd_queue(*a.gcount, a.queue);
     for (int g = Q.next(); g < Q.end; g = Q.next()) {
         const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
         GBox b;
@@ -836,11 +924,12 @@ __global__ __launch_bounds__(kWave) void k_list_grav(ListArgs a) {
 // in-support particles in the same order.  Sets *flag when some group's box
 // has outgrown its list.
 // A compacted list (k_list_compact) also needs every 2.5 h_i <= R_g = crad[g].
-__global__ __launch_bounds__(kWave) void k_list_box_check(int ngroups, const int32_t *gstart, const double *x,
-                                                          const double *y, const double *z, const double *h,
-                                                          const double *sbox, const double *crad, int32_t *flag) {
+__global__ __launch_bounds__(kWave) void k_list_box_check(const int32_t *gcount, const int32_t *gstart,
+                                                          const double *x, const double *y, const double *z,
+                                                          const double *h, const double *sbox, const double *crad,
+                                                          int32_t *flag) {
     const int g = blockIdx.x;
-    if (g >= ngroups) return;
+    if (g >= *gcount) return;
     const int lane = threadIdx.x;
     const int i0 = gstart[g], ni = gstart[g + 1] - i0;
     const bool act = lane < ni;
@@ -916,12 +1005,14 @@ __device__ __forceinline__ uint64_t mfma_dot_any_(const float4 *q, int nt, float
 // chunks after it start at 64c + 64.
 // 8 waves per SIMD: 62 registers, no spills (the compiler's own choice, 106
 // registers at 4 waves, measured 212 against 197 us at 1M)
-__global__ __launch_bounds__(kWave, 8) void k_list_compact(int ngroups, const int32_t *gstart, const double *x,
-                                                        const double *y, const double *z, const double *h,
-                                                        const double4 *pos, double hfac, int32_t *idx, int32_t *cnt,
-                                                        int64_t cap, double *crad, const int32_t *need) {
+__global__ __launch_bounds__(kWave, 8) void k_list_compact(const int32_t *gcount, const int32_t *gstart,
+                                                        const double *x, const double *y, const double *z,
+                                                        const double *h, const double4 *pos, double hfac, int32_t *idx,
+                                                        int32_t *cnt, int64_t cap, double *crad, const int32_t *need) {
     if (need && *need == 0) return;
-    const int g = xcd_remap((int) blockIdx.x, ngroups);
+    const int G = *gcount;
+    if ((int) blockIdx.x >= G) return;
+    const int g = xcd_remap((int) blockIdx.x, G);
     const int lane = threadIdx.x;
     const int i0 = gstart[g], ni = gstart[g + 1] - i0;
     const bool act = lane < ni;
@@ -1000,31 +1091,35 @@ static void compute_sys_boxes(Tree &t, const TreeParams &tp, int64_t n, const PA
     SPH_LAUNCH_CHECK();
 }
 
+// capacity for a count found: 1/16 and 16 more, so that the counts that creep
+// up from step to step stay inside it for many steps
+static int32_t cap_for(int64_t c) { return (int32_t) std::min<int64_t>(INT32_MAX / 2, c + c / 16 + 16); }
+
+// the device counts into t.hinfo (the stream must then drain before they are read)
+static void fetch_info(Tree &t, hipStream_t s) {
+    int32_t *h = t.hinfo.ensure(kInfoWords);
+    SPH_HIP(hipMemcpyAsync(h, t.info.get(), sizeof(int32_t) * kInfoWords, hipMemcpyDeviceToHost, s));
+}
+
 static void build_levels(Tree &t, const TreeParams &tp, int64_t n, const PArrays &P, hipStream_t s) {
     t.n = n;
+    const int L = tp.nlevels;
+    t.nlevels = L;
+    const bool discover = !t.caps_known;
+    int32_t *info = t.info.ensure(kInfoWords);
     t.cnt.ensure(n);
     hipLaunchKernelGGL(k_lca, dim3(nblk(n, 256)), dim3(256), 0, s, n, t.key.get(), tp.bits, tp.sysbits, t.cnt.get());
     SPH_LAUNCH_CHECK();
-    const int L = tp.nlevels;
-    t.flags.ensure(n);
-    t.sel_out.ensure((size_t) (L + 1) * n);
-    t.counts.ensure(L + 1);
-    hipcub::CountingInputIterator<int32_t> it(0);
-    size_t need = 0;
-    SPH_HIP(hipcub::DeviceSelect::Flagged(nullptr, need, it, t.flags.get(), t.sel_out.get(), t.counts.get(), (int) n, s));
-    if (need > t.sel_tmp_bytes || !t.sel_tmp.get()) {
-        t.sel_tmp.ensure(need);
-        t.sel_tmp_bytes = need;
-    }
+    t.sel_out.ensure((size_t) L * n);
     {  // node starts of all levels: per-block counts, their scan, the scatter
         const int nb = nblk(n, kSelTile);
-        int32_t *blk = t.sel_blk.ensure((size_t) nb * kMaxLevels);
+        int32_t *blk = t.sel_blk.ensure((size_t) std::max(nb, 1) * kMaxLevels);
         if (nb > 0) {
             hipLaunchKernelGGL(k_sel_count, dim3(nb), dim3(kSelThreads), 0, s, n, t.cnt.get(), L,
                                (int64_t) tp.n_leaf, blk);
             SPH_LAUNCH_CHECK();
         }
-        hipLaunchKernelGGL(k_sel_scan, dim3(1), dim3(kSelThreads), 0, s, nb, L, blk, t.counts.get());
+        hipLaunchKernelGGL(k_sel_scan, dim3(1), dim3(kSelThreads), 0, s, nb, L, blk, info);
         SPH_LAUNCH_CHECK();
         if (nb > 0) {
             hipLaunchKernelGGL(k_sel_scatter, dim3(nb), dim3(kSelThreads), 0, s, n, t.cnt.get(), L,
@@ -1032,99 +1127,101 @@ static void build_levels(Tree &t, const TreeParams &tp, int64_t n, const PArrays
             SPH_LAUNCH_CHECK();
         }
     }
-    int32_t *cnts = t.hcnt.ensure(L + 1);  // pinned: a direct copy, no staging
-    SPH_HIP(hipMemcpyAsync(cnts, t.counts.get(), sizeof(int32_t) * L, hipMemcpyDeviceToHost, s));
-    SPH_HIP(hipStreamSynchronize(s));
-    t.nlevels = L;
-    t.level_off[0] = 0;
-    for (int l = 0; l < L; ++l) t.level_off[l + 1] = t.level_off[l] + cnts[l];
-    t.nnodes = t.level_off[L];
-    const size_t nn = (size_t) t.nnodes;
+    if (discover) {  // first build: wait for the counts and size the capacities by them
+        fetch_info(t, s);
+        SPH_HIP(hipStreamSynchronize(s));
+        for (int l = 0; l < L; ++l) t.cap_lv[l] = cap_for(t.hinfo.p[kInfoRawCnt + l]);
+    }
+    LevelCaps caps{};
+    int64_t total = 0, cmax = 1;
+    for (int l = 0; l < L; ++l) {
+        caps.cap[l] = t.cap_lv[l];
+        total += t.cap_lv[l];
+        cmax = std::max<int64_t>(cmax, t.cap_lv[l]);
+    }
+    t.cap_nodes = total;
+    hipLaunchKernelGGL(k_level_layout, dim3(1), dim3(64), 0, s, L, caps, info);
+    SPH_LAUNCH_CHECK();
+    const size_t nn = (size_t) std::max<int64_t>(total, 1);
     t.nstart.ensure(nn); t.nend.ensure(nn); t.c0.ensure(nn); t.c1.ensure(nn);
     t.mass.ensure(nn); t.mx.ensure(nn); t.my.ensure(nn); t.mz.ensure(nn);
     t.comx.ensure(nn); t.comy.ensure(nn); t.comz.ensure(nn);
     t.lox.ensure(nn); t.loy.ensure(nn); t.loz.ensure(nn);
     t.hix.ensure(nn); t.hiy.ensure(nn); t.hiz.ensure(nn); t.maxH.ensure(nn);
     {  // one launch per job for all levels
-        LevelLayout lv{};
-        int64_t cmax = 1;
-        for (int l = 0; l < L; ++l) {
-            lv.cnt[l] = cnts[l];
-            lv.off[l] = t.level_off[l];
-            cmax = std::max<int64_t>(cmax, cnts[l]);
-        }
         const dim3 grid((unsigned) std::min<int64_t>(nblk(cmax, 256), 1024), (unsigned) L);
-        hipLaunchKernelGGL(k_copy_levels, grid, dim3(256), 0, s, n, lv, t.sel_out.get(), t.nstart.get());
+        hipLaunchKernelGGL(k_copy_levels, grid, dim3(256), 0, s, n, info, t.sel_out.get(), t.nstart.get());
         SPH_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_node_ranges, grid, dim3(256), 0, s, n, lv, t.nstart.get(), t.nend.get(), t.c0.get(),
+        hipLaunchKernelGGL(k_node_ranges, grid, dim3(256), 0, s, n, info, t.nstart.get(), t.nend.get(), t.c0.get(),
                            t.c1.get());
         SPH_LAUNCH_CHECK();
     }
     MomArgs ma{t.nstart.get(), t.nend.get(), t.c0.get(), t.c1.get(), t.mass.get(), t.mx.get(), t.my.get(), t.mz.get(),
                t.comx.get(), t.comy.get(), t.comz.get(), t.lox.get(), t.loy.get(), t.loz.get(), t.hix.get(),
                t.hiy.get(), t.hiz.get(), t.maxH.get()};
-    hipLaunchKernelGGL(k_moments_leaf, dim3(nblk(cnts[0], 128)), dim3(128), 0, s, (int64_t) cnts[0], ma, P.x, P.y, P.z,
-                       P.m, P.h);
+    hipLaunchKernelGGL(k_moments_leaf, dim3(nblk(t.cap_lv[0], 128)), dim3(128), 0, s, info, ma, P.x, P.y, P.z, P.m,
+                       P.h);
     SPH_LAUNCH_CHECK();
     for (int l = 1; l < L; ++l) {
-        hipLaunchKernelGGL(k_moments_level, dim3(nblk(cnts[l], 128)), dim3(128), 0, s, (int64_t) cnts[l],
-                           t.level_off[l], ma);
+        hipLaunchKernelGGL(k_moments_level, dim3(nblk(t.cap_lv[l], 128)), dim3(128), 0, s, l, info, ma);
         SPH_LAUNCH_CHECK();
     }
     t.sys_root.ensure(tp.nsys);
     hipLaunchKernelGGL(k_fill_i32, dim3(nblk(tp.nsys, 64)), dim3(64), 0, s, (int64_t) tp.nsys, t.sys_root.get(), -1);
     SPH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_sys_root, dim3(nblk(cnts[L - 1], 64)), dim3(64), 0, s, (int64_t) cnts[L - 1],
-                       t.level_off[L - 1], t.nstart.get(), P.sys, t.sys_root.get());
+    hipLaunchKernelGGL(k_sys_root, dim3(nblk(t.cap_lv[L - 1], 64)), dim3(64), 0, s, L - 1, info, t.nstart.get(), P.sys,
+                       t.sys_root.get());
     SPH_LAUNCH_CHECK();
     // i-groups: leaves packed greedily inside each level-2 node (or the top level)
     const int pl = std::min(2, L - 1);
-    hipLaunchKernelGGL(k_fill_u8, dim3(nblk(n, 256)), dim3(256), 0, s, n, t.flags.get());
+    int32_t *gcnt = t.gcnt.ensure((size_t) t.cap_lv[pl] + 1);
+    hipLaunchKernelGGL(k_group_count, dim3(nblk(t.cap_lv[pl], 64)), dim3(64), 0, s, pl, info, t.nstart.get(),
+                       t.nend.get(), t.c0.get(), t.c1.get(), tp.n_group, gcnt);
     SPH_LAUNCH_CHECK();
-    const int64_t cpl = cnts[pl];
-    hipLaunchKernelGGL(k_pack_groups, dim3(nblk(cpl, 64)), dim3(64), 0, s, cpl, t.level_off[pl], t.nstart.get(),
-                       t.nend.get(), t.c0.get(), t.c1.get(), pl, tp.n_group, t.flags.get());
+    hipLaunchKernelGGL(k_group_scan, dim3(1), dim3(kScanThreads), 0, s, pl, info, gcnt);
     SPH_LAUNCH_CHECK();
-    size_t bytes = t.sel_tmp_bytes;
-    SPH_HIP(hipcub::DeviceSelect::Flagged(t.sel_tmp.get(), bytes, it, t.flags.get(), t.sel_out.get() + (size_t) L * n,
-                                          t.counts.get() + L, (int) n, s));
-    int32_t *hng = t.hcnt.p + L;
-    SPH_HIP(hipMemcpyAsync(hng, t.counts.get() + L, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    SPH_HIP(hipStreamSynchronize(s));
-    t.ngroups = *hng;
-    t.gstart.ensure(t.ngroups + 1);
-    SPH_HIP(hipMemcpyAsync(t.gstart.get(), t.sel_out.get() + (size_t) L * n, sizeof(int32_t) * t.ngroups,
-                           hipMemcpyDeviceToDevice, s));
-    const int32_t nn32 = (int32_t) n;
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s, (int64_t) 1, t.gstart.get() + t.ngroups, nn32);
+    if (discover) {  // first build: wait for the group count
+        fetch_info(t, s);
+        SPH_HIP(hipStreamSynchronize(s));
+        t.gcap = cap_for(t.hinfo.p[kInfoRawG]);
+    }
+    t.gstart.ensure((size_t) t.gcap + 1);
+    hipLaunchKernelGGL(k_group_layout, dim3(1), dim3(64), 0, s, info, t.gcap, n, t.gstart.get());
     SPH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_group_write, dim3(nblk(t.cap_lv[pl], 64)), dim3(64), 0, s, pl, info, t.nstart.get(),
+                       t.nend.get(), t.c0.get(), t.c1.get(), tp.n_group, gcnt, t.gstart.get());
+    SPH_LAUNCH_CHECK();
+    t.caps_known = true;
 }
 
 static void compute_keys(Tree &t, const TreeParams &tp, int64_t n, const PArrays &P, hipStream_t s) {
     compute_sys_boxes(t, tp, n, P, s);
     t.key.ensure(n);
     t.key_alt.ensure(n);
-    t.perm.ensure(n);
-    t.iota.ensure(n);
     hipLaunchKernelGGL(k_keys, dim3(nblk(n, 256)), dim3(256), 0, s, n, P.x, P.y, P.z, P.sys, tp.bits, t.sys_lo.get(),
-                       t.sys_scale.get(), t.key.get(), t.iota.get());
+                       t.sys_scale.get(), t.key.get());
     SPH_LAUNCH_CHECK();
 }
 
 void tree_sort_and_build(Tree &t, const TreeParams &tp, int64_t n, PArrays &P, PArrays &Palt, hipStream_t s) {
     compute_keys(t, tp, n, P, s);
-    size_t need = 0;
-    SPH_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need, t.key.get(), t.key_alt.get(), t.iota.get(), t.perm.get(),
-                                                (int) n, 0, 63, s));
-    if (need > t.sort_tmp_bytes || !t.sort_tmp.get()) {
-        t.sort_tmp.ensure(need);
-        t.sort_tmp_bytes = need;
+    t.perm.ensure(n);
+    t.vtmp.ensure(n);
+    const int nb = radix_sort_tiles(n);
+    t.rs_bh.ensure((size_t) std::max(nb, 1) * 256);
+    t.rs_tot.ensure(256);
+    uint64_t *kres = nullptr;
+    int32_t *vres = nullptr;
+    radix_sort_pairs(n, 3 * tp.bits + tp.sysbits, t.key.get(), t.key_alt.get(), t.perm.get(), t.vtmp.get(),
+                     t.rs_bh.get(), t.rs_tot.get(), &kres, &vres, s);
+    if (kres != t.key.get()) {
+        std::swap(t.key.p, t.key_alt.p);
+        std::swap(t.key.n, t.key_alt.n);
     }
-    size_t bytes = t.sort_tmp_bytes;
-    SPH_HIP(hipcub::DeviceRadixSort::SortPairs(t.sort_tmp.get(), bytes, t.key.get(), t.key_alt.get(), t.iota.get(),
-                                                t.perm.get(), (int) n, 0, 63, s));
-    std::swap(t.key.p, t.key_alt.p);
-    std::swap(t.key.n, t.key_alt.n);
+    if (vres != t.perm.get()) {
+        std::swap(t.perm.p, t.vtmp.p);
+        std::swap(t.perm.n, t.vtmp.n);
+    }
     // carry the fields that survive into the next pass of the step
     PermArgs pa;
     pa.perm = t.perm.get();
@@ -1149,21 +1246,51 @@ void tree_build_only(Tree &t, const TreeParams &tp, int64_t n, const PArrays &P,
 
 void tree_update_maxH(Tree &t, const double *h, hipStream_t s) {
     const int L = t.nlevels;
-    const int64_t c0n = t.level_off[1] - t.level_off[0];
-    hipLaunchKernelGGL(k_maxH_leaf, dim3(nblk(c0n, 128)), dim3(128), 0, s, c0n, t.nstart.get(), t.nend.get(), h,
-                       t.maxH.get());
+    const int32_t *info = t.info.get();
+    hipLaunchKernelGGL(k_maxH_leaf, dim3(nblk(t.cap_lv[0], 128)), dim3(128), 0, s, info, t.nstart.get(), t.nend.get(),
+                       h, t.maxH.get());
     SPH_LAUNCH_CHECK();
     for (int l = 1; l < L; ++l) {
-        const int64_t cl = t.level_off[l + 1] - t.level_off[l];
-        hipLaunchKernelGGL(k_maxH_level, dim3(nblk(cl, 128)), dim3(128), 0, s, cl, t.level_off[l], t.c0.get(),
+        hipLaunchKernelGGL(k_maxH_level, dim3(nblk(t.cap_lv[l], 128)), dim3(128), 0, s, l, info, t.c0.get(),
                            t.c1.get(), t.maxH.get());
         SPH_LAUNCH_CHECK();
     }
 }
 
+void tree_apply_info(Tree &t, const int32_t *h) {
+    const int L = t.nlevels;
+    for (int l = 0; l <= L; ++l) t.level_off[l] = h[kInfoOff + l];
+    t.nnodes = h[kInfoOff + L];
+    t.ngroups = h[kInfoG];
+}
+
+bool tree_grow_caps(Tree &t, const int32_t *h) {
+    bool grown = false;
+    for (int l = 0; l < t.nlevels; ++l) {
+        const int64_t c = h[kInfoRawCnt + l];
+        if (c + c / 64 + 8 > t.cap_lv[l]) {  // over, or close to, the capacity
+            t.cap_lv[l] = cap_for(c);
+            grown = true;
+        }
+    }
+    const int64_t g = h[kInfoRawG];
+    if ((h[kInfoErr] & 1) == 0 && g + g / 64 + 8 > t.gcap) {  // (after a node overflow the group count is 0)
+        t.gcap = cap_for(g);
+        grown = true;
+    }
+    return grown;
+}
+
+void tree_sync_counts(Tree &t, hipStream_t s) {
+    fetch_info(t, s);
+    SPH_HIP(hipStreamSynchronize(s));
+    if (t.hinfo.p[kInfoErr]) throw Error(-2, "tree: counts beyond the capacities of the last build");
+    tree_apply_info(t, t.hinfo.p);
+}
+
 static ListArgs list_args(const Tree &t, const TreeParams &tp, const PArrays &P, Lists &L) {
     ListArgs a;
-    a.ngroups = (int) t.ngroups;
+    a.gcount = t.info.get() + kInfoG;
     a.gstart = t.gstart.get();
     a.sys = P.sys;
     a.x = P.x; a.y = P.y; a.z = P.z; a.h = P.h;
@@ -1189,8 +1316,8 @@ static ListArgs list_args(const Tree &t, const TreeParams &tp, const PArrays &P,
     return a;
 }
 
-static int list_grid(int64_t ngroups) {
-    int64_t b = std::min<int64_t>(ngroups, kListSlots);
+static int list_grid(int64_t ngroups, int waves) {
+    int64_t b = std::min<int64_t>(ngroups, waves > 0 ? std::min(waves, kListSlots) : kListWavesDefault);
     b = (b + 7) & ~7LL;
     return (int) std::max<int64_t>(8, b);
 }
@@ -1205,7 +1332,7 @@ void lists_reset(Lists &L, hipStream_t s) {
 
 void enqueue_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, double hfac,
                    hipStream_t s, const int32_t *need) {
-    const int64_t G = t.ngroups;
+    const int64_t G = t.gcap;
     if (kind != 0) hfac = 1.0;  // symmetric lists are exact: v_sig_max runs over the whole list
     L.idx.ensure((size_t) G * L.cap);
     L.cnt.ensure(G);
@@ -1214,7 +1341,7 @@ void enqueue_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int ki
         L.scnt.ensure(G);
     }
     if (L.fcap <= 0) L.fcap = kFrontier0;
-    const int grid = list_grid(G);
+    const int grid = list_grid(G, L.waves);
     L.frontier.ensure((size_t) grid * 2 * L.fcap);
     L.err.ensure(kListWords);  // error bits, construction counter, the 8 per-XCD group counters
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int64_t) 8, L.err.get() + 2, 0);
@@ -1239,12 +1366,12 @@ void enqueue_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int ki
 void enqueue_gather_compact(const Tree &t, const PArrays &P, Lists &L, double hfac, hipStream_t s,
                             const int32_t *need, const double4 *pos) {
     if (L.kind != 0) throw Error(-1, "enqueue_gather_compact: the lists held are not gather lists");
-    const int64_t G = t.ngroups;
+    const int64_t G = t.gcap;
     L.crad.ensure((size_t) std::max<int64_t>(G, 1));
     L.compact = true;
     if (G <= 0) return;
-    hipLaunchKernelGGL(k_list_compact, dim3((unsigned) G), dim3(kWave), 0, s, (int) G, t.gstart.get(), P.x, P.y,
-                       P.z, P.h, pos, hfac, L.idx.get(), L.cnt.get(), L.cap, L.crad.get(), need);
+    hipLaunchKernelGGL(k_list_compact, dim3((unsigned) G), dim3(kWave), 0, s, t.info.get() + kInfoG, t.gstart.get(),
+                       P.x, P.y, P.z, P.h, pos, hfac, L.idx.get(), L.cnt.get(), L.cap, L.crad.get(), need);
     SPH_LAUNCH_CHECK();
 }
 
@@ -1253,8 +1380,7 @@ void lists_fetch(Lists &L, hipStream_t s, int slot) {
     SPH_HIP(hipMemcpyAsync(h + 2 * slot, L.err.get(), sizeof(int32_t) * 2, hipMemcpyDeviceToHost, s));
 }
 
-bool lists_grow(Lists &L) {
-    const int32_t err = L.hflag.p[0] | L.hflag.p[2];
+bool lists_grow_bits(Lists &L, int32_t err) {
     if (err & 4) L.fcap *= 2;
     if (err & 1) L.cap *= 2;
     if (err & 2) L.scap *= 2;
@@ -1262,12 +1388,14 @@ bool lists_grow(Lists &L) {
     return err == 0;
 }
 
+bool lists_grow(Lists &L) { return lists_grow_bits(L, L.hflag.p[0] | L.hflag.p[2]); }
+
 void enqueue_gather_check(const Tree &t, const PArrays &P, Lists &L, hipStream_t s) {
     L.flag.ensure(1);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int64_t) 1, L.flag.get(), 0);
     SPH_LAUNCH_CHECK();
-    if (t.ngroups <= 0) return;
-    hipLaunchKernelGGL(k_list_box_check, dim3((unsigned) t.ngroups), dim3(kWave), 0, s, (int) t.ngroups,
+    if (t.gcap <= 0) return;
+    hipLaunchKernelGGL(k_list_box_check, dim3((unsigned) t.gcap), dim3(kWave), 0, s, t.info.get() + kInfoG,
                        t.gstart.get(), P.x, P.y, P.z, P.h, L.sbox.get(), L.compact ? L.crad.get() : nullptr,
                        L.flag.get());
     SPH_LAUNCH_CHECK();

@@ -227,6 +227,16 @@ class Simulation:
         check(self._L.sphgpu_sim_list_builds(self._h, C.byref(g), C.byref(f)))
         return g.value, f.value
 
+    def set_capacities(self, sph: int, epj: int, spj: int):
+        """Interaction-list capacities of the next force evaluations."""
+        check(self._L.sphgpu_sim_set_capacities(self._h, int(sph), int(epj), int(spj)))
+
+    def graph_stats(self) -> tuple[int, int]:
+        """(steps run as graph replays, graphs recorded) since init."""
+        r, c = C.c_int64(), C.c_int64()
+        check(self._L.sphgpu_sim_graph_stats(self._h, C.byref(r), C.byref(c)))
+        return r.value, c.value
+
     def set_stats(self, on: bool = True):
         check(self._L.sphgpu_sim_set_stats(self._h, int(bool(on))))
 

@@ -192,6 +192,18 @@ typedef struct {
                                 all systems from per-XCD counters: 1 density, 2 derivative,
                                 4 hydro, 8 gravity; 0 (default): one workgroup per group.
                                 Results are identical either way. */
+    int32_t graph;           /* 1: each step runs as the replay of a recorded HIP graph (one per
+                                parity of the particle buffer sets, recorded after a direct step
+                                with the same capacities, recorded again whenever a capacity,
+                                the precision or an allocation changes); 0 (default): launched
+                                step by step.  A replayed step whose force evaluation overflows
+                                a capacity leaves the state untouched and is finished directly.
+                                Results are identical either way.  Ignored while kernel
+                                statistics are on. */
+    int32_t walk_waves;      /* persistent waves of each tree walk that builds interaction lists
+                                (rounded up to a multiple of 8, at most 8192); 0 (default): 4096.
+                                Walks beside force kernels share the GPU with them; fewer
+                                waves leave the kernels more of it.  Results are identical. */
 } sphgpu_sim_params;
 
 void sphgpu_sim_default_params(sphgpu_sim_params *p);
@@ -234,6 +246,19 @@ int32_t sphgpu_sim_stage_ms(sphgpu_ctx *ctx, double *ms, int32_t n);
  * one gather list (built with an 8 % margin on h) for as long as it covers every
  * particle's support, so gather_builds / force_evals is 1 when h settles. */
 int32_t sphgpu_sim_list_builds(sphgpu_ctx *ctx, int64_t *gather_builds, int64_t *force_evals);
+
+/* Sets the per-group interaction-list capacities of the next force
+ * evaluations (all >= 1): gather/symmetric j-list, gravity particle list,
+ * gravity monopole list, as sphgpu_sim_params.list_cap_*.  For callers that
+ * bound memory: an evaluation that needs more runs again with the capacity
+ * doubled (a recorded step leaves the state untouched first), so results do
+ * not depend on the capacities. */
+int32_t sphgpu_sim_set_capacities(sphgpu_ctx *ctx, int32_t list_cap_sph, int32_t list_cap_epj,
+                                  int32_t list_cap_spj);
+
+/* Steps run as replays of a recorded graph and graphs recorded since
+ * sphgpu_sim_init (sphgpu_sim_params.graph). */
+int32_t sphgpu_sim_graph_stats(sphgpu_ctx *ctx, int64_t *replayed_steps, int64_t *captures);
 
 /* Per-kernel statistics of the interaction kernels over the steps run since
  * statistics were switched on: summed kernel time (hipEvents on the
