@@ -173,7 +173,7 @@ struct SimState {
     DevBuf<int32_t> sched;  // per-XCD group counters of persistent force-kernel launches
     DevBuf<double> dt_sys, time_sys, dt_part;
     hipEvent_t ev[kStages + 1] = {};
-    hipEvent_t xev[4] = {};  // cross-stream order of a force evaluation (no timing)
+    hipEvent_t xev[5] = {};  // cross-stream order of a force evaluation (no timing)
     double stage_ms[kStages] = {0};
     bool ready = false;
     // per-kernel statistics (density x4, derivative, hydro, gravity per step)
@@ -636,6 +636,11 @@ void launch_stash(StashArgs a, hipStream_t s) {
     SPH_LAUNCH_CHECK();
 }
 
+__global__ void k_fill_i32_api(int64_t n, int32_t *a, int32_t v) {
+    const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] = v;
+}
+
 __global__ void k_copy_d(int64_t n, const double *src, double *dst) {
     const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) dst[k] = src[k];
@@ -729,8 +734,16 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
     };
     lists_reset(L, s);
     lists_reset(Lg, s);
+    // per density pass: set when some group's gather list no longer covers the
+    // supports of the h that pass computed (the list is then rebuilt first)
+    int32_t *gflag = L.flag.ensure(4);
+    hipLaunchKernelGGL(k_fill_i32_api, dim3(1), dim3(64), 0, s, (int64_t) 4, gflag, 0);
+    SPH_LAUNCH_CHECK();
     enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s);
     enqueue_gather_compact(S.tree, P, L, kGatherMargin, s, nullptr, kpos);
+    // the gravity walk beside the density passes (started beside the gather
+    // walk instead, the step measured the same: two walks together take as
+    // long as one after the other)
     signal(S.xev[0], s);  // tree, h restored, counters cleared, first gather lists
     await(s2, S.xev[0]);
     enqueue_lists(S.tree, S.tp, P, 2, Lg, 1.0, s2);
@@ -738,16 +751,19 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
     // 4 density passes, each over lists that cover the current h
     // (src/main.cpp:138-140).  One gather list, built with a margin on h,
     // serves every pass (and the derivative pass) while each group's
-    // search box still fits inside the box it was built for.
+    // search box still fits inside the box it was built for; each pass checks
+    // that for the h it computes.
     for (int loop = 0; loop <= 3; ++loop) {
         if (loop > 0) {
-            enqueue_gather_check(S.tree, P, L, s);
-            enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s, L.flag.get());
-            enqueue_gather_compact(S.tree, P, L, kGatherMargin, s, L.flag.get(), kpos);
+            enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s, gflag + loop - 1);
+            enqueue_gather_compact(S.tree, P, L, kGatherMargin, s, gflag + loop - 1, kpos);
         }
         DensArgs a{G, gs, P.x, P.y, P.z, P.m, P.h, P.m, P.x, P.y, P.z, lref(), P.rho, P.h, P.ngb, kpos};
         a.sched = sched_for(1);
         a.gcount = gcount;
+        a.sbox = L.sbox.get();
+        a.crad = L.crad.get();
+        a.flag = gflag + loop;
         // the density of the last pass feeds the EoS; the precision form
         // keeps it FP64 unless asked otherwise (the earlier passes only steer h)
         const bool d64 = !mixed || (loop == 3 && S.prm.eos_density_fp64);
@@ -757,11 +773,15 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
         kend();
     }
     rec_ev(3);
+    // h is final: the tree's largest supports for the symmetric lists, beside
+    // the EoS and the derivative pass
+    signal(S.xev[4], s);
+    await(s2, S.xev[4]);
+    tree_update_maxH(S.tree, P.h, s2);
     launch_eos_pressure(n, P.mat, P.rho, P.u, c->d_mats.get(), SPHGPU_MAX_MATERIALS, P.p, P.cs, s);
     rec_ev(4);
-    enqueue_gather_check(S.tree, P, L, s);
-    enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s, L.flag.get());
-    enqueue_gather_compact(S.tree, P, L, kGatherMargin, s, L.flag.get(), kpos);
+    enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s, gflag + 3);
+    enqueue_gather_compact(S.tree, P, L, kGatherMargin, s, gflag + 3, kpos);
     {
         DrvtArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.cs, P.m, P.x, P.y, P.z, P.vx, P.vy, P.vz,
                    lref(), P.divv, P.rotx, P.roty, P.rotz, P.gs, P.bal, kpos, kvel};
@@ -773,9 +793,8 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
         kend();
     }
     rec_ev(5);
-    signal(S.xev[2], s);  // final h; the derivative pass is done with the gather lists
+    signal(S.xev[2], s);  // the derivative pass is done with the gather lists
     await(s2, S.xev[2]);
-    tree_update_maxH(S.tree, P.h, s2);
     // gather-list bits and construction counter, before the symmetric lists reuse L
     launch_stash(StashArgs{{L.err.get(), nullptr, nullptr, nullptr}, {2, 0, 0, 0}, stash + kStGather}, s2);
     enqueue_lists(S.tree, S.tp, P, 1, L, 1.0, s2);

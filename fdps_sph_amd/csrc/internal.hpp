@@ -139,6 +139,11 @@ struct DensArgs {
     // if set: the group count is read here (a tree build's device count, at
     // most ngroups, which then only sizes the launch)
     const int32_t *gcount = nullptr;
+    // if flag is set: after the pass, whether each group's gather list still
+    // covers the new supports (gather_cover_check against the list's search
+    // box sbox and, if given, its compaction radius crad); *flag |= 1 if not
+    const double *sbox = nullptr, *crad = nullptr;
+    int32_t *flag = nullptr;
 };
 
 struct DrvtArgs {
@@ -336,14 +341,13 @@ struct Lists {
     DevBuf<int32_t> frontier;
     int64_t fcap = 0;        // per-workgroup BFS frontier capacity in global memory
     int flds = kFrontLds;    // frontier entries kept in LDS before the global part
-    int waves = 0;           // persistent waves of a walk (0: the default, 8192)
+    int waves = 0;           // persistent waves of a walk (0: the default, 4096)
     DevBuf<int32_t> err;       // overflow bits, construction counter, per-XCD group counters
-    DevBuf<int32_t> flag;      // gather-list validity: 1 when some group outgrew its list
+    DevBuf<int32_t> flag;      // per density pass: 1 when some group outgrew its gather list
     PinnedBuf<int32_t> hflag;  // host copy of the overflow bits and the construction counter
     DevBuf<double> sbox;     // gather lists: per-group search box (lo[3], hi[3])
     DevBuf<double> crad;     // compacted gather lists: per-group radius R_g they cover
     int kind = -1;           // kind of the lists held (-1 none, 0 gather, 1 symmetric, 2 gravity)
-    bool compact = false;    // the gather lists were compacted (enqueue_gather_compact)
 };
 // Search-radius factor of the gather lists that the density passes reuse:
 // 8 % on h covers the change of h over the density iterations of nearly every
@@ -367,10 +371,6 @@ void enqueue_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int ki
 void lists_fetch(Lists &L, hipStream_t s, int slot = 0);
 bool lists_grow(Lists &L);
 bool lists_grow_bits(Lists &L, int32_t err);
-// Sets L.flag (device) to 1 when some group's current search box (same tree,
-// positions unchanged, h possibly changed) has outgrown its gather list, or,
-// for compacted lists, when some 2.5 h_i of the group exceeds its R_g.
-void enqueue_gather_check(const Tree &t, const PArrays &P, Lists &L, hipStream_t s);
 // Compacts each group's gather list in place to the entries within R_g of
 // some particle of the group, R_g = hfac * max_i 2.5 h_i (h as now), keeping
 // their order; when `need` is given, only if that device flag is nonzero (as
@@ -692,6 +692,25 @@ __device__ __forceinline__ uint64_t mfma_dot_mask_sym(QF q, QSF qs, int nt, floa
     uint64_t mask = ((uint64_t) hi << 32) | lo;
     if (nt < kWave) mask &= (1ULL << nt) - 1;
     return mask;
+}
+
+// Whether group g's gather list still holds every particle within 2.5 h of
+// each of its particles, h as just updated by a density pass: the group's
+// search box (particle +- 2.5 h) must lie inside the box sbox[g] the list was
+// built for, and for a compacted list every 2.5 h must stay within its radius
+// crad[g] (see k_list_compact).  Sets *flag when not, so that the list is
+// rebuilt before it is used again.  The box's extremes lie outside exactly
+// when some particle's do, so each lane tests its own particle and one ballot
+// decides.  Every lane of the wave takes part; `act` marks the lanes that hold
+// a particle.
+__device__ __forceinline__ void gather_cover_check(const double *sbox, const double *crad, int32_t *flag, int g,
+                                                   bool act, double x, double y, double z, double h) {
+    const double H = kSupport * h;
+    const double *b = sbox + (int64_t) g * 6;
+    bool bad = (x - H < b[0]) || (y - H < b[1]) || (z - H < b[2]) || (x + H > b[3]) || (y + H > b[4]) ||
+               (z + H > b[5]);
+    if (crad) bad = bad || (H > crad[g]);
+    if (__ballot(act && bad) != 0 && threadIdx.x == 0) atomicOr(flag, 1);
 }
 
 __device__ __forceinline__ int64_t list_begin(const ListRef &L, int g) {

@@ -192,12 +192,15 @@ __device__ __forceinline__ void dens_group(const DensArgs &a, const int g) {
         }
         __syncthreads();
     }
+    double hn = 0.0;  // lanes without a particle take no part in the check
     if (act) {
         dens = dmax(5.0, dens * norm);  // FLAG_GI floor (src/force.h:18-20)
         a.rho_out[ip] = dens;
-        a.h_out[ip] = kSMTH * pow(mi / dens, 1.0 / 3.0);
+        hn = kSMTH * pow(mi / dens, 1.0 / 3.0);
+        a.h_out[ip] = hn;
         if (a.ngb_out) a.ngb_out[ip] = ngb;
     }
+    if (a.flag) gather_cover_check(a.sbox, a.crad, a.flag, g, act, xi, yi, zi, hn);
 }
 __global__ __launch_bounds__(kWave, 5) void k_dens(DensArgs a) {
     const int G = group_count(a);

@@ -157,12 +157,17 @@ __device__ __forceinline__ void dens_mixed_group(const DensArgs &a, const int g)
         }
         dens.add(part * norm);
     }
+    double hn = 0.0;  // lanes without a particle take no part in the check
     if (act) {
         const double d = dmax(5.0, dens.value());
         a.rho_out[ip] = d;
-        a.h_out[ip] = kSMTH * pow(a.mi[ip] / d, 1.0 / 3.0);
+        hn = kSMTH * pow(a.mi[ip] / d, 1.0 / 3.0);
+        a.h_out[ip] = hn;
         if (a.ngb_out) a.ngb_out[ip] = ngb;
     }
+    // the positions again from memory (cached): keeping the FP64 ones live
+    // through the tile loop cost a wave of occupancy
+    if (a.flag) gather_cover_check(a.sbox, a.crad, a.flag, g, act, a.xi[ip], a.yi[ip], a.zi[ip], hn);
 }
 __global__ __launch_bounds__(kWave, 5) void k_dens_mixed(DensArgs a) {
     const int G = group_count(a);

@@ -917,36 +917,6 @@ __global__ __launch_bounds__(kWave) void k_list_grav(ListArgs a) {
     }
 }
 
-// A gather list built for the search box sbox[g] still holds every particle
-// within 2.5 h_i of each i of group g when the group's current search box
-// (h_i as now) lies inside sbox[g]: the list is the sorted set of particle
-// indices inside that box, so it is then a superset of the exact list with the
-// in-support particles in the same order.  Sets *flag when some group's box
-// has outgrown its list.
-// A compacted list (k_list_compact) also needs every 2.5 h_i <= R_g = crad[g].
-__global__ __launch_bounds__(kWave) void k_list_box_check(const int32_t *gcount, const int32_t *gstart,
-                                                          const double *x, const double *y, const double *z,
-                                                          const double *h, const double *sbox, const double *crad,
-                                                          int32_t *flag) {
-    const int g = blockIdx.x;
-    if (g >= *gcount) return;
-    const int lane = threadIdx.x;
-    const int i0 = gstart[g], ni = gstart[g + 1] - i0;
-    const bool act = lane < ni;
-    const int p = i0 + (act ? lane : 0);
-    const double c[3] = {x[p], y[p], z[p]};
-    const double H = kSupport * h[p];
-    bool bad = false;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const double lo = wave_min(act ? c[k] - H : INFINITY);
-        const double hi = wave_max(act ? c[k] + H : -INFINITY);
-        bad = bad || (lo < sbox[(int64_t) g * 6 + k]) || (hi > sbox[(int64_t) g * 6 + 3 + k]);
-    }
-    if (crad) bad = bad || (wave_max(act ? H : 0.0) > crad[g]);
-    if (lane == 0 && bad) atomicOr(flag, 1);
-}
-
 __device__ __forceinline__ float wave_maxf(float v) { return dpp_wave_maxf(v); }
 
 // Which entries of a tile lie inside the cut of some lane: the dot form of
@@ -1360,7 +1330,6 @@ void enqueue_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int ki
     else hipLaunchKernelGGL(k_list_grav, dim3(grid), dim3(kWave), 0, s, a);
     SPH_LAUNCH_CHECK();
     L.kind = kind;
-    L.compact = false;
 }
 
 void enqueue_gather_compact(const Tree &t, const PArrays &P, Lists &L, double hfac, hipStream_t s,
@@ -1368,7 +1337,6 @@ void enqueue_gather_compact(const Tree &t, const PArrays &P, Lists &L, double hf
     if (L.kind != 0) throw Error(-1, "enqueue_gather_compact: the lists held are not gather lists");
     const int64_t G = t.gcap;
     L.crad.ensure((size_t) std::max<int64_t>(G, 1));
-    L.compact = true;
     if (G <= 0) return;
     hipLaunchKernelGGL(k_list_compact, dim3((unsigned) G), dim3(kWave), 0, s, t.info.get() + kInfoG, t.gstart.get(),
                        P.x, P.y, P.z, P.h, pos, hfac, L.idx.get(), L.cnt.get(), L.cap, L.crad.get(), need);
@@ -1389,17 +1357,6 @@ bool lists_grow_bits(Lists &L, int32_t err) {
 }
 
 bool lists_grow(Lists &L) { return lists_grow_bits(L, L.hflag.p[0] | L.hflag.p[2]); }
-
-void enqueue_gather_check(const Tree &t, const PArrays &P, Lists &L, hipStream_t s) {
-    L.flag.ensure(1);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int64_t) 1, L.flag.get(), 0);
-    SPH_LAUNCH_CHECK();
-    if (t.gcap <= 0) return;
-    hipLaunchKernelGGL(k_list_box_check, dim3((unsigned) t.gcap), dim3(kWave), 0, s, t.info.get() + kInfoG,
-                       t.gstart.get(), P.x, P.y, P.z, P.h, L.sbox.get(), L.compact ? L.crad.get() : nullptr,
-                       L.flag.get());
-    SPH_LAUNCH_CHECK();
-}
 
 // synchronous forms: build, wait, grow the capacities and build again until the lists fit
 static bool build_lists_sync(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, double hfac,
