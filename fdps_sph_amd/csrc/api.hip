@@ -809,6 +809,8 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
                    P.gx, P.gy, P.gz, P.pot};
         a.sched = sched_for(8);
         a.gcount = gcount;
+        a.jrec = kpos;
+        a.srec = t.nrec.get();
         kbegin(&Lg, true, 3);
         if (mixed) launch_grav_mixed(a, s);
         else launch_grav_fp64(a, s);

@@ -201,6 +201,9 @@ struct GravArgs {
     const double *xs, *ys, *zs, *ms;  // monopoles (SPJ); may be null
     ListRef Ls;
     double *ax, *ay, *az, *pot;
+    // if set: the particles' (x, y, z, m) and the monopoles' (x, y, z, m) as
+    // one 32-byte record each, read instead of the separate arrays
+    const double4 *jrec = nullptr, *srec = nullptr;
     // persistent scheduling: 8 per-XCD group counters, cleared before the launch
     // (null: one workgroup per group)
     int32_t *sched = nullptr;
@@ -305,6 +308,7 @@ struct Tree {
     int64_t nnodes = 0;
     DevBuf<int32_t> nstart, nend, c0, c1;
     DevBuf<double> mass, mx, my, mz, comx, comy, comz, lox, loy, loz, hix, hiy, hiz, maxH;
+    DevBuf<double4> nrec;  // (com x, y, z, mass) of each node, the gravity kernels' monopole record
     // i-groups
     int64_t ngroups = 0;
     DevBuf<int32_t> gstart;

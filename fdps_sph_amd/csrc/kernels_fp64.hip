@@ -572,6 +572,13 @@ __device__ __forceinline__ void grav_tile_(const double4 *sp, int nt, double xi,
     }
 }
 
+// j position and mass of list entry q of one pass, from the packed record
+// when there is one
+__device__ __forceinline__ double4 grav_load_(const double4 *rec, const double *X, const double *Y, const double *Z,
+                                              const double *M, int q) {
+    return rec ? rec[q] : make_double4(X[q], Y[q], Z[q], M[q]);
+}
+
 __device__ __forceinline__ void grav_group(const GravArgs &a, const int g) {
     const int lane = threadIdx.x;
     const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
@@ -584,17 +591,27 @@ __device__ __forceinline__ void grav_group(const GravArgs &a, const int g) {
     double ax = 0.0, ay = 0.0, az = 0.0, pot = 0.0;
     __shared__ double4 sp[kWave];
     for (int pass = 0; pass < 2; ++pass) {
-        if (pass && !a.ms) break;
+        if (pass && !a.ms && !a.srec) break;
         const ListRef &L = pass ? a.Ls : a.Lj;
+        const double4 *R = pass ? a.srec : a.jrec;
         const double *X = pass ? a.xs : a.xj, *Y = pass ? a.ys : a.yj, *Z = pass ? a.zs : a.zj,
                      *Mm = pass ? a.ms : a.mj;
         const int64_t lb = list_begin(L, g);
         const int nj = L.cnt[g];
+        // Software-pipelined tiles: the record of tile t+1 and the list index
+        // of tile t+2 are in flight while tile t is evaluated (a lane without
+        // an entry loads entry 0's record, which exists whenever nj > 0).
+        double4 r = make_double4(0.0, 0.0, 0.0, 0.0);
+        int qn = 0;
+        if (nj > 0) {
+            r = grav_load_(R, X, Y, Z, Mm, lane < nj ? L.idx[lb + lane] : L.idx[lb]);
+            qn = (kWave + lane < nj) ? L.idx[lb + kWave + lane] : L.idx[lb];
+        }
         for (int t = 0; t < nj; t += kWave) {
-            const int k = t + lane;
-            if (k < nj) {
-                const int q = L.idx[lb + k];
-                sp[lane] = make_double4(X[q], Y[q], Z[q], kG * Mm[q]);
+            if (t + lane < nj) sp[lane] = make_double4(r.x, r.y, r.z, kG * r.w);
+            if (t + kWave < nj) {
+                r = grav_load_(R, X, Y, Z, Mm, qn);
+                qn = (t + 2 * kWave + lane < nj) ? L.idx[lb + t + 2 * kWave + lane] : L.idx[lb];
             }
             __syncthreads();
             grav_tile_(sp, min(kWave, nj - t), xi, yi, zi, eps2, ax, ay, az, pot);

@@ -491,19 +491,22 @@ __device__ __forceinline__ void grav_mixed_group(const GravArgs &a, const int g)
     __shared__ float sx[kWave], sy[kWave], sz[kWave], sgm[kWave];
     for (int pass = 0; pass < 2; ++pass) {
         const ListRef &L = pass ? a.Ls : a.Lj;
-        if (pass && !a.ms) break;
+        if (pass && !a.ms && !a.srec) break;
+        const double4 *R = pass ? a.srec : a.jrec;
         const double *X = pass ? a.xs : a.xj, *Y = pass ? a.ys : a.yj, *Z = pass ? a.zs : a.zj,
                      *M = pass ? a.ms : a.mj;
+        auto load = [&](int q) { return R ? R[q] : make_double4(X[q], Y[q], Z[q], M[q]); };
         const int64_t lb = list_begin(L, g);
         const int nj = L.cnt[g];
+        // (no prefetch of the next tile: the kernel is bound by its VALU issue,
+        // and the registers a prefetch holds cost a wave of occupancy)
         for (int t = 0; t < nj; t += kWave) {
-            const int k = t + lane;
-            if (k < nj) {
-                const int q = L.idx[lb + k];
-                sx[lane] = (float) (X[q] - cx);
-                sy[lane] = (float) (Y[q] - cy);
-                sz[lane] = (float) (Z[q] - cz);
-                sgm[lane] = (float) (kG * M[q]);
+            if (t + lane < nj) {
+                const double4 r = load(L.idx[lb + t + lane]);
+                sx[lane] = (float) (r.x - cx);
+                sy[lane] = (float) (r.y - cy);
+                sz[lane] = (float) (r.z - cz);
+                sgm[lane] = (float) (kG * r.w);
             }
             __syncthreads();
             float px = 0.f, py = 0.f, pz = 0.f, pp = 0.f;

@@ -341,6 +341,7 @@ __global__ void k_node_ranges(int64_t n, const int32_t *info, int32_t *nstart, i
 struct MomArgs {
     const int32_t *nstart, *nend, *c0, *c1;
     double *mass, *mx, *my, *mz, *comx, *comy, *comz, *lox, *loy, *loz, *hix, *hiy, *hiz, *maxH;
+    double4 *rec;  // (com, mass) records
 };
 
 __global__ void k_moments_leaf(const int32_t *info, MomArgs a, const double *x, const double *y, const double *z,
@@ -362,6 +363,7 @@ __global__ void k_moments_leaf(const int32_t *info, MomArgs a, const double *x, 
     a.mass[id] = M;
     a.mx[id] = sx; a.my[id] = sy; a.mz[id] = sz;
     a.comx[id] = sx / M; a.comy[id] = sy / M; a.comz[id] = sz / M;
+    a.rec[id] = make_double4(sx / M, sy / M, sz / M, M);
     a.lox[id] = lo[0]; a.loy[id] = lo[1]; a.loz[id] = lo[2];
     a.hix[id] = hi[0]; a.hiy[id] = hi[1]; a.hiz[id] = hi[2];
     a.maxH[id] = mh;
@@ -385,6 +387,7 @@ __global__ void k_moments_level(int l, const int32_t *info, MomArgs a) {
     a.mass[id] = M;
     a.mx[id] = sx; a.my[id] = sy; a.mz[id] = sz;
     a.comx[id] = sx / M; a.comy[id] = sy / M; a.comz[id] = sz / M;
+    a.rec[id] = make_double4(sx / M, sy / M, sz / M, M);
     a.lox[id] = lo[0]; a.loy[id] = lo[1]; a.loz[id] = lo[2];
     a.hix[id] = hi[0]; a.hiy[id] = hi[1]; a.hiz[id] = hi[2];
     a.maxH[id] = mh;
@@ -1117,7 +1120,7 @@ static void build_levels(Tree &t, const TreeParams &tp, int64_t n, const PArrays
     t.mass.ensure(nn); t.mx.ensure(nn); t.my.ensure(nn); t.mz.ensure(nn);
     t.comx.ensure(nn); t.comy.ensure(nn); t.comz.ensure(nn);
     t.lox.ensure(nn); t.loy.ensure(nn); t.loz.ensure(nn);
-    t.hix.ensure(nn); t.hiy.ensure(nn); t.hiz.ensure(nn); t.maxH.ensure(nn);
+    t.hix.ensure(nn); t.hiy.ensure(nn); t.hiz.ensure(nn); t.maxH.ensure(nn); t.nrec.ensure(nn);
     {  // one launch per job for all levels
         const dim3 grid((unsigned) std::min<int64_t>(nblk(cmax, 256), 1024), (unsigned) L);
         hipLaunchKernelGGL(k_copy_levels, grid, dim3(256), 0, s, n, info, t.sel_out.get(), t.nstart.get());
@@ -1128,7 +1131,7 @@ static void build_levels(Tree &t, const TreeParams &tp, int64_t n, const PArrays
     }
     MomArgs ma{t.nstart.get(), t.nend.get(), t.c0.get(), t.c1.get(), t.mass.get(), t.mx.get(), t.my.get(), t.mz.get(),
                t.comx.get(), t.comy.get(), t.comz.get(), t.lox.get(), t.loy.get(), t.loz.get(), t.hix.get(),
-               t.hiy.get(), t.hiz.get(), t.maxH.get()};
+               t.hiy.get(), t.hiz.get(), t.maxH.get(), t.nrec.get()};
     hipLaunchKernelGGL(k_moments_leaf, dim3(nblk(t.cap_lv[0], 128)), dim3(128), 0, s, info, ma, P.x, P.y, P.z, P.m,
                        P.h);
     SPH_LAUNCH_CHECK();
