@@ -448,27 +448,19 @@ __device__ __forceinline__ void hydro_group(const HydroArgs &a, const int g) {
         const uint64_t cand = __ballot(k < tl.nj && !(bj < T));
         __syncthreads();
         const int nt = tl.count(t);
-        // pass 1, every entry that may matter: v_sig_max.  v_sig = (c_i + c_j) - 3w
-        // with |w| = |dv.dr|/|dr| <= |dv|; when even that bound stays below the
-        // current maximum (with a margin far above the rounding of either side)
-        // the square root and division are skipped -- the maximum cannot change.
+        // pass 1, every entry that may matter: v_sig_max, with w = dv.dr / |dr|
+        // when negative (else 0) formed as in pass 2.  Evaluated for every lane
+        // without branches: the earlier form, which skipped the square root
+        // where a bound on |w| kept the maximum, spent more on its divergent
+        // branches than it saved (FP64 hydro 1.96 -> 1.69 ms at 1M).
         for (uint64_t cm = cand; cm; cm &= cm - 1) {
             const int jj = __builtin_ctzll(cm);
             const HydroJ &J = sj[jj].r;
             const double drx = xi - J.x, dry = yi - J.y, drz = zi - J.z;
             const double dvx = vxi - J.vx, dvy = vyi - J.vy, dvz = vzi - J.vz;
-            const double dvdr = dvx * drx + dvy * dry + dvz * drz;
-            const double cij = csi + J.cs;
-            if (dvdr < 0) {
-                const double room = vmax - cij;
-                const double dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
-                if (!(room > 1.0e-4 * vmax && 9.0 * dv2 < (room * room) * (1.0 - 1.0e-10))) {
-                    const double w = dvdr * rsqrt_(norm2_(drx, dry, drz));  // dv.dr < 0: r2 > 0
-                    vmax = dmax(vmax, fma(-3.0, w, cij));
-                }
-            } else {
-                vmax = dmax(vmax, cij - 3.0 * 0.0);
-            }
+            const double dvdr = fma(dvz, drz, fma(dvy, dry, dvx * drx));
+            const double w = fmin(dvdr, 0.0) * dist_(norm2_(drx, dry, drz)).rinv;
+            vmax = dmax(vmax, fma(-3.0, w, csi + J.cs));
         }
         // pass 2: the force terms of the pairs inside either support.  Both
         // gradients are multiples of dr: gradW_i = K F_i dr with K = -22 and
