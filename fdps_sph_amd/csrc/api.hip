@@ -188,6 +188,7 @@ struct SimState {
     DevBuf<long long> kcount_try;  // the same for the force evaluation in progress
     DevBuf<double> hsave;      // h at the start of a force evaluation (for its redo)
     DevBuf<HydroRec> hrec;     // j records of the hydro pass
+    DevBuf<HydroRecF> hrecf;   // the same in the precision form
     DevBuf<double4> kpos, kvel; // j records of the density and derivative passes
     int64_t gather_builds = 0; // gather-list constructions since init
     int64_t force_evals = 0;   // force evaluations completed since init
@@ -818,12 +819,20 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
     }
     rec_ev(6);
     {
-        HydroRec *rec = S.hrec.ensure(n);
-        launch_pack_hydro(n, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.m, P.rho, P.cs, P.bal, P.gs, P.p, P.h, rec, s);
+        HydroRec *rec = nullptr;
+        HydroRecF *recf = nullptr;
+        if (mixed) {
+            recf = S.hrecf.ensure(n);
+            launch_pack_hydro_f(n, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.m, P.rho, P.cs, P.bal, P.gs, P.p, P.h, recf, s);
+        } else {
+            rec = S.hrec.ensure(n);
+            launch_pack_hydro(n, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.m, P.rho, P.cs, P.bal, P.gs, P.p, P.h, rec, s);
+        }
         await(s, S.xev[3]);
         HydroArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal,
                     P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal, P.m,
                     lref(), P.ax, P.ay, P.az, P.du, P.dti, rec};
+        a.jrecf = recf;
         a.sched = sched_for(4);
         a.gcount = gcount;
         kbegin(&L, false, 2);

@@ -173,6 +173,16 @@ struct alignas(16) HydroRec {
     double x, y, z, vx, vy, vz, m, rho, cs, bal, gs, A, reg, H, norm, invH;
 };
 static_assert(sizeof(HydroRec) == 128, "one cache line per record");
+// The precision form's hydro j record: position and velocity in FP64 (the
+// kernel forms their offsets from the group's centres), the other fields
+// converted to FP32 once per particle from the same FP64 values as HydroRec's.
+struct alignas(16) HydroRecF {
+    double x, y, z, vx, vy, vz;
+    float4 k;  // rho, Balsara switch, 1/H, H
+    float4 w;  // kernel normalisation, 1e-6 h, grad-h term f, mass
+    float4 c;  // sound speed, A = f P / rho^2, 0, 0
+};
+static_assert(sizeof(HydroRecF) == 96, "record stride");
 
 struct HydroArgs {
     int ngroups;
@@ -182,6 +192,7 @@ struct HydroArgs {
     ListRef L;
     double *ax, *ay, *az, *du, *dt;
     const HydroRec *jrec = nullptr;  // if set: the j side is read from here (list indices index it)
+    const HydroRecF *jrecf = nullptr;  // the precision form's records, read in preference to jrec
     // persistent scheduling: 8 per-XCD group counters, cleared before the launch
     // (null: one workgroup per group)
     int32_t *sched = nullptr;
@@ -228,6 +239,11 @@ void launch_pack_hydro(int64_t n, const double *x, const double *y, const double
                        const double *vy, const double *vz, const double *m, const double *rho, const double *cs,
                        const double *bal, const double *gs, const double *p, const double *h, HydroRec *out,
                        hipStream_t s);
+// The same particles' HydroRecF records (precision form).
+void launch_pack_hydro_f(int64_t n, const double *x, const double *y, const double *z, const double *vx,
+                         const double *vy, const double *vz, const double *m, const double *rho, const double *cs,
+                         const double *bal, const double *gs, const double *p, const double *h, HydroRecF *out,
+                         hipStream_t s);
 
 // FP64 (plain) interaction kernels -- kernels_fp64.hip
 void launch_dens_fp64(const DensArgs &a, hipStream_t s);

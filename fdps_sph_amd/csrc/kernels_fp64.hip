@@ -378,6 +378,22 @@ __global__ void k_pack_hydro(int64_t n, const double *x, const double *y, const 
     if (k < n) out[k] = make_hydro_rec(x[k], y[k], z[k], vx[k], vy[k], vz[k], m[k], rho[k], cs[k], bal[k], gs[k],
                                        p[k], h[k]);
 }
+__global__ void k_pack_hydro_f(int64_t n, const double *x, const double *y, const double *z, const double *vx,
+                               const double *vy, const double *vz, const double *m, const double *rho,
+                               const double *cs, const double *bal, const double *gs, const double *p,
+                               const double *h, HydroRecF *out) {
+    const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const HydroRec r = make_hydro_rec(x[k], y[k], z[k], vx[k], vy[k], vz[k], m[k], rho[k], cs[k], bal[k], gs[k], p[k],
+                                      h[k]);
+    HydroRecF f;
+    f.x = r.x; f.y = r.y; f.z = r.z;
+    f.vx = r.vx; f.vy = r.vy; f.vz = r.vz;
+    f.k = make_float4((float) r.rho, (float) r.bal, (float) r.invH, (float) r.H);
+    f.w = make_float4((float) r.norm, (float) r.reg, (float) r.gs, (float) r.m);
+    f.c = make_float4((float) r.cs, (float) r.A, 0.f, 0.f);
+    out[k] = f;
+}
 // 4 waves per SIMD (128 registers, seven spilled outside the pair loops):
 // measured 2.09 ms at 1M against 2.14 ms at the compiler's 3 waves.
 __device__ __forceinline__ void hydro_group(const HydroArgs &a, const int g) {
@@ -646,6 +662,15 @@ void launch_pack_hydro(int64_t n, const double *x, const double *y, const double
     if (n <= 0) return;
     hipLaunchKernelGGL(k_pack_hydro, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, n, x, y, z, vx, vy, vz, m,
                        rho, cs, bal, gs, p, h, out);
+    SPH_LAUNCH_CHECK();
+}
+void launch_pack_hydro_f(int64_t n, const double *x, const double *y, const double *z, const double *vx,
+                         const double *vy, const double *vz, const double *m, const double *rho, const double *cs,
+                         const double *bal, const double *gs, const double *p, const double *h, HydroRecF *out,
+                         hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_pack_hydro_f, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, n, x, y, z, vx, vy, vz,
+                       m, rho, cs, bal, gs, p, h, out);
     SPH_LAUNCH_CHECK();
 }
 namespace {

@@ -208,24 +208,39 @@ __device__ __forceinline__ void drvt_mixed_group(const DrvtArgs &a, const int g)
     __shared__ float4 so[kWave];  // FP32 offset from the centre, |offset|^2
     __shared__ float sm[kWave], svx[kWave], svy[kWave], svz[kWave];
     const Tiles tl(nj);
+    // software-pipelined tiles: the j records of tile t+1 and the list index of
+    // tile t+2 are in flight while tile t is evaluated (a lane without an entry
+    // loads particle 0, which exists whenever the list is not empty)
+    auto load = [&](int q, double4 &pq, double4 &vq) {
+        if (a.jpos) {
+            pq = a.jpos[q];
+            vq = a.jvel[q];
+        } else {
+            pq = make_double4(a.xj[q], a.yj[q], a.zj[q], a.mj[q]);
+            vq = make_double4(a.vxj[q], a.vyj[q], a.vzj[q], 0.0);
+        }
+    };
+    double4 pq = make_double4(0.0, 0.0, 0.0, 0.0), vq = pq;
+    int qn = 0;
+    if (tl.T > 0) {
+        const int k0 = tl.entry(0, lane), k1 = tl.entry(1, lane);
+        load(k0 < nj ? a.L.idx[lb + k0] : 0, pq, vq);
+        qn = (tl.T > 1 && k1 < nj) ? a.L.idx[lb + k1] : 0;
+    }
     for (int t = 0; t < tl.T; ++t) {
         const int k = tl.entry(t, lane);
         if (k < nj) {
-            const int q = a.L.idx[lb + k];
-            double4 pq, vq;
-            if (a.jpos) {
-                pq = a.jpos[q];
-                vq = a.jvel[q];
-            } else {
-                pq = make_double4(a.xj[q], a.yj[q], a.zj[q], a.mj[q]);
-                vq = make_double4(a.vxj[q], a.vyj[q], a.vzj[q], 0.0);
-            }
             sm[lane] = (float) pq.w;
             const float ox = (float) (pq.x - cx), oy = (float) (pq.y - cy), oz = (float) (pq.z - cz);
             so[lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
             svx[lane] = (float) (vq.x - cvx);
             svy[lane] = (float) (vq.y - cvy);
             svz[lane] = (float) (vq.z - cvz);
+        }
+        if (t + 1 < tl.T) {
+            load(qn, pq, vq);
+            const int k2 = tl.entry(t + 2, lane);
+            qn = (t + 2 < tl.T && k2 < nj) ? a.L.idx[lb + k2] : 0;
         }
         __syncthreads();
         const int nt = tl.count(t);
@@ -337,36 +352,55 @@ __device__ __forceinline__ void hydro_mixed_group(const HydroArgs &a, const int 
     __shared__ HydroSlotF slot[kWave];
     __shared__ float sQ[kWave];
     const Tiles tl(nj);
+    // the list index of tile t+1 is in flight while tile t is evaluated
+    int qn = (tl.T > 0 && tl.entry(0, lane) < nj) ? a.L.idx[lb + tl.entry(0, lane)] : 0;
     for (int t = 0; t < tl.T; ++t) {
         const int k = tl.entry(t, lane);
         float bnd = 0.f, nj1 = 0.f;
+        const int q = qn;
+        if (t + 1 < tl.T && tl.entry(t + 1, lane) < nj) qn = a.L.idx[lb + tl.entry(t + 1, lane)];
         if (k < nj) {
-            const int q = a.L.idx[lb + k];
-            double x, y, z, vx, vy, vz, m, rj, cj, bj, gj, Aj, regj, Hj, nj3;
-            if (a.jrec) {  // one 128-byte record per entry
-                const HydroRec r = a.jrec[q];
-                x = r.x; y = r.y; z = r.z; vx = r.vx; vy = r.vy; vz = r.vz; m = r.m;
-                rj = r.rho; cj = r.cs; bj = r.bal; gj = r.gs; Aj = r.A; regj = r.reg; Hj = r.H; nj3 = r.norm;
+            double x, y, z, vx, vy, vz;
+            float4 ek, ew;  // (rho, Balsara, 1/H, H), (kernel normalisation, 1e-6 h, grad-h term, mass)
+            float cjf, Ajf;
+            if (a.jrecf) {  // one 96-byte record per entry, its FP32 fields converted once per particle
+                const HydroRecF r = a.jrecf[q];
+                x = r.x; y = r.y; z = r.z; vx = r.vx; vy = r.vy; vz = r.vz;
+                ek = r.k;
+                ew = r.w;
+                cjf = r.c.x;
+                Ajf = r.c.y;
             } else {
-                x = a.xj[q]; y = a.yj[q]; z = a.zj[q]; vx = a.vxj[q]; vy = a.vyj[q]; vz = a.vzj[q]; m = a.mj[q];
-                rj = a.rhoj[q]; cj = a.csj[q]; bj = a.balj[q]; gj = a.gsj[q];
-                const double hj = a.hj[q];
-                regj = 1.0e-6 * hj;
-                Aj = gj * a.pj[q] / (rj * rj);
-                Hj = kSupport * hj;
-                nj3 = kNormD / (Hj * Hj * Hj);
+                double m, rj, cj, bj, gj, Aj, regj, Hj, nj3;
+                if (a.jrec) {  // one 128-byte record per entry
+                    const HydroRec r = a.jrec[q];
+                    x = r.x; y = r.y; z = r.z; vx = r.vx; vy = r.vy; vz = r.vz; m = r.m;
+                    rj = r.rho; cj = r.cs; bj = r.bal; gj = r.gs; Aj = r.A; regj = r.reg; Hj = r.H; nj3 = r.norm;
+                } else {
+                    x = a.xj[q]; y = a.yj[q]; z = a.zj[q]; vx = a.vxj[q]; vy = a.vyj[q]; vz = a.vzj[q]; m = a.mj[q];
+                    rj = a.rhoj[q]; cj = a.csj[q]; bj = a.balj[q]; gj = a.gsj[q];
+                    const double hj = a.hj[q];
+                    regj = 1.0e-6 * hj;
+                    Aj = gj * a.pj[q] / (rj * rj);
+                    Hj = kSupport * hj;
+                    nj3 = kNormD / (Hj * Hj * Hj);
+                }
+                ek = make_float4((float) rj, (float) bj, (float) (1.0 / Hj), (float) Hj);
+                ew = make_float4((float) nj3, (float) regj, (float) gj, (float) m);
+                cjf = (float) cj;
+                Ajf = (float) Aj;
             }
             const float ox = (float) (x - cx), oy = (float) (y - cy), oz = (float) (z - cz);
             nj1 = norm1_(ox, oy, oz);
-            sQ[lane] = dot_cut_j_((float) Hj * 1.0001f, nj1, n1max);
+            sQ[lane] = dot_cut_j_(ek.w * 1.0001f, nj1, n1max);
             const float ux = (float) (vx - cvx), uy = (float) (vy - cvy), uz = (float) (vz - cvz);
-            bnd = (float) cj + 3.f * sqrtf(ux * ux + uy * uy + uz * uz);
+            bnd = cjf + 3.f * sqrtf(ux * ux + uy * uy + uz * uz);
             HydroSlotF e;
             e.o = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
-            e.v = make_float4(ux, uy, uz, (float) cj);
-            e.k = make_float4((float) rj, (float) bj, (float) (1.0 / Hj), (float) Hj);
-            e.w = make_float4((float) nj3, (float) regj, (float) gj, (float) m);
-            e.a = make_float4((float) Aj, 0.f, 0.f, 0.f);
+            e.v = make_float4(ux, uy, uz, cjf);
+            e.k = ek;
+            e.w = ew;
+            e.a = make_float4(Ajf, 0.f, 0.f, 0.f);
             slot[lane] = e;
         }
         const float nj1max = dpp_wave_maxf(nj1);  // every lane takes part: outside the condition
