@@ -415,16 +415,24 @@ __device__ __forceinline__ void hydro_mixed_group(const HydroArgs &a, const int 
         __syncthreads();
         const int nt = tl.count(t);
         float px = 0.f, py = 0.f, pz = 0.f, pu = 0.f;
-        // pass 1: v_sig_max over the candidates
-        for (uint64_t cm = cand; cm; cm &= cm - 1) {
-            const int jj = __builtin_ctzll(cm);
+        // pass 1: v_sig_max over the candidates, two per pass (two chains in
+        // flight; an odd last one is taken twice): 1.11 -> 1.03 ms at 1M
+        auto vsig = [&](int jj) {
             const float4 oj = slot[jj].o, vj = slot[jj].v;
             const float drx = xi - oj.x, dry = yi - oj.y, drz = zi - oj.z;
             const float dvx = vxi - vj.x, dvy = vyi - vj.y, dvz = vzi - vj.z;
             const float dvdr = dvx * drx + dvy * dry + dvz * drz;
             const float r2 = drx * drx + dry * dry + drz * drz;
             const float w = (dvdr < 0.f) ? dvdr * rsq(r2 + 1e-30f) : 0.f;
-            vmax = fmaxf(vmax, csi + vj.w - 3.f * w);
+            return csi + vj.w - 3.f * w;
+        };
+        for (uint64_t cm = cand; cm;) {
+            const int j1 = __builtin_ctzll(cm);
+            cm &= cm - 1;
+            const int j2 = cm ? __builtin_ctzll(cm) : j1;
+            if (cm) cm &= cm - 1;
+            const float v1 = vsig(j1), v2 = vsig(j2);
+            vmax = fmaxf(vmax, fmaxf(v1, v2));
         }
         // the pairs inside either support (a superset: pairs beyond both give zero terms)
         const uint64_t mask = mfma_dot_mask_sym([&](int e) { return slot[e].o; }, [&](int e) { return sQ[e]; }, nt, tx,
