@@ -15,8 +15,10 @@ knobs of this implementation:
 * walk_waves: the persistent waves of each list walk, which runs beside the
   force kernels and shares the GPU with them.
 
-On this device the i-group is one 64-lane wavefront (one lane per
-i-particle), so n_group ranges over sizes up to the wave width.  The search
+A 64-lane wavefront holds up to 64 i-particles, one per lane; a larger
+i-group (n_group 128 .. 1024, FDPS's own range) runs as one work item per
+wave of its particles, the waves sharing the group's lists: fewer, longer
+lists and fewer walks, against more pairs outside the supports.  The search
 is staged: every (n_group, n_leaf) pair first, then the launch form and the
 LDS frontier at the best pair, then the walks' waves.  Each candidate builds
 a Simulation from the same initial state, runs `warmup` steps untimed and
@@ -29,7 +31,7 @@ from dataclasses import dataclass, field
 
 from .sph import Context, Simulation
 
-N_GROUP_CHOICES = (32, 48, 64)
+N_GROUP_CHOICES = (32, 64, 128, 256, 512, 1024)
 N_LEAF_CHOICES = (8, 16, 32)
 # persistent-launch bitmask: none, gravity only, every interaction kernel
 PERSISTENT_CHOICES = (0, 8, 15)

@@ -701,6 +701,8 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
     rec_ev(2);
     PArrays &P = S.P;
     const int G = (int) S.tree.gcap;  // launches are sized by the capacity, the kernels read the count
+    // a group of more than 64 particles runs as one work item per wave of them
+    const int chunks = (S.tp.n_group + kWave - 1) / kWave;
     const int32_t *gs = S.tree.gstart.get();
     const int32_t *gcount = S.tree.info.get() + kInfoG;
     double4 *kpos = S.kpos.ensure(n), *kvel = S.kvel.ensure(n);  // j records of the density / derivative passes
@@ -762,6 +764,8 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
         DensArgs a{G, gs, P.x, P.y, P.z, P.m, P.h, P.m, P.x, P.y, P.z, lref(), P.rho, P.h, P.ngb, kpos};
         a.sched = sched_for(1);
         a.gcount = gcount;
+        a.ngroups = G * chunks;
+        a.chunks = chunks;
         a.sbox = L.sbox.get();
         a.crad = L.crad.get();
         a.flag = gflag + loop;
@@ -788,6 +792,8 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
                    lref(), P.divv, P.rotx, P.roty, P.rotz, P.gs, P.bal, kpos, kvel};
         a.sched = sched_for(2);
         a.gcount = gcount;
+        a.ngroups = G * chunks;
+        a.chunks = chunks;
         kbegin(&L, false, 1);
         if (mixed) launch_drvt_mixed(a, s);
         else launch_drvt_fp64(a, s);
@@ -810,6 +816,8 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
                    P.gx, P.gy, P.gz, P.pot};
         a.sched = sched_for(8);
         a.gcount = gcount;
+        a.ngroups = G * chunks;
+        a.chunks = chunks;
         a.jrec = kpos;
         a.srec = t.nrec.get();
         kbegin(&Lg, true, 3);
@@ -835,6 +843,8 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
         a.jrecf = recf;
         a.sched = sched_for(4);
         a.gcount = gcount;
+        a.ngroups = G * chunks;
+        a.chunks = chunks;
         kbegin(&L, false, 2);
         if (mixed) launch_hydro_mixed(a, s);
         else launch_hydro_fp64(a, s);
@@ -1334,8 +1344,10 @@ int32_t sphgpu_sim_init(sphgpu_ctx *c, const sphgpu_sim_params *prm, int64_t n, 
         if (!prm) throw Error(SPHGPU_ERR_ARG, "null params");
         if (n <= 0 || n > INT32_MAX / 2) throw Error(SPHGPU_ERR_ARG, "particle count out of range");
         if (!pos || !vel || !mass || !dens || !eng || !material) throw Error(SPHGPU_ERR_ARG, "null particle array");
-        if (prm->n_leaf < 1 || prm->n_leaf > kWave || prm->n_group < 1 || prm->n_group > kWave)
-            throw Error(SPHGPU_ERR_ARG, "n_leaf and n_group must be in [1, 64]");
+        if (prm->n_leaf < 1 || prm->n_leaf > kWave)
+            throw Error(SPHGPU_ERR_ARG, "n_leaf must be in [1, 64]");
+        if (prm->n_group < 1 || prm->n_group > SPHGPU_MAX_GROUP)
+            throw Error(SPHGPU_ERR_ARG, "n_group must be in [1, SPHGPU_MAX_GROUP]");
         if (!(prm->theta >= 0)) throw Error(SPHGPU_ERR_ARG, "theta must be >= 0");
         const int nsys = prm->nsys > 0 ? prm->nsys : 1;
         if (nsys > 1 && !system) throw Error(SPHGPU_ERR_ARG, "nsys > 1 needs a system array");
@@ -1369,6 +1381,7 @@ int32_t sphgpu_sim_init(sphgpu_ctx *c, const sphgpu_sim_params *prm, int64_t n, 
         TreeParams &tp = S->tp;
         tp.n_leaf = prm->n_leaf;
         tp.n_group = prm->n_group;
+        S->tree.max_group = prm->n_group;
         tp.nsys = nsys;
         tp.sysbits = sb;
         tp.bits = std::min(21, (63 - sb) / 3);

@@ -139,6 +139,9 @@ struct DensArgs {
     // if set: the group count is read here (a tree build's device count, at
     // most ngroups, which then only sizes the launch)
     const int32_t *gcount = nullptr;
+    // work items per group: a group of up to 64 * chunks particles runs as
+    // `chunks` one-wave items sharing its list (ngroups then counts items)
+    int chunks = 1;
     // if flag is set: after the pass, whether each group's gather list still
     // covers the new supports (gather_cover_check against the list's search
     // box sbox and, if given, its compaction radius crad); *flag |= 1 if not
@@ -162,6 +165,9 @@ struct DrvtArgs {
     // if set: the group count is read here (a tree build's device count, at
     // most ngroups, which then only sizes the launch)
     const int32_t *gcount = nullptr;
+    // work items per group: a group of up to 64 * chunks particles runs as
+    // `chunks` one-wave items sharing its list (ngroups then counts items)
+    int chunks = 1;
 };
 
 // The j side of the hydro pass as one 128-byte record per particle (one
@@ -199,6 +205,9 @@ struct HydroArgs {
     // if set: the group count is read here (a tree build's device count, at
     // most ngroups, which then only sizes the launch)
     const int32_t *gcount = nullptr;
+    // work items per group: a group of up to 64 * chunks particles runs as
+    // `chunks` one-wave items sharing its list (ngroups then counts items)
+    int chunks = 1;
 };
 
 struct GravArgs {
@@ -221,6 +230,9 @@ struct GravArgs {
     // if set: the group count is read here (a tree build's device count, at
     // most ngroups, which then only sizes the launch)
     const int32_t *gcount = nullptr;
+    // work items per group: a group of up to 64 * chunks particles runs as
+    // `chunks` one-wave items sharing its list (ngroups then counts items)
+    int chunks = 1;
 };
 
 // Pack the j records of the density and derivative passes for particles
@@ -317,6 +329,7 @@ struct Tree {
     int32_t cap_lv[kMaxLevels] = {0};
     int64_t cap_nodes = 0;
     int64_t gcap = 0;
+    int max_group = 64;  // the most particles a group holds (n_group)
     // nodes (levels flattened, level 0 = leaves); the counts on the host are
     // those of the last tree_apply_info
     int nlevels = 0;
@@ -437,7 +450,12 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
 // The group count of an interaction launch: a device count when the args
 // carry one (the grid is then sized by a capacity), else the launch's own.
 template <class A>
-__device__ __forceinline__ int group_count(const A &a) { return a.gcount ? *a.gcount : a.ngroups; }
+__device__ __forceinline__ int group_count(const A &a) { return a.gcount ? *a.gcount * a.chunks : a.ngroups; }
+// The group of work item k: a group of more than 64 particles is processed as
+// `chunks` items of one wave each (items g*chunks .. g*chunks + chunks - 1),
+// which share the group's list.
+template <class A>
+__device__ __forceinline__ int group_of(const A &a, int k) { return a.chunks == 1 ? k : k / a.chunks; }
 
 // Work distribution of the interaction kernels over the i-groups.  Without a
 // scheduler each workgroup takes one group through xcd_remap.  With one (8

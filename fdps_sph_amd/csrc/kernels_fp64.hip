@@ -131,9 +131,11 @@ __device__ __forceinline__ float wave_maxf_(float v) { return dpp_wave_maxf(v); 
 // measured faster than the compiler's own choice (4 waves); the kernel now fits
 // 71 registers (7 waves).  A cap for 8 waves (64 registers, two spilled)
 // measured slower: 0.508 -> 0.516 ms at 1M.
-__device__ __forceinline__ void dens_group(const DensArgs &a, const int g) {
+__device__ __forceinline__ void dens_group(const DensArgs &a, const int item) {
     const int lane = threadIdx.x;
-    const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
+    const int g = group_of(a, item);
+    const int i0 = a.gstart[g] + kWave * (item - g * a.chunks), ni = min(kWave, a.gstart[g + 1] - i0);
+    if (ni <= 0) return;  // a chunk beyond the group's particles
     const bool act = lane < ni;
     const int ip = i0 + (act ? lane : 0);
     const double xi = a.xi[ip], yi = a.yi[ip], zi = a.zi[ip], hi = a.hi[ip], mi = a.mi[ip];
@@ -218,9 +220,11 @@ __global__ __launch_bounds__(kWave, 5) void k_dens_persistent(DensArgs a) {
 // ---------------------------------------------------------------- derivative
 // CalcDerivative::operator() (src/force.h:89-106) + Balsara switch of
 // RealPtcl::copyFromForce (src/class.h:132-142) when csi is given.
-__device__ __forceinline__ void drvt_group(const DrvtArgs &a, const int g) {
+__device__ __forceinline__ void drvt_group(const DrvtArgs &a, const int item) {
     const int lane = threadIdx.x;
-    const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
+    const int g = group_of(a, item);
+    const int i0 = a.gstart[g] + kWave * (item - g * a.chunks), ni = min(kWave, a.gstart[g + 1] - i0);
+    if (ni <= 0) return;  // a chunk beyond the group's particles
     const bool act = lane < ni;
     const int ip = i0 + (act ? lane : 0);
     const double xi = a.xi[ip], yi = a.yi[ip], zi = a.zi[ip];
@@ -396,9 +400,11 @@ __global__ void k_pack_hydro_f(int64_t n, const double *x, const double *y, cons
 }
 // 4 waves per SIMD (128 registers, seven spilled outside the pair loops):
 // measured 2.09 ms at 1M against 2.14 ms at the compiler's 3 waves.
-__device__ __forceinline__ void hydro_group(const HydroArgs &a, const int g) {
+__device__ __forceinline__ void hydro_group(const HydroArgs &a, const int item) {
     const int lane = threadIdx.x;
-    const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
+    const int g = group_of(a, item);
+    const int i0 = a.gstart[g] + kWave * (item - g * a.chunks), ni = min(kWave, a.gstart[g + 1] - i0);
+    if (ni <= 0) return;  // a chunk beyond the group's particles
     const bool act = lane < ni;
     const int ip = i0 + (act ? lane : 0);
     const double xi = a.xi[ip], yi = a.yi[ip], zi = a.zi[ip];
@@ -604,9 +610,11 @@ __device__ __forceinline__ double4 grav_load_(const double4 *rec, const double *
     return rec ? rec[q] : make_double4(X[q], Y[q], Z[q], M[q]);
 }
 
-__device__ __forceinline__ void grav_group(const GravArgs &a, const int g) {
+__device__ __forceinline__ void grav_group(const GravArgs &a, const int item) {
     const int lane = threadIdx.x;
-    const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
+    const int g = group_of(a, item);
+    const int i0 = a.gstart[g] + kWave * (item - g * a.chunks), ni = min(kWave, a.gstart[g + 1] - i0);
+    if (ni <= 0) return;  // a chunk beyond the group's particles
     const bool act = lane < ni;
     const int ip = i0 + (act ? lane : 0);
     const double xi = a.xi[ip], yi = a.yi[ip], zi = a.zi[ip];

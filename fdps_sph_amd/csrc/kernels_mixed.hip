@@ -89,9 +89,11 @@ __device__ __forceinline__ void load_xyzm(const DensArgs &a, int q, double &x, d
 
 // ---------------------------------------------------------------- density
 // At least 5 waves per SIMD (no spills; measured faster than 4).
-__device__ __forceinline__ void dens_mixed_group(const DensArgs &a, const int g) {
+__device__ __forceinline__ void dens_mixed_group(const DensArgs &a, const int item) {
     const int lane = threadIdx.x;
-    const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
+    const int g = group_of(a, item);
+    const int i0 = a.gstart[g] + kWave * (item - g * a.chunks), ni = min(kWave, a.gstart[g + 1] - i0);
+    if (ni <= 0) return;  // a chunk beyond the group's particles
     const bool act = lane < ni;
     const int ip = i0 + (act ? lane : 0);
     const double xd = a.xi[ip], yd = a.yi[ip], zd = a.zi[ip];
@@ -183,9 +185,11 @@ __global__ __launch_bounds__(kWave, 5) void k_dens_mixed_persistent(DensArgs a) 
 }
 
 // ---------------------------------------------------------------- derivative
-__device__ __forceinline__ void drvt_mixed_group(const DrvtArgs &a, const int g) {
+__device__ __forceinline__ void drvt_mixed_group(const DrvtArgs &a, const int item) {
     const int lane = threadIdx.x;
-    const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
+    const int g = group_of(a, item);
+    const int i0 = a.gstart[g] + kWave * (item - g * a.chunks), ni = min(kWave, a.gstart[g + 1] - i0);
+    if (ni <= 0) return;  // a chunk beyond the group's particles
     const bool act = lane < ni;
     const int ip = i0 + (act ? lane : 0);
     const double xd = a.xi[ip], yd = a.yi[ip], zd = a.zi[ip];
@@ -318,9 +322,11 @@ static_assert(sizeof(HydroSlotF) == 80, "LDS record stride");
 
 // 4 waves per SIMD: at 5 waves (96 registers) 27 spill, and the kernel took
 // 1.33 ms at 1M against 1.20 ms at 4 waves (124 registers).
-__device__ __forceinline__ void hydro_mixed_group(const HydroArgs &a, const int g) {
+__device__ __forceinline__ void hydro_mixed_group(const HydroArgs &a, const int item) {
     const int lane = threadIdx.x;
-    const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
+    const int g = group_of(a, item);
+    const int i0 = a.gstart[g] + kWave * (item - g * a.chunks), ni = min(kWave, a.gstart[g + 1] - i0);
+    if (ni <= 0) return;  // a chunk beyond the group's particles
     const bool act = lane < ni;
     const int ip = i0 + (act ? lane : 0);
     const double xd = a.xi[ip], yd = a.yi[ip], zd = a.zi[ip];
@@ -517,9 +523,11 @@ __device__ __forceinline__ void grav_tile_mixed(const float *sx, const float *sy
     }
 }
 
-__device__ __forceinline__ void grav_mixed_group(const GravArgs &a, const int g) {
+__device__ __forceinline__ void grav_mixed_group(const GravArgs &a, const int item) {
     const int lane = threadIdx.x;
-    const int i0 = a.gstart[g], ni = a.gstart[g + 1] - i0;
+    const int g = group_of(a, item);
+    const int i0 = a.gstart[g] + kWave * (item - g * a.chunks), ni = min(kWave, a.gstart[g + 1] - i0);
+    if (ni <= 0) return;  // a chunk beyond the group's particles
     const bool act = lane < ni;
     const int ip = i0 + (act ? lane : 0);
     const double xd = a.xi[ip], yd = a.yi[ip], zd = a.zi[ip];

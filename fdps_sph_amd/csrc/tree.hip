@@ -571,20 +571,34 @@ struct GBox {
     double r2;  // gather lists: (largest search radius)^2 (1 + 1e-12), the reach beyond the group's box
 };
 
+// (a group of more than 64 particles is read 64 at a time)
 __device__ __forceinline__ void group_box(const ListArgs &a, int i0, int ni, GBox &b) {
     const int lane = threadIdx.x;
-    const bool act = lane < ni;
-    const int p = i0 + (act ? lane : 0);
-    const double c[3] = {a.x[p], a.y[p], a.z[p]};
-    const double H = a.h ? kSupport * a.h[p] * a.hfac : 0.0;  // the gravity walk reads no h
+    double glo[3] = {INFINITY, INFINITY, INFINITY}, ghi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double slo[3] = {INFINITY, INFINITY, INFINITY}, shi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    double R = 0.0;
+    for (int base = 0; base < ni; base += kWave) {
+        if (base + lane >= ni) continue;
+        const int p = i0 + base + lane;
+        const double c[3] = {a.x[p], a.y[p], a.z[p]};
+        const double H = a.h ? kSupport * a.h[p] * a.hfac : 0.0;  // the gravity walk reads no h
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            glo[k] = dmin(glo[k], c[k]);
+            ghi[k] = dmax(ghi[k], c[k]);
+            slo[k] = dmin(slo[k], c[k] - H);
+            shi[k] = dmax(shi[k], c[k] + H);
+        }
+        R = dmax(R, H);
+    }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        b.glo[k] = wave_min(act ? c[k] : INFINITY);
-        b.ghi[k] = wave_max(act ? c[k] : -INFINITY);
-        b.slo[k] = wave_min(act ? c[k] - H : INFINITY);
-        b.shi[k] = wave_max(act ? c[k] + H : -INFINITY);
+        b.glo[k] = wave_min(glo[k]);
+        b.ghi[k] = wave_max(ghi[k]);
+        b.slo[k] = wave_min(slo[k]);
+        b.shi[k] = wave_max(shi[k]);
     }
-    const double R = wave_max(act ? H : 0.0);
+    R = wave_max(R);
     b.r2 = (R * R) * (1.0 + 1.0e-12);
 }
 
@@ -1041,6 +1055,75 @@ __global__ __launch_bounds__(kWave, 8) void k_list_compact(const int32_t *gcount
     if (lane == 0) cnt[g] = count;
 }
 
+// The same for groups of more than 64 particles: the group's radius and box
+// come from all its particles, read 64 at a time, and an entry is kept when
+// some particle of any of those waves admits it.
+__global__ __launch_bounds__(kWave) void k_list_compact_multi(const int32_t *gcount, const int32_t *gstart,
+                                                              const double *x, const double *y, const double *z,
+                                                              const double *h, const double4 *pos, double hfac,
+                                                              int32_t *idx, int32_t *cnt, int64_t cap, double *crad,
+                                                              const int32_t *need) {
+    if (need && *need == 0) return;
+    const int G = *gcount;
+    if ((int) blockIdx.x >= G) return;
+    const int g = xcd_remap((int) blockIdx.x, G);
+    const int lane = threadIdx.x;
+    const int i0 = gstart[g], ni = gstart[g + 1] - i0;
+    double Hm = 0.0, lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int b = 0; b < ni; b += kWave) {
+        if (b + lane >= ni) continue;
+        const int p = i0 + b + lane;
+        Hm = dmax(Hm, kSupport * h[p]);
+        lo[0] = dmin(lo[0], x[p]); hi[0] = dmax(hi[0], x[p]);
+        lo[1] = dmin(lo[1], y[p]); hi[1] = dmax(hi[1], y[p]);
+        lo[2] = dmin(lo[2], z[p]); hi[2] = dmax(hi[2], z[p]);
+    }
+    const double R = hfac * wave_max(Hm);
+    if (lane == 0) crad[g] = R;
+    const double cx = 0.5 * (wave_min(lo[0]) + wave_max(hi[0]));
+    const double cy = 0.5 * (wave_min(lo[1]) + wave_max(hi[1]));
+    const double cz = 0.5 * (wave_min(lo[2]) + wave_max(hi[2]));
+    const float Rc = (float) R * 1.0001f;
+    int32_t *list = idx + (int64_t) g * cap;
+    const int nj = cnt[g];
+    const uint64_t lt = (1ULL << lane) - 1;
+    __shared__ float4 so[kWave];
+    int count = 0;
+    for (int base = 0; base < nj; base += kWave) {
+        const int k = base + lane;
+        float n1 = 0.f;
+        int qk = 0;
+        if (k < nj) {
+            qk = list[k];
+            const double4 r = pos ? pos[qk] : make_double4(x[qk], y[qk], z[qk], 0.0);
+            const float ox = (float) (r.x - cx), oy = (float) (r.y - cy), oz = (float) (r.z - cz);
+            so[lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
+            n1 = norm1_(ox, oy, oz);
+        }
+        const float n1max = wave_maxf(n1);  // every lane takes part: outside the condition
+        __syncthreads();
+        uint64_t kept = 0;
+        for (int b = 0; b < ni; b += kWave) {  // the group's particles, one wave at a time
+            const bool act = b + lane < ni;
+            const int p = i0 + b + (act ? lane : 0);
+            const float oxi = (float) (x[p] - cx), oyi = (float) (y[p] - cy), ozi = (float) (z[p] - cz);
+            const float wi = fmaf(oxi, oxi, fmaf(oyi, oyi, ozi * ozi)), n1i = norm1_(oxi, oyi, ozi);
+            const float tx = -2.f * oxi, ty = -2.f * oyi, tz = -2.f * ozi;
+            const float ptx = __shfl_xor(tx, 32, kWave), pty = __shfl_xor(ty, 32, kWave),
+                        ptz = __shfl_xor(tz, 32, kWave);
+            const float negc = act ? -dot_cut_(Rc, wi, n1i, n1max) : INFINITY;
+            const float pnegc = xor32_f(negc);
+            kept |= mfma_dot_any_(so, min(kWave, nj - base), tx, ty, tz, negc, ptx, pty, ptz, pnegc);
+        }
+        const bool keep = (k < nj) && ((kept >> lane) & 1);
+        const uint64_t bk = __ballot(keep);
+        __syncthreads();  // every lane has read its entry before any is written over
+        if (keep) list[count + __popcll(bk & lt)] = qk;
+        count += __popcll(bk);
+    }
+    if (lane == 0) cnt[g] = count;
+}
+
 inline int nblk(int64_t n, int b) { return (int) ((n + b - 1) / b); }
 
 }  // namespace
@@ -1341,8 +1424,9 @@ void enqueue_gather_compact(const Tree &t, const PArrays &P, Lists &L, double hf
     const int64_t G = t.gcap;
     L.crad.ensure((size_t) std::max<int64_t>(G, 1));
     if (G <= 0) return;
-    hipLaunchKernelGGL(k_list_compact, dim3((unsigned) G), dim3(kWave), 0, s, t.info.get() + kInfoG, t.gstart.get(),
-                       P.x, P.y, P.z, P.h, pos, hfac, L.idx.get(), L.cnt.get(), L.cap, L.crad.get(), need);
+    hipLaunchKernelGGL(t.max_group > kWave ? k_list_compact_multi : k_list_compact, dim3((unsigned) G), dim3(kWave), 0,
+                       s, t.info.get() + kInfoG, t.gstart.get(), P.x, P.y, P.z, P.h, pos, hfac, L.idx.get(),
+                       L.cnt.get(), L.cap, L.crad.get(), need);
     SPH_LAUNCH_CHECK();
 }
 

@@ -169,9 +169,11 @@ int32_t sphgpu_eos_internal_energy(sphgpu_ctx *ctx, int64_t n, const int32_t *ma
                                    const double *ent, int32_t abs_int, double *eng);
 
 /* ---- device-resident simulation (src/main.cpp:69-173) ---- */
+#define SPHGPU_MAX_GROUP 1024
 typedef struct {
     int32_t n_leaf;          /* tree leaf limit (1..64), default 16 */
-    int32_t n_group;         /* i-group limit (1..64), default 64 */
+    int32_t n_group;         /* i-group limit (1..SPHGPU_MAX_GROUP), default 64; a group of more than
+                                64 particles runs as one work item per wave of 64, sharing its list */
     double theta;            /* gravity opening angle, default 0.5 */
     double damping;          /* dampMotion factor (src/GI.h:32), default 1 */
     double ext_force_until;  /* addExternalForce window (src/GI.h:459), default 5000 s */

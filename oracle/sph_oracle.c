@@ -742,6 +742,10 @@ static int64_t *starts_from_counts_(const int64_t *cnt, int64_t n, int64_t T, in
 
 orc_tree *orc_tree_build(const orc_state *s, const orc_params *prm)
 {
+    if (prm->n_group < 1 || prm->n_group > ORC_MAX_GROUP) {
+        fprintf(stderr, "oracle: n_group %d outside [1, %d]\n", prm->n_group, ORC_MAX_GROUP);
+        abort();
+    }
     const int64_t n = s->n;
     const int nsys = prm->nsys > 0 ? prm->nsys : 1;
     const int sb = sys_bits_(nsys), bits = morton_bits_(sb);
@@ -1121,9 +1125,9 @@ static void pass_density_(orc_state *s, orc_tree *t, const orc_params *prm)
     {
         int32_t *lst = (int32_t *) malloc(sizeof(int32_t) * LIST_CAP_);
         orc_epj_dens *epj = (orc_epj_dens *) malloc(sizeof(orc_epj_dens) * LIST_CAP_);
-        orc_epi_dens epi[64];
-        orc_res_dens res[64];
-        int32_t ngb[64];
+        orc_epi_dens epi[ORC_MAX_GROUP];
+        orc_res_dens res[ORC_MAX_GROUP];
+        int32_t ngb[ORC_MAX_GROUP];
 #pragma omp for schedule(dynamic, 4)
         for (int64_t g = 0; g < G; ++g) {
             const int64_t i0 = t->gstart[g], ni = t->gstart[g + 1] - i0;
@@ -1168,8 +1172,8 @@ static void pass_derivative_(orc_state *s, orc_tree *t, const orc_params *prm)
     {
         int32_t *lst = (int32_t *) malloc(sizeof(int32_t) * LIST_CAP_);
         orc_epj_drvt *epj = (orc_epj_drvt *) malloc(sizeof(orc_epj_drvt) * LIST_CAP_);
-        orc_epi_drvt epi[64];
-        orc_res_drvt res[64];
+        orc_epi_drvt epi[ORC_MAX_GROUP];
+        orc_res_drvt res[ORC_MAX_GROUP];
 #pragma omp for schedule(dynamic, 4)
         for (int64_t g = 0; g < G; ++g) {
             const int64_t i0 = t->gstart[g], ni = t->gstart[g + 1] - i0;
@@ -1209,8 +1213,8 @@ static void pass_hydro_(orc_state *s, orc_tree *t, const orc_params *prm)
     {
         int32_t *lst = (int32_t *) malloc(sizeof(int32_t) * LIST_CAP_);
         orc_epj_hydro *epj = (orc_epj_hydro *) malloc(sizeof(orc_epj_hydro) * LIST_CAP_);
-        orc_epi_hydro epi[64];
-        orc_res_hydro res[64];
+        orc_epi_hydro epi[ORC_MAX_GROUP];
+        orc_res_hydro res[ORC_MAX_GROUP];
 #pragma omp for schedule(dynamic, 4)
         for (int64_t g = 0; g < G; ++g) {
             const int64_t i0 = t->gstart[g], ni = t->gstart[g + 1] - i0;
@@ -1257,8 +1261,8 @@ static void pass_gravity_(orc_state *s, orc_tree *t, const orc_params *prm)
         int32_t *lj = (int32_t *) malloc(sizeof(int32_t) * LIST_CAP_), *ls = (int32_t *) malloc(sizeof(int32_t) * LIST_CAP_);
         orc_epj_grav *epj = (orc_epj_grav *) malloc(sizeof(orc_epj_grav) * LIST_CAP_);
         orc_spj_mono *spj = (orc_spj_mono *) malloc(sizeof(orc_spj_mono) * LIST_CAP_);
-        orc_epi_grav epi[64];
-        orc_res_grav res[64];
+        orc_epi_grav epi[ORC_MAX_GROUP];
+        orc_res_grav res[ORC_MAX_GROUP];
 #pragma omp for schedule(dynamic, 4)
         for (int64_t g = 0; g < G; ++g) {
             const int64_t i0 = t->gstart[g], ni = t->gstart[g + 1] - i0;
@@ -1391,8 +1395,8 @@ int orc_state_term_scales(const orc_state *s, const orc_params *prm, double *acc
     orc_epj_drvt *ed = (orc_epj_drvt *) malloc(sizeof(orc_epj_drvt) * LIST_CAP_);
     for (int64_t g = 0; g < t->ngroups; ++g) {
         const int64_t i0 = t->gstart[g], ni = t->gstart[g + 1] - i0;
-        orc_epi_hydro ih[64];
-        orc_epi_drvt id[64];
+        orc_epi_hydro ih[ORC_MAX_GROUP];
+        orc_epi_drvt id[ORC_MAX_GROUP];
         for (int64_t a = 0; a < ni; ++a) {
             const int64_t p = i0 + a;
             ih[a].pos[0] = s->x[p]; ih[a].pos[1] = s->y[p]; ih[a].pos[2] = s->z[p];

@@ -22,6 +22,8 @@
  */
 #ifndef SPH_ORACLE_H
 #define SPH_ORACLE_H
+
+#define ORC_MAX_GROUP 1024 /* the most particles an i-group may hold */
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -95,7 +97,7 @@ typedef struct {
 } orc_state;
 
 typedef struct {
-    int n_leaf, n_group;     /* tree leaf / i-group particle limits (<= 64) */
+    int n_leaf, n_group;     /* tree leaf (<= 64) / i-group (<= ORC_MAX_GROUP) particle limits */
     double theta;            /* gravity opening angle */
     double damping;          /* dampMotion factor (src/GI.h:32) */
     double ext_force_until;  /* addExternalForce window (5000 s, src/GI.h:459) */

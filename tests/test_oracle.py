@@ -366,6 +366,37 @@ def test_symmetric_lists(oracle, sorted_impact):
             assert set(np.nonzero(d < 2.5 * np.maximum(s.h[i], s.h))[0].tolist()) <= members
 
 
+@pytest.mark.parametrize("n_group", [256, 1024])
+def test_groups_of_several_waves(oracle, sorted_impact, n_group):
+    """i-groups of up to n_group particles (FDPS's n_group, more than one
+    64-lane wave): packed leaves within the limit, and every list kind still
+    equals its brute-force construction and covers every support."""
+    s = sorted_impact
+    t = oracle.Tree(s, oracle.params(n_group=n_group))
+    t.update_maxH()
+    gs = t.group_start()
+    sizes = np.diff(gs)
+    assert gs[0] == 0 and gs[-1] == s.n and np.all(sizes >= 1) and np.all(sizes <= n_group)
+    assert sizes.max() > 64  # groups span several waves
+    pos = np.stack([s.x, s.y, s.z], 1)
+    for g in range(0, t.ngroups, max(1, t.ngroups // 6)):
+        for kind in (0, 1):
+            lst = t.list_sph(g, kind, brute=False)
+            assert np.array_equal(lst, t.list_sph(g, kind, brute=True))
+            members = set(lst.tolist())
+            for i in range(gs[g], gs[g + 1], 7):
+                d = np.linalg.norm(pos - pos[i], axis=1)
+                H = 2.5 * (s.h[i] if kind == 0 else np.maximum(s.h[i], s.h))
+                assert set(np.nonzero(d < H)[0].tolist()) <= members
+        ej, es = t.list_grav(g, 0.5)
+        cover = np.zeros(s.n, dtype=np.int64)
+        cover[ej] += 1
+        for k in es:
+            rg, _, _ = t.node(k)
+            cover[rg[0]:rg[1]] += 1
+        assert np.all(cover == 1), g
+
+
 def test_gravity_lists_partition_and_accuracy(oracle, sorted_impact):
     s = sorted_impact
     t = oracle.Tree(s, oracle.params())
