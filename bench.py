@@ -737,8 +737,12 @@ def main():
         out["fp64_sensitivity"] = sens if len(sens) > 1 else sens[0]
         params["theta"] = th0
     out.update(extra)
-    # every timed step a replay of a recorded graph (the warm-up steps record them)
-    out["graph_captured"] = bool(args.graph) and R["graph"]["replayed_steps_timed"] == steps
+    # every timed step a replay of a recorded graph (the warm-up steps record
+    # them), except the direct step that precedes each recording again after a
+    # capacity grew (captures_timed counts those recordings)
+    g = R["graph"]
+    out["graph_captured"] = bool(args.graph) and g["replayed_steps_timed"] > 0 and \
+        g["replayed_steps_timed"] + g["captures_timed"] >= steps
 
     if not args.no_cpu_baseline:
         if args.workload == "granite10k":
