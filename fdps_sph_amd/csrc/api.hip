@@ -637,11 +637,6 @@ void launch_stash(StashArgs a, hipStream_t s) {
     SPH_LAUNCH_CHECK();
 }
 
-__global__ void k_fill_i32_api(int64_t n, int32_t *a, int32_t v) {
-    const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) a[k] = v;
-}
-
 __global__ void k_copy_d(int64_t n, const double *src, double *dst) {
     const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) dst[k] = src[k];
@@ -706,8 +701,7 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
     const int32_t *gs = S.tree.gstart.get();
     const int32_t *gcount = S.tree.info.get() + kInfoG;
     double4 *kpos = S.kpos.ensure(n), *kvel = S.kvel.ensure(n);  // j records of the density / derivative passes
-    launch_pack_kin(n, P.x, P.y, P.z, P.m, P.vx, P.vy, P.vz, kpos, kvel, s);
-    launch_copy_d(n, P.h, hsave, s);
+    launch_pack_kin(n, P.x, P.y, P.z, P.m, P.vx, P.vy, P.vz, kpos, kvel, s, P.h, hsave);  // h saved beside
     Lists &L = S.lsph, &Lg = S.lgrav;
     int32_t *stash = S.stash.p;
     // persistent launches of the force kernels named by the bitmask (1 density,
@@ -735,21 +729,19 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
         SPH_HIP(hipEventRecord(S.kev[S.kslot][1], s));
         ++S.kslot;
     };
-    lists_reset(L, s);
-    lists_reset(Lg, s);
     // per density pass: set when some group's gather list no longer covers the
-    // supports of the h that pass computed (the list is then rebuilt first)
+    // supports of the h that pass computed (the list is then rebuilt first);
+    // cleared with both lists' words in one launch
     int32_t *gflag = L.flag.ensure(4);
-    hipLaunchKernelGGL(k_fill_i32_api, dim3(1), dim3(64), 0, s, (int64_t) 4, gflag, 0);
-    SPH_LAUNCH_CHECK();
-    enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s);
+    lists_reset_all(L, Lg, gflag, 4, s);
+    enqueue_lists(S.tree, S.tp, P, 0, L, kGatherMargin, s, nullptr, true);
     enqueue_gather_compact(S.tree, P, L, kGatherMargin, s, nullptr, kpos);
     // the gravity walk beside the density passes (started beside the gather
     // walk instead, the step measured the same: two walks together take as
     // long as one after the other)
     signal(S.xev[0], s);  // tree, h restored, counters cleared, first gather lists
     await(s2, S.xev[0]);
-    enqueue_lists(S.tree, S.tp, P, 2, Lg, 1.0, s2);
+    enqueue_lists(S.tree, S.tp, P, 2, Lg, 1.0, s2, nullptr, true);
     signal(S.xev[1], s2);  // gravity lists
     // 4 density passes, each over lists that cover the current h
     // (src/main.cpp:138-140).  One gather list, built with a margin on h,
@@ -914,8 +906,7 @@ int sim_parity(const SimState &S) { return S.P.x == S.f[0][0].get() ? 0 : 1; }
 void step_enqueue(sphgpu_ctx *c, SimState &S, bool timed) {
     hipStream_t s = c->stream;
     if (timed) SPH_HIP(hipEventRecord(S.ev[0], s));
-    launch_kick_drift(S.n, S.P, S.dt_sys.get(), s);
-    launch_time_advance(S.nsys, S.time_sys.get(), S.dt_sys.get(), s);
+    launch_kick_drift(S.n, S.P, S.dt_sys.get(), S.nsys, S.time_sys.get(), s);
     if (timed) SPH_HIP(hipEventRecord(S.ev[1], s));
     forces_enqueue(c, S, timed, false);
 }

@@ -3,6 +3,8 @@
 // (src/class.h:196-229), addExternalForce (src/GI.h:458-465) and
 // getTimeStepGlobal (src/integral.h:3-21) as a per-system min-reduction.
 // Compiled with -ffp-contract=off (the reference's operand order).
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace sphgpu {
@@ -20,7 +22,9 @@ __global__ void k_init_h(int64_t n, PArrays P) {
     P.bal[p] = 1.0;
 }
 
-__global__ void k_kick_drift(int64_t n, PArrays P, const double *dt_sys) {
+__global__ void k_kick_drift(int64_t n, PArrays P, const double *dt_sys, int nsys, double *time_sys) {
+    if (blockIdx.x == 0)  // and each system's time advances by its dt
+        for (int q = threadIdx.x; q < nsys; q += blockDim.x) time_sys[q] += dt_sys[q];
     const int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const double dt = dt_sys[P.sys[p]];
@@ -41,10 +45,6 @@ __global__ void k_kick_drift(int64_t n, PArrays P, const double *dt_sys) {
     P.u[p] += dt * P.du[p];
 }
 
-__global__ void k_time_advance(int nsys, double *time_sys, const double *dt_sys) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < nsys) time_sys[s] += dt_sys[s];
-}
 
 // addExternalForce: acc += -vel * 0.05 / dt while time < until
 __device__ __forceinline__ void ext_force(PArrays &P, int64_t p, const double *time_sys, double until) {
@@ -118,14 +118,8 @@ void launch_init_h(int64_t n, PArrays P, hipStream_t s) {
     SPH_LAUNCH_CHECK();
 }
 
-void launch_kick_drift(int64_t n, PArrays P, const double *dt_sys, hipStream_t s) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_kick_drift, dim3(nblk(n, 256)), dim3(256), 0, s, n, P, dt_sys);
-    SPH_LAUNCH_CHECK();
-}
-
-void launch_time_advance(int nsys, double *time_sys, const double *dt_sys, hipStream_t s) {
-    hipLaunchKernelGGL(k_time_advance, dim3(nblk(nsys, 64)), dim3(64), 0, s, nsys, time_sys, dt_sys);
+void launch_kick_drift(int64_t n, PArrays P, const double *dt_sys, int nsys, double *time_sys, hipStream_t s) {
+    hipLaunchKernelGGL(k_kick_drift, dim3(std::max(1, nblk(n, 256))), dim3(256), 0, s, n, P, dt_sys, nsys, time_sys);
     SPH_LAUNCH_CHECK();
 }
 

@@ -245,7 +245,8 @@ struct GravArgs {
 void launch_sched_reset(int32_t *sched, hipStream_t s);
 int persistent_grid(const void *kernel, int ngroups);
 void launch_pack_kin(int64_t n, const double *x, const double *y, const double *z, const double *m, const double *vx,
-                     const double *vy, const double *vz, double4 *pos, double4 *vel, hipStream_t s);
+                     const double *vy, const double *vz, double4 *pos, double4 *vel, hipStream_t s,
+                     const double *h = nullptr, double *hcopy = nullptr);  // and h to hcopy, if given
 // Pack the hydro records of particles [0, n) from the SoA fields.
 void launch_pack_hydro(int64_t n, const double *x, const double *y, const double *z, const double *vx,
                        const double *vy, const double *vz, const double *m, const double *rho, const double *cs,
@@ -399,8 +400,12 @@ constexpr int kListWords = 10;  // Lists::err: overflow bits, construction count
 // L.hflag (4 ints), and after the stream has drained lists_grow enlarges the
 // capacities the bits of either slot name and returns true when there were none.
 void lists_reset(Lists &L, hipStream_t s);
+// lists_reset of two lists and nflag zero words of flag in one launch, which
+// also clears the lists' group counters: the next walk of each may then pass
+// `cleared` to enqueue_lists, which otherwise clears them itself
+void lists_reset_all(Lists &a, Lists &b, int32_t *flag, int nflag, hipStream_t s);
 void enqueue_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, double hfac,
-                   hipStream_t s, const int32_t *need = nullptr);
+                   hipStream_t s, const int32_t *need = nullptr, bool cleared = false);
 void lists_fetch(Lists &L, hipStream_t s, int slot = 0);
 bool lists_grow(Lists &L);
 bool lists_grow_bits(Lists &L, int32_t err);
@@ -430,8 +435,8 @@ struct StepGuard {
     }
 };
 void launch_init_h(int64_t n, PArrays P, hipStream_t s);
-void launch_kick_drift(int64_t n, PArrays P, const double *dt_sys, hipStream_t s);
-void launch_time_advance(int nsys, double *time_sys, const double *dt_sys, hipStream_t s);
+// kick and drift of every particle, and each system's time advanced by its dt
+void launch_kick_drift(int64_t n, PArrays P, const double *dt_sys, int nsys, double *time_sys, hipStream_t s);
 void launch_finalize(int64_t n, PArrays P, const double *dt_sys, const double *time_sys, double until,
                      double damping, StepGuard guard, hipStream_t s);
 void launch_ext_force(int64_t n, PArrays P, const double *time_sys, double until, hipStream_t s);

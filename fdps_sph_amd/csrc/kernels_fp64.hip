@@ -366,11 +366,13 @@ __device__ __forceinline__ HydroRec make_hydro_rec(double x, double y, double z,
 }
 
 __global__ void k_pack_kin(int64_t n, const double *x, const double *y, const double *z, const double *m,
-                           const double *vx, const double *vy, const double *vz, double4 *pos, double4 *vel) {
+                           const double *vx, const double *vy, const double *vz, double4 *pos, double4 *vel,
+                           const double *h, double *hcopy) {
     const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) {
         pos[k] = make_double4(x[k], y[k], z[k], m[k]);
         vel[k] = make_double4(vx[k], vy[k], vz[k], 0.0);
+        if (hcopy) hcopy[k] = h[k];
     }
 }
 
@@ -674,10 +676,11 @@ __global__ __launch_bounds__(kWave) void k_grav_persistent(GravArgs a) {
 }  // namespace
 
 void launch_pack_kin(int64_t n, const double *x, const double *y, const double *z, const double *m, const double *vx,
-                     const double *vy, const double *vz, double4 *pos, double4 *vel, hipStream_t s) {
+                     const double *vy, const double *vz, double4 *pos, double4 *vel, hipStream_t s, const double *h,
+                     double *hcopy) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_pack_kin, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, n, x, y, z, m, vx, vy, vz, pos,
-                       vel);
+                       vel, h, hcopy);
     SPH_LAUNCH_CHECK();
 }
 void launch_pack_hydro(int64_t n, const double *x, const double *y, const double *z, const double *vx,

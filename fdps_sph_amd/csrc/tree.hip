@@ -1378,6 +1378,27 @@ static int list_grid(int64_t ngroups, int waves) {
     return (int) std::max<int64_t>(8, b);
 }
 
+namespace {
+struct FillRanges {
+    int32_t *p[3];
+    int n[3];
+};
+__global__ void k_fill_ranges(FillRanges f) {
+    for (int k = 0; k < 3; ++k)
+        for (int i = threadIdx.x; i < f.n[k]; i += blockDim.x) f.p[k][i] = 0;
+}
+}  // namespace
+
+void lists_reset_all(Lists &a, Lists &b, int32_t *flag, int nflag, hipStream_t s) {
+    FillRanges f{{a.err.ensure(kListWords), b.err.ensure(kListWords), flag}, {kListWords, kListWords, flag ? nflag : 0}};
+    hipLaunchKernelGGL(k_fill_ranges, dim3(1), dim3(64), 0, s, f);
+    SPH_LAUNCH_CHECK();
+    for (Lists *L : {&a, &b}) {
+        int32_t *h = L->hflag.ensure(4);
+        for (int k = 0; k < 4; ++k) h[k] = 0;
+    }
+}
+
 void lists_reset(Lists &L, hipStream_t s) {
     L.err.ensure(kListWords);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int64_t) 2, L.err.get(), 0);
@@ -1387,7 +1408,7 @@ void lists_reset(Lists &L, hipStream_t s) {
 }
 
 void enqueue_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, double hfac,
-                   hipStream_t s, const int32_t *need) {
+                   hipStream_t s, const int32_t *need, bool cleared) {
     const int64_t G = t.gcap;
     if (kind != 0) hfac = 1.0;  // symmetric lists are exact: v_sig_max runs over the whole list
     L.idx.ensure((size_t) G * L.cap);
@@ -1400,8 +1421,10 @@ void enqueue_lists(const Tree &t, const TreeParams &tp, const PArrays &P, int ki
     const int grid = list_grid(G, L.waves);
     L.frontier.ensure((size_t) grid * 2 * L.fcap);
     L.err.ensure(kListWords);  // error bits, construction counter, the 8 per-XCD group counters
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int64_t) 8, L.err.get() + 2, 0);
-    SPH_LAUNCH_CHECK();
+    if (!cleared) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int64_t) 8, L.err.get() + 2, 0);
+        SPH_LAUNCH_CHECK();
+    }
     ListArgs a = list_args(t, tp, P, L);
     a.need = need;
     if (kind == 2) a.h = nullptr;  // geometry only: h may be in flux on another stream
