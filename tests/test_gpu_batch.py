@@ -54,3 +54,21 @@ def test_batch_precision_form_matches_oracle(ctxmix, oracle, batch_ic):
     tabs = [oracle.Table(E.PATHS[k]) for k in ("granite", "iron", "dunite")]
     prm = oracle.params(nsys=NSYS)
     run_per_step(sim, oracle, batch_ic, tabs, prm, 2, mixed_check(tabs, prm), dt_tol=1e-5)
+
+
+def test_autotune_picks_a_timed_candidate(ctx64, batch_ic):
+    """The autotuner times every (n_group, n_leaf) pair it is given -- groups of
+    one wave and of several -- then the launch forms and walk waves at the best
+    pair, and returns the fastest row of its table; that configuration runs."""
+    from fdps_sph_amd import autotune
+    res = autotune.tune(ctx64, batch_ic, n_groups=(64, 256), n_leaves=(8, 16), persistents=(0, 8),
+                        front_lds=(512,), wave_counts=(2048, 4096), warmup=1, steps=1, nsys=NSYS)
+    print(f"[autotune] best {res.params()} {res.ms_per_step:.3f} ms; {len(res.table)} candidates", flush=True)
+    pairs = {(r["n_group"], r["n_leaf"]) for r in res.table}
+    assert pairs == {(64, 8), (64, 16), (256, 8), (256, 16)}
+    assert {r["persistent"] for r in res.table} == {0, 8}
+    assert {r["walk_waves"] for r in res.table} == {2048, 4096}
+    assert res.ms_per_step == min(r["ms_per_step"] for r in res.table) > 0
+    sim = Simulation(ctx64, batch_ic, nsys=NSYS, **res.params())
+    sim.step(2)
+    assert np.isfinite(sim.get("ACC")).all() and np.all(sim.time()[1] > 0)
