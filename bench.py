@@ -56,11 +56,16 @@ HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6    # MI355X vector FP64
 FP32_PEAK_TFLOPS = 157.3   # MI355X vector FP32
 GRAV_FLOPS = 38            # flops per P2P / P2M interaction (acc + potential)
-# algorithmic bytes per i-particle (read + result) and per j-list entry, FP64 fields
-SPH_BYTES = {"dens": (40 + 16, 32), "drvt": (64 + 40, 56), "hydro": (88 + 40, 104), "dens_fp64": (40 + 16, 32)}
+# algorithmic bytes per i-particle (read + result) and per j-list entry, FP64
+# fields.  hydro: 12 inputs, acc, du and dt out (dt only when it forms v_sig_max
+# itself); vsig: position, velocity, c_s and h in, dt out; per entry position,
+# velocity and c_s
+SPH_BYTES = {"dens": (40 + 16, 32), "drvt": (64 + 40, 56), "hydro": (96 + 40, 104), "dens_fp64": (40 + 16, 32),
+             "vsig": (64 + 8, 56)}
+HYDRO_BYTES_SPLIT = (96 + 32, 104)
 # kernel -> (symbol prefix, FP64 kernel even in the precision form)
 KERNEL_SYMBOL = {"dens": ("k_dens", False), "drvt": ("k_drvt", False), "hydro": ("k_hydro", False),
-                 "grav": ("k_grav", False), "dens_fp64": ("k_dens", True)}
+                 "grav": ("k_grav", False), "dens_fp64": ("k_dens", True), "vsig": ("k_vsig", True)}
 
 
 def kernel_symbol(kernel, mixed):
@@ -169,7 +174,8 @@ def kernel_roofline(stats, mixed):
                       "per_launch": {"interactions": s["pairs"] / L, "flops": flops / L, "n_i": s["n_i"] / L,
                                      "n_epj": s["n_j"] / L, "n_spj": s["n_s"] / L}}
         else:
-            bi, bj = SPH_BYTES[k]
+            split = k == "hydro" and stats.get("vsig", {}).get("launches", 0) > 0
+            bi, bj = HYDRO_BYTES_SPLIT if split else SPH_BYTES[k]
             nbytes = bi * s["n_i"] + bj * s["n_j"]
             ach = nbytes / sec / 1e9
             out[k] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",

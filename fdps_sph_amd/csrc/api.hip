@@ -141,7 +141,7 @@ constexpr int kStages = 8;
 }  // namespace
 
 // ------------------------------------------------------------------ simulation state
-constexpr int kStatKinds = 5;
+constexpr int kStatKinds = 6;
 
 // one recorded step (the graph of one parity of the particle buffer sets)
 struct StepGraph {
@@ -173,15 +173,15 @@ struct SimState {
     DevBuf<int32_t> sched;  // per-XCD group counters of persistent force-kernel launches
     DevBuf<double> dt_sys, time_sys, dt_part;
     hipEvent_t ev[kStages + 1] = {};
-    hipEvent_t xev[5] = {};  // cross-stream order of a force evaluation (no timing)
+    hipEvent_t xev[6] = {};  // cross-stream order of a force evaluation (no timing)
     double stage_ms[kStages] = {0};
     bool ready = false;
     // per-kernel statistics (density x4, derivative, hydro, gravity per step)
     bool stats = false;
     // kinds: 0 density, 1 derivative, 2 hydro, 3 gravity, 4 density pass run in
-    // FP64 by the precision form
-    hipEvent_t kev[7][2] = {};
-    int kkind[7] = {0, 0, 0, 0, 1, 2, 3};
+    // FP64 by the precision form, 5 v_sig maximum (dt)
+    hipEvent_t kev[8][2] = {};
+    int kkind[8] = {0, 0, 0, 0, 1, 2, 3, 5};
     double kms[kStatKinds] = {0};
     int64_t klaunch[kStatKinds] = {0};
     DevBuf<long long> kcount;  // [kStatKinds][4]: n_i, n_j, n_s, pairs
@@ -599,6 +599,7 @@ std::vector<int64_t> step_sig(const sphgpu_ctx *c, const SimState &S) {
     v.push_back(c->precision);
     v.push_back(S.prm.persistent);
     v.push_back(S.prm.eos_density_fp64);
+    v.push_back(S.prm.vsig_split);
     v.push_back(S.tree.gcap);
     for (int l = 0; l < S.tree.nlevels; ++l) v.push_back(S.tree.cap_lv[l]);
     for (const Lists *L : {&S.lsph, &S.lgrav}) {
@@ -792,12 +793,44 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
         kend();
     }
     rec_ev(5);
-    signal(S.xev[2], s);  // the derivative pass is done with the gather lists
+    // the hydro pass's j records, packed here rather than on the second stream:
+    // there they would wait behind the gravity kernel's workgroups
+    HydroRec *hrec = nullptr;
+    HydroRecF *hrecf = nullptr;
+    if (mixed) {
+        hrecf = S.hrecf.ensure(n);
+        launch_pack_hydro_f(n, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.m, P.rho, P.cs, P.bal, P.gs, P.p, P.h, hrecf, s);
+    } else {
+        hrec = S.hrec.ensure(n);
+        launch_pack_hydro(n, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.m, P.rho, P.cs, P.bal, P.gs, P.p, P.h, hrec, s);
+    }
+    signal(S.xev[2], s);  // the derivative pass is done with the gather lists; records packed
     await(s2, S.xev[2]);
     // gather-list bits and construction counter, before the symmetric lists reuse L
     launch_stash(StashArgs{{L.err.get(), nullptr, nullptr, nullptr}, {2, 0, 0, 0}, stash + kStGather}, s2);
     enqueue_lists(S.tree, S.tp, P, 1, L, 1.0, s2);
     signal(S.xev[3], s2);  // symmetric lists
+    // the dt part of the hydro pass (v_sig_max over each list, FP64) as a kernel
+    // of its own beside the gravity kernel, when asked for (vsig_split bit of
+    // the precision form); the hydro kernel then forms the force terms only
+    const bool split = (S.prm.vsig_split >> (mixed ? 1 : 0)) & 1;
+    auto hydro_args = [&]() {
+        HydroArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal,
+                    P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal, P.m,
+                    lref(), P.ax, P.ay, P.az, P.du, P.dti, hrec};
+        a.jrecf = hrecf;
+        a.gcount = gcount;
+        a.ngroups = G * chunks;
+        a.chunks = chunks;
+        a.pass1 = split ? 0 : 1;
+        return a;
+    };
+    if (split) {
+        kbegin(&L, false, 5);
+        launch_vsig(hydro_args(), s2);
+        kend();
+        signal(S.xev[5], s2);  // dt
+    }
     await(s, S.xev[1]);
     {
         const Tree &t = S.tree;
@@ -819,29 +852,15 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
     }
     rec_ev(6);
     {
-        HydroRec *rec = nullptr;
-        HydroRecF *recf = nullptr;
-        if (mixed) {
-            recf = S.hrecf.ensure(n);
-            launch_pack_hydro_f(n, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.m, P.rho, P.cs, P.bal, P.gs, P.p, P.h, recf, s);
-        } else {
-            rec = S.hrec.ensure(n);
-            launch_pack_hydro(n, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.m, P.rho, P.cs, P.bal, P.gs, P.p, P.h, rec, s);
-        }
         await(s, S.xev[3]);
-        HydroArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal,
-                    P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal, P.m,
-                    lref(), P.ax, P.ay, P.az, P.du, P.dti, rec};
-        a.jrecf = recf;
+        HydroArgs a = hydro_args();
         a.sched = sched_for(4);
-        a.gcount = gcount;
-        a.ngroups = G * chunks;
-        a.chunks = chunks;
         kbegin(&L, false, 2);
         if (mixed) launch_hydro_mixed(a, s);
         else launch_hydro_fp64(a, s);
         kend();
     }
+    if (split) await(s, S.xev[5]);
     rec_ev(7);
     launch_stash(StashArgs{{L.err.get(), Lg.err.get(), S.tree.info.get(), nullptr}, {2, 2, kInfoWords, 0},
                            stash + kStSph},
@@ -1325,6 +1344,7 @@ void sphgpu_sim_default_params(sphgpu_sim_params *p) {
     p->walk_waves = 0;
     p->persistent = 0;
     p->graph = 0;
+    p->vsig_split = 1;
 }
 
 int32_t sphgpu_sim_init(sphgpu_ctx *c, const sphgpu_sim_params *prm, int64_t n, const double *pos, const double *vel,
@@ -1340,6 +1360,7 @@ int32_t sphgpu_sim_init(sphgpu_ctx *c, const sphgpu_sim_params *prm, int64_t n, 
         if (prm->n_group < 1 || prm->n_group > SPHGPU_MAX_GROUP)
             throw Error(SPHGPU_ERR_ARG, "n_group must be in [1, SPHGPU_MAX_GROUP]");
         if (!(prm->theta >= 0)) throw Error(SPHGPU_ERR_ARG, "theta must be >= 0");
+        if (prm->vsig_split < 0 || prm->vsig_split > 3) throw Error(SPHGPU_ERR_ARG, "vsig_split must be in [0, 3]");
         const int nsys = prm->nsys > 0 ? prm->nsys : 1;
         if (nsys > 1 && !system) throw Error(SPHGPU_ERR_ARG, "nsys > 1 needs a system array");
         for (int64_t i = 0; i < n; ++i) {

@@ -208,6 +208,10 @@ struct HydroArgs {
     // work items per group: a group of up to 64 * chunks particles runs as
     // `chunks` one-wave items sharing its list (ngroups then counts items)
     int chunks = 1;
+    // 1: the hydro kernel also forms v_sig_max over the list and writes dt;
+    // 0: launch_vsig does that (a kernel of its own, which may run beside
+    // other work); the hydro kernel then writes acc and du only
+    int pass1 = 1;
 };
 
 struct GravArgs {
@@ -262,6 +266,10 @@ void launch_pack_hydro_f(int64_t n, const double *x, const double *y, const doub
 void launch_dens_fp64(const DensArgs &a, hipStream_t s);
 void launch_drvt_fp64(const DrvtArgs &a, hipStream_t s);
 void launch_hydro_fp64(const HydroArgs &a, hipStream_t s);
+// dt from v_sig_max over each group's list in FP64 (both precision forms): the
+// j side from jrec, else from jrecf's positions and velocities with csj, else
+// from the arrays
+void launch_vsig(const HydroArgs &a, hipStream_t s);
 void launch_grav_fp64(const GravArgs &a, hipStream_t s);
 
 // Mixed-precision interaction kernels -- kernels_mixed.hip

@@ -401,7 +401,10 @@ __global__ void k_pack_hydro_f(int64_t n, const double *x, const double *y, cons
     out[k] = f;
 }
 // 4 waves per SIMD (128 registers, seven spilled outside the pair loops):
-// measured 2.09 ms at 1M against 2.14 ms at the compiler's 3 waves.
+// measured 2.09 ms at 1M against 2.14 ms at the compiler's 3 waves.  P1: the
+// kernel also forms v_sig_max over the list and writes dt (else k_vsig does;
+// without pass 1 the kernel needs 126 registers and spills none).
+template <bool P1>
 __device__ __forceinline__ void hydro_group(const HydroArgs &a, const int item) {
     const int lane = threadIdx.x;
     const int g = group_of(a, item);
@@ -452,8 +455,10 @@ __device__ __forceinline__ void hydro_group(const HydroArgs &a, const int item) 
                                     : make_hydro_rec(a.xj[q], a.yj[q], a.zj[q], a.vxj[q], a.vyj[q], a.vzj[q], a.mj[q],
                                                      a.rhoj[q], a.csj[q], a.balj[q], a.gsj[q], a.pj[q], a.hj[q]);
             sj[lane].r = r;
-            const double ux = r.vx - cvx, uy = r.vy - cvy, uz = r.vz - cvz;
-            bj = r.cs + 3.0 * sqrt(ux * ux + uy * uy + uz * uz);
+            if (P1) {
+                const double ux = r.vx - cvx, uy = r.vy - cvy, uz = r.vz - cvz;
+                bj = r.cs + 3.0 * sqrt(ux * ux + uy * uy + uz * uz);
+            }
             const float ox = (float) (r.x - fr.cx), oy = (float) (r.y - fr.cy), oz = (float) (r.z - fr.cz);
             sj[lane].o = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
             nj1 = norm1_(ox, oy, oz);
@@ -468,8 +473,8 @@ __device__ __forceinline__ void hydro_group(const HydroArgs &a, const int item) 
         // b_i + b_j stays below each lane's maximum by a margin far above the
         // rounding of either side, so pass 1 visits only these (none: the tile
         // is skipped)
-        const double T = dpp_wave_min(act ? vmax * (1.0 - 1.0e-10) - bi : INFINITY);
-        const uint64_t cand = __ballot(k < tl.nj && !(bj < T));
+        const double T = P1 ? dpp_wave_min(act ? vmax * (1.0 - 1.0e-10) - bi : INFINITY) : 0.0;
+        const uint64_t cand = P1 ? __ballot(k < tl.nj && !(bj < T)) : 0ULL;
         __syncthreads();
         const int nt = tl.count(t);
         // pass 1, every entry that may matter: v_sig_max, with w = dv.dr / |dr|
@@ -554,20 +559,133 @@ __device__ __forceinline__ void hydro_group(const HydroArgs &a, const int item) 
         a.ay[ip] = 22.0 * ay;
         a.az[ip] = 22.0 * az;
         a.du[ip] = -11.0 * du;
-        a.dt[ip] = kCFL * 2.0 * hi / vmax;
+        if (P1) a.dt[ip] = kCFL * 2.0 * hi / vmax;
     }
 }
+template <bool P1>
 __global__ __launch_bounds__(kWave, 4) void k_hydro(HydroArgs a) {
     const int G = group_count(a);
-    if ((int) blockIdx.x < G) hydro_group(a, xcd_remap((int) blockIdx.x, G));
+    if ((int) blockIdx.x < G) hydro_group<P1>(a, xcd_remap((int) blockIdx.x, G));
 }
 // persistent form: a fixed grid taking groups from the per-XCD counters
+template <bool P1>
 __global__ __launch_bounds__(kWave, 4) void k_hydro_persistent(HydroArgs a) {
     GroupIter it(a.sched, group_count(a));
     for (int g = it.first(); g >= 0; g = it.next()) {
-        hydro_group(a, g);
+        hydro_group<P1>(a, g);
         __syncthreads();  // the next group reuses the LDS tiles
     }
+}
+
+// ------------------------------------------------------------ v_sig maximum
+// The dt of CalcHydroForce (src/force.h:122-124, 145): v_sig_max over every j
+// of the list, i.e. pass 1 of the hydro kernel, as a kernel of its own.  It
+// needs only positions, velocities and sound speeds, so the force evaluation
+// runs it on its second stream beside the gravity kernel, and the hydro kernel
+// (pass 2 alone) keeps its registers for the force terms.  Each tile is tested
+// with the same bound, each candidate's v_sig is formed with the same
+// arithmetic, and the maximum does not depend on the order: dt is the number
+// the one-kernel form gives.
+struct alignas(16) VsigSlot {
+    double x, y, z, vx, vy, vz, cs, pad;
+};
+// the entry's position, velocity and sound speed: from the FP64 hydro records
+// (three 16-byte loads and one 8-byte load), from the precision form's records
+// (position and velocity) and the FP64 sound speeds, or from the arrays
+__device__ __forceinline__ VsigSlot vsig_load(const HydroArgs &a, int q) {
+    VsigSlot r{};
+    if (q < 0) return r;
+    if (a.jrec || a.jrecf) {
+        const double *p = a.jrec ? &a.jrec[q].x : &a.jrecf[q].x;
+        const double2 u = reinterpret_cast<const double2 *>(p)[0], v = reinterpret_cast<const double2 *>(p)[1],
+                      w = reinterpret_cast<const double2 *>(p)[2];
+        r.x = u.x; r.y = u.y; r.z = v.x; r.vx = v.y; r.vy = w.x; r.vz = w.y;
+        r.cs = a.jrec ? a.jrec[q].cs : a.csj[q];
+    } else {
+        r.x = a.xj[q]; r.y = a.yj[q]; r.z = a.zj[q];
+        r.vx = a.vxj[q]; r.vy = a.vyj[q]; r.vz = a.vzj[q];
+        r.cs = a.csj[q];
+    }
+    return r;
+}
+__device__ __forceinline__ void vsig_group(const HydroArgs &a, const int item) {
+    const int lane = threadIdx.x;
+    const int g = group_of(a, item);
+    const int i0 = a.gstart[g] + kWave * (item - g * a.chunks), ni = min(kWave, a.gstart[g + 1] - i0);
+    if (ni <= 0) return;  // a chunk beyond the group's particles
+    const bool act = lane < ni;
+    const int ip = i0 + (act ? lane : 0);
+    const double xi = a.xi[ip], yi = a.yi[ip], zi = a.zi[ip];
+    const double vxi = a.vxi[ip], vyi = a.vyi[ip], vzi = a.vzi[ip];
+    const double csi = a.csi[ip];
+    // the bound of the hydro kernel's pass 1: v_sig <= b_i + b_j, b = c + 3|v - v0|
+    const double cvx = 0.5 * (wave_min_(act ? vxi : INFINITY) + wave_max_(act ? vxi : -INFINITY));
+    const double cvy = 0.5 * (wave_min_(act ? vyi : INFINITY) + wave_max_(act ? vyi : -INFINITY));
+    const double cvz = 0.5 * (wave_min_(act ? vzi : INFINITY) + wave_max_(act ? vzi : -INFINITY));
+    const double bi = csi + 3.0 * sqrt((vxi - cvx) * (vxi - cvx) + (vyi - cvy) * (vyi - cvy) + (vzi - cvz) * (vzi - cvz));
+    const int64_t lb = list_begin(a.L, g);
+    const Tiles tl(a.L.cnt[g]);  // interleaved as in the hydro kernel: 20 % fewer candidates than list order
+    __shared__ VsigSlot sv[kWave];
+    auto index_at = [&](int t) {
+        const int k = tl.entry(t, lane);
+        return (t < tl.T && k < tl.nj) ? a.L.idx[lb + k] : -1;
+    };
+    // list indices two tiles ahead, records one tile ahead
+    VsigSlot rn = vsig_load(a, index_at(0));
+    int qn = index_at(1);
+    double vmax = 0.0;
+    for (int t = 0; t < tl.T; ++t) {
+        const int k = tl.entry(t, lane);
+        const VsigSlot r = rn;
+        const int q2 = index_at(t + 2);
+        rn = vsig_load(a, qn);
+        qn = q2;
+        double cj = 0.0, u2 = 0.0;
+        if (k < tl.nj) {
+            sv[lane] = r;
+            const double ux = r.vx - cvx, uy = r.vy - cvy, uz = r.vz - cvz;
+            u2 = ux * ux + uy * uy + uz * uz;
+            cj = r.cs;
+        }
+        // the entry is a candidate unless c_j + 3|u_j| < T, tested without
+        // the square root as e = T - c_j > 0 and 9|u_j|^2 < e^2 (the 1e-10
+        // margin of T covers the rounding of either form)
+        const double T = dpp_wave_min(act ? vmax * (1.0 - 1.0e-10) - bi : INFINITY);
+        const double e = T - cj;
+        const uint64_t cand = __ballot(k < tl.nj && !(e > 0.0 && 9.0 * u2 < e * e));
+        __syncthreads();
+        auto vsig = [&](const VsigSlot &J) {
+            const double drx = xi - J.x, dry = yi - J.y, drz = zi - J.z;
+            const double dvx = vxi - J.vx, dvy = vyi - J.vy, dvz = vzi - J.vz;
+            const double dvdr = fma(dvz, drz, fma(dvy, dry, dvx * drx));
+            const double w = fmin(dvdr, 0.0) * dist_(norm2_(drx, dry, drz)).rinv;
+            return fma(-3.0, w, csi + J.cs);
+        };
+        // four candidates per pass (a short tail repeats the first, which the
+        // maximum absorbs); fmax is std::max here (v_sig >= +0, vmax never
+        // NaN).  Measured at 1M beside the gravity kernel, FP64 step: two per
+        // pass 7.54 ms, four 7.47 ms (96 registers: fewer waves of it share
+        // the compute units with gravity; capping them further through LDS
+        // made it slower, 7.56 and 7.97 ms)
+        for (uint64_t cm = cand; cm;) {
+            const int j1 = __builtin_ctzll(cm);
+            cm &= cm - 1;
+            const int j2 = cm ? __builtin_ctzll(cm) : j1;
+            if (cm) cm &= cm - 1;
+            const int j3 = cm ? __builtin_ctzll(cm) : j1;
+            if (cm) cm &= cm - 1;
+            const int j4 = cm ? __builtin_ctzll(cm) : j1;
+            if (cm) cm &= cm - 1;
+            const double v1 = vsig(sv[j1]), v2 = vsig(sv[j2]), v3 = vsig(sv[j3]), v4 = vsig(sv[j4]);
+            vmax = fmax(fmax(vmax, fmax(v1, v2)), fmax(v3, v4));
+        }
+        __syncthreads();
+    }
+    if (act) a.dt[ip] = kCFL * 2.0 * a.hi[ip] / vmax;
+}
+__global__ __launch_bounds__(kWave) void k_vsig(HydroArgs a) {
+    const int G = group_count(a);
+    if ((int) blockIdx.x < G) vsig_group(a, xcd_remap((int) blockIdx.x, G));
 }
 
 // ---------------------------------------------------------------- gravity
@@ -747,7 +865,17 @@ static void launch_groups(void (*k)(A), void (*kp)(A), const A &a, hipStream_t s
 
 void launch_dens_fp64(const DensArgs &a, hipStream_t s) { launch_groups(k_dens, k_dens_persistent, a, s); }
 void launch_drvt_fp64(const DrvtArgs &a, hipStream_t s) { launch_groups(k_drvt, k_drvt_persistent, a, s); }
-void launch_hydro_fp64(const HydroArgs &a, hipStream_t s) { launch_groups(k_hydro, k_hydro_persistent, a, s); }
+void launch_hydro_fp64(const HydroArgs &a, hipStream_t s) {
+    if (a.pass1) launch_groups(k_hydro<true>, k_hydro_persistent<true>, a, s);
+    else launch_groups(k_hydro<false>, k_hydro_persistent<false>, a, s);
+}
+// one workgroup per group (no persistent form: the persistent counters belong
+// to the kernels of the main stream)
+void launch_vsig(const HydroArgs &a, hipStream_t s) {
+    if (a.ngroups <= 0) return;
+    hipLaunchKernelGGL(k_vsig, dim3(a.ngroups), dim3(kWave), 0, s, a);
+    SPH_LAUNCH_CHECK();
+}
 void launch_grav_fp64(const GravArgs &a, hipStream_t s) { launch_groups(k_grav, k_grav_persistent, a, s); }
 
 }  // namespace sphgpu

@@ -321,7 +321,9 @@ struct alignas(16) HydroSlotF {
 static_assert(sizeof(HydroSlotF) == 80, "LDS record stride");
 
 // 4 waves per SIMD: at 5 waves (96 registers) 27 spill, and the kernel took
-// 1.33 ms at 1M against 1.20 ms at 4 waves (124 registers).
+// 1.33 ms at 1M against 1.20 ms at 4 waves (124 registers).  P1: the kernel
+// also forms v_sig_max (in FP32) and writes dt; else the FP64 k_vsig does.
+template <bool P1>
 __device__ __forceinline__ void hydro_mixed_group(const HydroArgs &a, const int item) {
     const int lane = threadIdx.x;
     const int g = group_of(a, item);
@@ -400,7 +402,7 @@ __device__ __forceinline__ void hydro_mixed_group(const HydroArgs &a, const int 
             nj1 = norm1_(ox, oy, oz);
             sQ[lane] = dot_cut_j_(ek.w * 1.0001f, nj1, n1max);
             const float ux = (float) (vx - cvx), uy = (float) (vy - cvy), uz = (float) (vz - cvz);
-            bnd = cjf + 3.f * sqrtf(ux * ux + uy * uy + uz * uz);
+            if (P1) bnd = cjf + 3.f * sqrtf(ux * ux + uy * uy + uz * uz);
             HydroSlotF e;
             e.o = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
             e.v = make_float4(ux, uy, uz, cjf);
@@ -416,8 +418,8 @@ __device__ __forceinline__ void hydro_mixed_group(const HydroArgs &a, const int 
         // the entries that can raise some lane's v_sig maximum: bound_j >= T,
         // T the least over the lanes of vmax_i (1 - 1e-5) - b_i (a margin far
         // above the FP32 rounding); no other entry can change any maximum
-        const float T = wave_minf(act ? vmax * (1.f - 1.0e-5f) - bi : INFINITY);
-        const uint64_t cand = __ballot(k < nj && !(bnd < T));
+        const float T = P1 ? wave_minf(act ? vmax * (1.f - 1.0e-5f) - bi : INFINITY) : 0.f;
+        const uint64_t cand = P1 ? __ballot(k < nj && !(bnd < T)) : 0ULL;
         __syncthreads();
         const int nt = tl.count(t);
         float px = 0.f, py = 0.f, pz = 0.f, pu = 0.f;
@@ -489,18 +491,20 @@ __device__ __forceinline__ void hydro_mixed_group(const HydroArgs &a, const int 
         a.ay[ip] = ay.value();
         a.az[ip] = az.value();
         a.du[ip] = du.value();
-        a.dt[ip] = kCFL * 2.0 * hid / (double) vmax;
+        if (P1) a.dt[ip] = kCFL * 2.0 * hid / (double) vmax;
     }
 }
+template <bool P1>
 __global__ __launch_bounds__(kWave, 4) void k_hydro_mixed(HydroArgs a) {
     const int G = group_count(a);
-    if ((int) blockIdx.x < G) hydro_mixed_group(a, xcd_remap((int) blockIdx.x, G));
+    if ((int) blockIdx.x < G) hydro_mixed_group<P1>(a, xcd_remap((int) blockIdx.x, G));
 }
 // persistent form: a fixed grid taking groups from the per-XCD counters
+template <bool P1>
 __global__ __launch_bounds__(kWave, 4) void k_hydro_mixed_persistent(HydroArgs a) {
     GroupIter it(a.sched, group_count(a));
     for (int g = it.first(); g >= 0; g = it.next()) {
-        hydro_mixed_group(a, g);
+        hydro_mixed_group<P1>(a, g);
         __syncthreads();  // the next group reuses the LDS tiles
     }
 }
@@ -609,7 +613,8 @@ void launch_drvt_mixed(const DrvtArgs &a, hipStream_t s) {
     launch_groups(k_drvt_mixed, k_drvt_mixed_persistent, a, s);
 }
 void launch_hydro_mixed(const HydroArgs &a, hipStream_t s) {
-    launch_groups(k_hydro_mixed, k_hydro_mixed_persistent, a, s);
+    if (a.pass1) launch_groups(k_hydro_mixed<true>, k_hydro_mixed_persistent<true>, a, s);
+    else launch_groups(k_hydro_mixed<false>, k_hydro_mixed_persistent<false>, a, s);
 }
 void launch_grav_mixed(const GravArgs &a, hipStream_t s) {
     launch_groups(k_grav_mixed, k_grav_mixed_persistent, a, s);

@@ -241,7 +241,7 @@ class Simulation:
         check(self._L.sphgpu_sim_set_stats(self._h, int(bool(on))))
 
     def kernel_stats(self) -> dict:
-        names = ["dens", "drvt", "hydro", "grav", "dens_fp64"]
+        names = ["dens", "drvt", "hydro", "grav", "dens_fp64", "vsig"]
         arr = (KernelStat * len(names))()
         check(self._L.sphgpu_sim_kernel_stats(self._h, arr, len(names)))
         return {names[k]: {f: getattr(arr[k], f) for f, _ in KernelStat._fields_} for k in range(len(names))}

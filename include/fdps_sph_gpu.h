@@ -206,6 +206,13 @@ typedef struct {
                                 (rounded up to a multiple of 8, at most 8192); 0 (default): 4096.
                                 Walks beside force kernels share the GPU with them; fewer
                                 waves leave the kernels more of it.  Results are identical. */
+    int32_t vsig_split;      /* bitmask of the precision forms (1 FP64, 2 the precision form)
+                                whose hydro pass forms dt -- v_sig_max over each list -- in a
+                                kernel of its own (FP64 in both forms), run on the second
+                                stream beside the gravity kernel, the hydro kernel forming the
+                                force terms only; default 1.  The FP64 form's results are
+                                identical either way; the precision form's dt becomes the
+                                FP64 dt of its state. */
 } sphgpu_sim_params;
 
 void sphgpu_sim_default_params(sphgpu_sim_params *p);
@@ -267,9 +274,10 @@ int32_t sphgpu_sim_graph_stats(sphgpu_ctx *ctx, int64_t *replayed_steps, int64_t
  * context's stream, one pair per launch), launches, and the interaction-list
  * sizes those launches processed: n_i = i-particles, n_j = j-list entries
  * (particles), n_s = monopole entries, pairs = sum over groups of
- * n_i * (n_j + n_s).  Index 0 density, 1 derivative, 2 hydro, 3 gravity,
- * 4 the density passes the precision form runs in FP64 (eos_density_fp64);
- * n may be 4 or 5. */
+ * n_i * (n_j + n_s).  Index 0 density, 1 derivative, 2 hydro (force
+ * terms), 3 gravity, 4 the density passes the precision form runs in FP64
+ * (eos_density_fp64), 5 the hydro pass's dt (v_sig_max over each list, a
+ * kernel of its own in a simulation step); n may be 4 to 6. */
 typedef struct {
     double ms;
     int64_t launches, n_i, n_j, n_s, pairs;
