@@ -580,6 +580,8 @@ def main():
     ap.add_argument("--graph", type=int, choices=[0, 1], default=1,
                     help="1 (default): every step after the first replays a recorded HIP graph of the whole step; "
                          "0: steps launched kernel by kernel")
+    ap.add_argument("--vsig-split", type=int, choices=[0, 1, 2, 3], default=1, metavar="MASK",
+                    help="precision forms (1 FP64, 2 mixed) whose hydro dt runs as a kernel of its own")
     ap.add_argument("--walk-waves", type=int, default=0,
                     help="persistent waves of each interaction-list walk (0: the library's default, 4096)")
     ap.add_argument("--eos-density-fp32", action="store_true",
@@ -624,7 +626,8 @@ def main():
     load_tables(C, tables)
 
     params = {"theta": args.theta, "nsys": nsys, "eos_density_fp64": 0 if args.eos_density_fp32 else 1,
-              "persistent": args.persistent, "graph": args.graph, "walk_waves": args.walk_waves}
+              "persistent": args.persistent, "graph": args.graph, "walk_waves": args.walk_waves,
+              "vsig_split": args.vsig_split}
     extra = {}
     if args.workload == "batch64":
         from fdps_sph_amd import autotune
@@ -679,7 +682,7 @@ def main():
                    "precision": main_prec, "n_group": params.get("n_group", 64),
                    "n_leaf": params.get("n_leaf", 16), "persistent": params.get("persistent", 0),
                    "list_front_lds": params.get("list_front_lds", 512), "graph": args.graph,
-                   "walk_waves": params.get("walk_waves", 0) or 4096,
+                   "walk_waves": params.get("walk_waves", 0) or 4096, "vsig_split": params.get("vsig_split", 1),
                    "eos_density": "fp32" if args.eos_density_fp32 else "fp64"},
         "roofline": roofline,
         main_prec: block,

@@ -62,12 +62,14 @@ def test_autotune_picks_a_timed_candidate(ctx64, batch_ic):
     pair, and returns the fastest row of its table; that configuration runs."""
     from fdps_sph_amd import autotune
     res = autotune.tune(ctx64, batch_ic, n_groups=(64, 256), n_leaves=(8, 16), persistents=(0, 8),
-                        front_lds=(512,), wave_counts=(2048, 4096), warmup=1, steps=1, nsys=NSYS)
+                        front_lds=(512,), wave_counts=(2048, 4096), vsig_splits=(0, 1), warmup=1, steps=1,
+                        nsys=NSYS)
     print(f"[autotune] best {res.params()} {res.ms_per_step:.3f} ms; {len(res.table)} candidates", flush=True)
     pairs = {(r["n_group"], r["n_leaf"]) for r in res.table}
     assert pairs == {(64, 8), (64, 16), (256, 8), (256, 16)}
     assert {r["persistent"] for r in res.table} == {0, 8}
     assert {r["walk_waves"] for r in res.table} == {2048, 4096}
+    assert {r["vsig_split"] for r in res.table} == {0, 1}
     assert res.ms_per_step == min(r["ms_per_step"] for r in res.table) > 0
     sim = Simulation(ctx64, batch_ic, nsys=NSYS, **res.params())
     sim.step(2)
