@@ -189,6 +189,7 @@ struct SimState {
     DevBuf<double> hsave;      // h at the start of a force evaluation (for its redo)
     DevBuf<HydroRec> hrec;     // j records of the hydro pass
     DevBuf<HydroRecF> hrecf;   // the same in the precision form
+    DevBuf<VsigRec> vrec;      // j records of the dt kernel (vsig_split)
     DevBuf<double4> kpos, kvel; // j records of the density and derivative passes
     int64_t gather_builds = 0; // gather-list constructions since init
     int64_t force_evals = 0;   // force evaluations completed since init
@@ -795,14 +796,20 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
     rec_ev(5);
     // the hydro pass's j records, packed here rather than on the second stream:
     // there they would wait behind the gravity kernel's workgroups
+    // the dt part of the hydro pass (v_sig_max over each list, FP64) as a kernel
+    // of its own beside the gravity kernel, when asked for (vsig_split bit of
+    // the precision form); the hydro kernel then forms the force terms only
+    const bool split = (S.prm.vsig_split >> (mixed ? 1 : 0)) & 1;
     HydroRec *hrec = nullptr;
     HydroRecF *hrecf = nullptr;
+    VsigRec *vrec = split ? S.vrec.ensure(n) : nullptr;
     if (mixed) {
         hrecf = S.hrecf.ensure(n);
-        launch_pack_hydro_f(n, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.m, P.rho, P.cs, P.bal, P.gs, P.p, P.h, hrecf, s);
+        launch_pack_hydro_f(n, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.m, P.rho, P.cs, P.bal, P.gs, P.p, P.h, hrecf, s,
+                            vrec);
     } else {
         hrec = S.hrec.ensure(n);
-        launch_pack_hydro(n, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.m, P.rho, P.cs, P.bal, P.gs, P.p, P.h, hrec, s);
+        launch_pack_hydro(n, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.m, P.rho, P.cs, P.bal, P.gs, P.p, P.h, hrec, s, vrec);
     }
     signal(S.xev[2], s);  // the derivative pass is done with the gather lists; records packed
     await(s2, S.xev[2]);
@@ -810,10 +817,6 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
     launch_stash(StashArgs{{L.err.get(), nullptr, nullptr, nullptr}, {2, 0, 0, 0}, stash + kStGather}, s2);
     enqueue_lists(S.tree, S.tp, P, 1, L, 1.0, s2);
     signal(S.xev[3], s2);  // symmetric lists
-    // the dt part of the hydro pass (v_sig_max over each list, FP64) as a kernel
-    // of its own beside the gravity kernel, when asked for (vsig_split bit of
-    // the precision form); the hydro kernel then forms the force terms only
-    const bool split = (S.prm.vsig_split >> (mixed ? 1 : 0)) & 1;
     auto hydro_args = [&]() {
         HydroArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal,
                     P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal, P.m,
@@ -827,7 +830,9 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
     };
     if (split) {
         kbegin(&L, false, 5);
-        launch_vsig(hydro_args(), s2);
+        HydroArgs va = hydro_args();
+        va.vrec = vrec;
+        launch_vsig(va, s2);
         kend();
         signal(S.xev[5], s2);  // dt
     }

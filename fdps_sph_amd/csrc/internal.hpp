@@ -189,6 +189,12 @@ struct alignas(16) HydroRecF {
     float4 c;  // sound speed, A = f P / rho^2, 0, 0
 };
 static_assert(sizeof(HydroRecF) == 96, "record stride");
+// The dt part's j record: position, velocity and sound speed, half a cache
+// line (the 128-byte hydro record would be fetched whole for 56 bytes of it).
+struct alignas(16) VsigRec {
+    double x, y, z, vx, vy, vz, cs, pad;
+};
+static_assert(sizeof(VsigRec) == 64, "record stride");
 
 struct HydroArgs {
     int ngroups;
@@ -199,6 +205,7 @@ struct HydroArgs {
     double *ax, *ay, *az, *du, *dt;
     const HydroRec *jrec = nullptr;  // if set: the j side is read from here (list indices index it)
     const HydroRecF *jrecf = nullptr;  // the precision form's records, read in preference to jrec
+    const VsigRec *vrec = nullptr;     // the dt kernel's records, read in preference to the others
     // persistent scheduling: 8 per-XCD group counters, cleared before the launch
     // (null: one workgroup per group)
     int32_t *sched = nullptr;
@@ -255,12 +262,12 @@ void launch_pack_kin(int64_t n, const double *x, const double *y, const double *
 void launch_pack_hydro(int64_t n, const double *x, const double *y, const double *z, const double *vx,
                        const double *vy, const double *vz, const double *m, const double *rho, const double *cs,
                        const double *bal, const double *gs, const double *p, const double *h, HydroRec *out,
-                       hipStream_t s);
+                       hipStream_t s, VsigRec *vout = nullptr);
 // The same particles' HydroRecF records (precision form).
 void launch_pack_hydro_f(int64_t n, const double *x, const double *y, const double *z, const double *vx,
                          const double *vy, const double *vz, const double *m, const double *rho, const double *cs,
                          const double *bal, const double *gs, const double *p, const double *h, HydroRecF *out,
-                         hipStream_t s);
+                         hipStream_t s, VsigRec *vout = nullptr);
 
 // FP64 (plain) interaction kernels -- kernels_fp64.hip
 void launch_dens_fp64(const DensArgs &a, hipStream_t s);

@@ -379,17 +379,19 @@ __global__ void k_pack_kin(int64_t n, const double *x, const double *y, const do
 __global__ void k_pack_hydro(int64_t n, const double *x, const double *y, const double *z, const double *vx,
                              const double *vy, const double *vz, const double *m, const double *rho,
                              const double *cs, const double *bal, const double *gs, const double *p, const double *h,
-                             HydroRec *out) {
+                             HydroRec *out, VsigRec *vout) {
     const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) out[k] = make_hydro_rec(x[k], y[k], z[k], vx[k], vy[k], vz[k], m[k], rho[k], cs[k], bal[k], gs[k],
-                                       p[k], h[k]);
+    if (k >= n) return;
+    out[k] = make_hydro_rec(x[k], y[k], z[k], vx[k], vy[k], vz[k], m[k], rho[k], cs[k], bal[k], gs[k], p[k], h[k]);
+    if (vout) vout[k] = VsigRec{x[k], y[k], z[k], vx[k], vy[k], vz[k], cs[k], 0.0};
 }
 __global__ void k_pack_hydro_f(int64_t n, const double *x, const double *y, const double *z, const double *vx,
                                const double *vy, const double *vz, const double *m, const double *rho,
                                const double *cs, const double *bal, const double *gs, const double *p,
-                               const double *h, HydroRecF *out) {
+                               const double *h, HydroRecF *out, VsigRec *vout) {
     const int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
+    if (vout) vout[k] = VsigRec{x[k], y[k], z[k], vx[k], vy[k], vz[k], cs[k], 0.0};
     const HydroRec r = make_hydro_rec(x[k], y[k], z[k], vx[k], vy[k], vz[k], m[k], rho[k], cs[k], bal[k], gs[k], p[k],
                                       h[k]);
     HydroRecF f;
@@ -586,16 +588,19 @@ __global__ __launch_bounds__(kWave, 4) void k_hydro_persistent(HydroArgs a) {
 // with the same bound, each candidate's v_sig is formed with the same
 // arithmetic, and the maximum does not depend on the order: dt is the number
 // the one-kernel form gives.
-struct alignas(16) VsigSlot {
-    double x, y, z, vx, vy, vz, cs, pad;
-};
-// the entry's position, velocity and sound speed: from the FP64 hydro records
-// (three 16-byte loads and one 8-byte load), from the precision form's records
-// (position and velocity) and the FP64 sound speeds, or from the arrays
+using VsigSlot = VsigRec;
+// the entry's position, velocity and sound speed: from the dt records (four
+// 16-byte loads), from the FP64 hydro records (three 16-byte loads and one
+// 8-byte load), from the precision form's records (position and velocity)
+// and the FP64 sound speeds, or from the arrays
 __device__ __forceinline__ VsigSlot vsig_load(const HydroArgs &a, int q) {
     VsigSlot r{};
     if (q < 0) return r;
-    if (a.jrec || a.jrecf) {
+    if (a.vrec) {
+        const double2 *p = reinterpret_cast<const double2 *>(a.vrec + q);
+        const double2 u = p[0], v = p[1], w = p[2], c = p[3];
+        r.x = u.x; r.y = u.y; r.z = v.x; r.vx = v.y; r.vy = w.x; r.vz = w.y; r.cs = c.x;
+    } else if (a.jrec || a.jrecf) {
         const double *p = a.jrec ? &a.jrec[q].x : &a.jrecf[q].x;
         const double2 u = reinterpret_cast<const double2 *>(p)[0], v = reinterpret_cast<const double2 *>(p)[1],
                       w = reinterpret_cast<const double2 *>(p)[2];
@@ -804,19 +809,19 @@ void launch_pack_kin(int64_t n, const double *x, const double *y, const double *
 void launch_pack_hydro(int64_t n, const double *x, const double *y, const double *z, const double *vx,
                        const double *vy, const double *vz, const double *m, const double *rho, const double *cs,
                        const double *bal, const double *gs, const double *p, const double *h, HydroRec *out,
-                       hipStream_t s) {
+                       hipStream_t s, VsigRec *vout) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_pack_hydro, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, n, x, y, z, vx, vy, vz, m,
-                       rho, cs, bal, gs, p, h, out);
+                       rho, cs, bal, gs, p, h, out, vout);
     SPH_LAUNCH_CHECK();
 }
 void launch_pack_hydro_f(int64_t n, const double *x, const double *y, const double *z, const double *vx,
                          const double *vy, const double *vz, const double *m, const double *rho, const double *cs,
                          const double *bal, const double *gs, const double *p, const double *h, HydroRecF *out,
-                         hipStream_t s) {
+                         hipStream_t s, VsigRec *vout) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_pack_hydro_f, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, n, x, y, z, vx, vy, vz,
-                       m, rho, cs, bal, gs, p, h, out);
+                       m, rho, cs, bal, gs, p, h, out, vout);
     SPH_LAUNCH_CHECK();
 }
 namespace {
