@@ -205,7 +205,7 @@ struct HydroArgs {
     double *ax, *ay, *az, *du, *dt;
     const HydroRec *jrec = nullptr;  // if set: the j side is read from here (list indices index it)
     const HydroRecF *jrecf = nullptr;  // the precision form's records, read in preference to jrec
-    const VsigRec *vrec = nullptr;     // the dt kernel's records, read in preference to the others
+    const VsigRec *vrec = nullptr;     // the dt kernel's j records (launch_vsig reads only these)
     // persistent scheduling: 8 per-XCD group counters, cleared before the launch
     // (null: one workgroup per group)
     int32_t *sched = nullptr;
@@ -273,9 +273,8 @@ void launch_pack_hydro_f(int64_t n, const double *x, const double *y, const doub
 void launch_dens_fp64(const DensArgs &a, hipStream_t s);
 void launch_drvt_fp64(const DrvtArgs &a, hipStream_t s);
 void launch_hydro_fp64(const HydroArgs &a, hipStream_t s);
-// dt from v_sig_max over each group's list in FP64 (both precision forms): the
-// j side from jrec, else from jrecf's positions and velocities with csj, else
-// from the arrays
+// dt from v_sig_max over each group's list in FP64 (both precision forms),
+// the j side from the dt records a.vrec (packed by launch_pack_hydro[_f])
 void launch_vsig(const HydroArgs &a, hipStream_t s);
 void launch_grav_fp64(const GravArgs &a, hipStream_t s);
 

@@ -589,28 +589,14 @@ __global__ __launch_bounds__(kWave, 4) void k_hydro_persistent(HydroArgs a) {
 // arithmetic, and the maximum does not depend on the order: dt is the number
 // the one-kernel form gives.
 using VsigSlot = VsigRec;
-// the entry's position, velocity and sound speed: from the dt records (four
-// 16-byte loads), from the FP64 hydro records (three 16-byte loads and one
-// 8-byte load), from the precision form's records (position and velocity)
-// and the FP64 sound speeds, or from the arrays
+// the entry's position, velocity and sound speed: four 16-byte loads of its
+// dt record (none for q < 0)
 __device__ __forceinline__ VsigSlot vsig_load(const HydroArgs &a, int q) {
     VsigSlot r{};
     if (q < 0) return r;
-    if (a.vrec) {
-        const double2 *p = reinterpret_cast<const double2 *>(a.vrec + q);
-        const double2 u = p[0], v = p[1], w = p[2], c = p[3];
-        r.x = u.x; r.y = u.y; r.z = v.x; r.vx = v.y; r.vy = w.x; r.vz = w.y; r.cs = c.x;
-    } else if (a.jrec || a.jrecf) {
-        const double *p = a.jrec ? &a.jrec[q].x : &a.jrecf[q].x;
-        const double2 u = reinterpret_cast<const double2 *>(p)[0], v = reinterpret_cast<const double2 *>(p)[1],
-                      w = reinterpret_cast<const double2 *>(p)[2];
-        r.x = u.x; r.y = u.y; r.z = v.x; r.vx = v.y; r.vy = w.x; r.vz = w.y;
-        r.cs = a.jrec ? a.jrec[q].cs : a.csj[q];
-    } else {
-        r.x = a.xj[q]; r.y = a.yj[q]; r.z = a.zj[q];
-        r.vx = a.vxj[q]; r.vy = a.vyj[q]; r.vz = a.vzj[q];
-        r.cs = a.csj[q];
-    }
+    const double2 *p = reinterpret_cast<const double2 *>(a.vrec + q);
+    const double2 u = p[0], v = p[1], w = p[2], c = p[3];
+    r.x = u.x; r.y = u.y; r.z = v.x; r.vx = v.y; r.vy = w.x; r.vz = w.y; r.cs = c.x;
     return r;
 }
 __device__ __forceinline__ void vsig_group(const HydroArgs &a, const int item) {
@@ -878,6 +864,7 @@ void launch_hydro_fp64(const HydroArgs &a, hipStream_t s) {
 // to the kernels of the main stream)
 void launch_vsig(const HydroArgs &a, hipStream_t s) {
     if (a.ngroups <= 0) return;
+    if (!a.vrec) throw Error(-1, "launch_vsig needs the dt records (vrec)");  // -1: SPHGPU_ERR_ARG
     hipLaunchKernelGGL(k_vsig, dim3(a.ngroups), dim3(kWave), 0, s, a);
     SPH_LAUNCH_CHECK();
 }
