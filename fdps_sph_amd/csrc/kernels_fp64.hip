@@ -37,6 +37,27 @@ __device__ __forceinline__ double wpoly_(double s) { return fma(fma(fma(32.0, s,
 __device__ __forceinline__ double dwds_(double s, double o7) { return (s * fma(fma(16.0, s, 7.0), s, 1.0)) * o7; }
 __device__ __forceinline__ double pow7f_(double o) { const double o2 = o * o, o4 = o2 * o2; return (o4 * o2) * o; }
 
+// The polynomials' constants held in registers for the pair loops: v_fma_f64
+// takes no literal operand (and at most one scalar one), so an fma of two
+// constants otherwise rebuilds one of them with two moves in every pair.  Set
+// up once per group; the empty asm statements emit nothing, they only hide
+// the values from constant folding.  The arithmetic is unchanged.
+struct PolyK {
+    double k32 = 32.0, k25 = 25.0, k16 = 16.0, k7 = 7.0;
+    __device__ __forceinline__ PolyK() {
+        asm volatile("" : "+s"(k32));
+        asm volatile("" : "+v"(k25));
+        asm volatile("" : "+s"(k16));
+        asm volatile("" : "+v"(k7));
+    }
+};
+__device__ __forceinline__ double wpoly_(double s, const PolyK &K) {
+    return fma(fma(fma(K.k32, s, K.k25), s, 8.0), s, 1.0);
+}
+__device__ __forceinline__ double dwds_(double s, double o7, const PolyK &K) {
+    return (s * fma(fma(K.k16, s, K.k7), s, 1.0)) * o7;
+}
+
 // Reciprocal of a positive, finite x: the hardware estimate refined by two
 // Newton steps (five instructions against ten for the IEEE divide; the
 // reference's -ffast-math build also divides through reciprocals).
@@ -136,6 +157,7 @@ __device__ __forceinline__ void dens_group(const DensArgs &a, const int item) {
     const int g = group_of(a, item);
     const int i0 = a.gstart[g] + kWave * (item - g * a.chunks), ni = min(kWave, a.gstart[g + 1] - i0);
     if (ni <= 0) return;  // a chunk beyond the group's particles
+    const PolyK PK;  // polynomial constants in registers
     const bool act = lane < ni;
     const int ip = i0 + (act ? lane : 0);
     const double xi = a.xi[ip], yi = a.yi[ip], zi = a.zi[ip], hi = a.hi[ip], mi = a.mi[ip];
@@ -189,7 +211,7 @@ __device__ __forceinline__ void dens_group(const DensArgs &a, const int item) {
             const double dx = J.x - xi, dy = J.y - yi, dz = J.z - zi;
             const double s = dist_(norm2_(dx, dy, dz)).r * invH;  // r / H with 1/H hoisted out of the j loop
             const double o = fmax(1.0 - s, 0.0);
-            dens = fma(J.m, wpoly_(s) * pow8_(o), dens);  // sum of m_j W_ij / norm
+            dens = fma(J.m, wpoly_(s, PK) * pow8_(o), dens);  // sum of m_j W_ij / norm
             ngb += (o > 0.0) ? 1 : 0;
         }
         __syncthreads();
@@ -225,6 +247,7 @@ __device__ __forceinline__ void drvt_group(const DrvtArgs &a, const int item) {
     const int g = group_of(a, item);
     const int i0 = a.gstart[g] + kWave * (item - g * a.chunks), ni = min(kWave, a.gstart[g + 1] - i0);
     if (ni <= 0) return;  // a chunk beyond the group's particles
+    const PolyK PK;  // polynomial constants in registers
     const bool act = lane < ni;
     const int ip = i0 + (act ? lane : 0);
     const double xi = a.xi[ip], yi = a.yi[ip], zi = a.zi[ip];
@@ -293,13 +316,13 @@ __device__ __forceinline__ void drvt_group(const DrvtArgs &a, const int item) {
             const double s = r * invH;
             const double o = fmax(1.0 - s, 0.0);
             const double o7 = pow7f_(o);
-            const double mf = J.m * (dwds_(s, o7) * rcp_(fma(r, H, reg)));  // m_j gradW / (K dr)
+            const double mf = J.m * (dwds_(s, o7, PK) * rcp_(fma(r, H, reg)));  // m_j gradW / (K dr)
             const double dvdr = fma(dvz, drz, fma(dvy, dry, dvx * drx));
             divv = fma(mf, dvdr, divv);                     // -divv / K: sum of m_j dv . gradW / K
             rx = fma(mf, fma(dvy, drz, -(dvz * dry)), rx);  // -rot / K: sum of m_j dv x gradW / K
             ry = fma(mf, fma(dvz, drx, -(dvx * drz)), ry);
             rz = fma(mf, fma(dvx, dry, -(dvy * drx)), rz);
-            gsw = fma(J.m, wpoly_(s) * (o7 * o), gsw);  // sum of m_j W / norm
+            gsw = fma(J.m, wpoly_(s, PK) * (o7 * o), gsw);  // sum of m_j W / norm
             gsf = fma(mf, r2, gsf);                       // sum of m_j dr . gradW / K
         }
         __syncthreads();
@@ -412,6 +435,7 @@ __device__ __forceinline__ void hydro_group(const HydroArgs &a, const int item) 
     const int g = group_of(a, item);
     const int i0 = a.gstart[g] + kWave * (item - g * a.chunks), ni = min(kWave, a.gstart[g + 1] - i0);
     if (ni <= 0) return;  // a chunk beyond the group's particles
+    const PolyK PK;  // polynomial constants in registers
     const bool act = lane < ni;
     const int ip = i0 + (act ? lane : 0);
     const double xi = a.xi[ip], yi = a.yi[ip], zi = a.zi[ip];
@@ -522,8 +546,8 @@ __device__ __forceinline__ void hydro_group(const HydroArgs &a, const int item) 
             const double si = d.r * invHi, sj = d.r * J.invH;
             const double ai = fma(d.r, Hi, regi), aj = fma(d.r, J.H, J.reg);
             const double rab = rcp_(ai * aj);
-            const double Fi = (normi * dwds_(si, pow7f_(fmax(1.0 - si, 0.0)))) * (aj * rab);
-            const double Fj = (J.norm * dwds_(sj, pow7f_(fmax(1.0 - sj, 0.0)))) * (ai * rab);
+            const double Fi = (normi * dwds_(si, pow7f_(fmax(1.0 - si, 0.0)), PK)) * (aj * rab);
+            const double Fj = (J.norm * dwds_(sj, pow7f_(fmax(1.0 - sj, 0.0)), PK)) * (ai * rab);
             const double Fm = fma(Fj, J.gs, Fi * gsi);  // 2 (gradW_i gs_i + gradW_j gs_j) / 2 / (K dr)
             const double mj = J.m;
             C = mj * fma(AVh, Fm, fma(J.A, Fj, Ai * Fi));
@@ -697,6 +721,13 @@ __device__ __forceinline__ void grav_tile_(const double4 *sp, int nt, double xi,
     // y^3 (1 - e)^(-3/2) with e = 1 - x y^2, whose series 1 + 3e/2 + 15e^2/8
     // leaves O(e^3) ~ 1e-21 (|e| <= 2^-23): 18 instructions per interaction
     // against 19 through a refined 1/r.
+    // The series' constants stay in registers: v_fma_f64 takes no literal
+    // operand, and the compiler otherwise rebuilds 1.5 with two moves in every
+    // interaction (20 instructions instead of 18).  The empty asm statements
+    // emit nothing; they only keep the values opaque to constant folding.
+    double c15 = 1.5, c1875 = 1.875;
+    asm volatile("" : "+v"(c15));
+    asm volatile("" : "+s"(c1875));
 #pragma unroll 8
     for (int jj = 0; jj < nt; ++jj) {
         const double4 pj = sp[jj];  // x, y, z, G m
@@ -706,7 +737,7 @@ __device__ __forceinline__ void grav_tile_(const double4 *sp, int nt, double xi,
         const double y2 = y * y;
         const double e = fma(-x2, y2, 1.0);
         const double y3 = y2 * y;
-        const double g3 = pj.w * fma(y3, e * fma(1.875, e, 1.5), y3);  // G m_j / (r^2 + eps^2)^(3/2)
+        const double g3 = pj.w * fma(y3, e * fma(c1875, e, c15), y3);  // G m_j / (r^2 + eps^2)^(3/2)
         ax = fma(-g3, drx, ax);
         ay = fma(-g3, dry, ay);
         az = fma(-g3, drz, az);
