@@ -582,6 +582,9 @@ def main():
                          "0: steps launched kernel by kernel")
     ap.add_argument("--vsig-split", type=int, choices=[0, 1, 2, 3], default=1, metavar="MASK",
                     help="precision forms (1 FP64, 2 mixed) whose hydro dt runs as a kernel of its own")
+    ap.add_argument("--hydro-compact", type=int, choices=[0, 1], default=1,
+                    help="1 (default): where dt runs in a kernel of its own, the hydro force terms read the "
+                         "symmetric lists compacted to the entries within reach of either support; 0: whole lists")
     ap.add_argument("--walk-waves", type=int, default=0,
                     help="persistent waves of each interaction-list walk (0: the library's default, 4096)")
     ap.add_argument("--eos-density-fp32", action="store_true",
@@ -627,7 +630,7 @@ def main():
 
     params = {"theta": args.theta, "nsys": nsys, "eos_density_fp64": 0 if args.eos_density_fp32 else 1,
               "persistent": args.persistent, "graph": args.graph, "walk_waves": args.walk_waves,
-              "vsig_split": args.vsig_split}
+              "vsig_split": args.vsig_split, "hydro_compact": args.hydro_compact}
     extra = {}
     if args.workload == "batch64":
         from fdps_sph_amd import autotune
@@ -683,6 +686,7 @@ def main():
                    "n_leaf": params.get("n_leaf", 16), "persistent": params.get("persistent", 0),
                    "list_front_lds": params.get("list_front_lds", 512), "graph": args.graph,
                    "walk_waves": params.get("walk_waves", 0) or 4096, "vsig_split": params.get("vsig_split", 1),
+                   "hydro_compact": params.get("hydro_compact", 1),
                    "eos_density": "fp32" if args.eos_density_fp32 else "fp64"},
         "roofline": roofline,
         main_prec: block,

@@ -39,7 +39,8 @@ class SimParams(C.Structure):
                 ("ext_force_until", C.c_double), ("nsys", C.c_int32), ("list_cap_sph", C.c_int32),
                 ("list_cap_epj", C.c_int32), ("list_cap_spj", C.c_int32), ("eos_density_fp64", C.c_int32),
                 ("list_cap_front", C.c_int32), ("list_front_lds", C.c_int32), ("persistent", C.c_int32),
-                ("graph", C.c_int32), ("walk_waves", C.c_int32), ("vsig_split", C.c_int32)]
+                ("graph", C.c_int32), ("walk_waves", C.c_int32), ("vsig_split", C.c_int32),
+                ("hydro_compact", C.c_int32)]
 
 
 class KernelStat(C.Structure):

@@ -190,6 +190,7 @@ struct SimState {
     DevBuf<HydroRec> hrec;     // j records of the hydro pass
     DevBuf<HydroRecF> hrecf;   // the same in the precision form
     DevBuf<VsigRec> vrec;      // j records of the dt kernel (vsig_split)
+    DevBuf<int32_t> hidx, hcnt; // symmetric lists compacted for the hydro force terms (hydro_compact)
     DevBuf<double4> kpos, kvel; // j records of the density and derivative passes
     int64_t gather_builds = 0; // gather-list constructions since init
     int64_t force_evals = 0;   // force evaluations completed since init
@@ -601,6 +602,7 @@ std::vector<int64_t> step_sig(const sphgpu_ctx *c, const SimState &S) {
     v.push_back(S.prm.persistent);
     v.push_back(S.prm.eos_density_fp64);
     v.push_back(S.prm.vsig_split);
+    v.push_back(S.prm.hydro_compact);
     v.push_back(S.tree.gcap);
     for (int l = 0; l < S.tree.nlevels; ++l) v.push_back(S.tree.cap_lv[l]);
     for (const Lists *L : {&S.lsph, &S.lgrav}) {
@@ -718,10 +720,10 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
         hipLaunchKernelGGL(k_fill_ll, dim3(1), dim3(64), 0, s, (int64_t) 4 * kStatKinds, S.kcount_try.get());
         SPH_LAUNCH_CHECK();
     }
-    auto kbegin = [&](const Lists *Lp, bool spj, int kind) {
+    auto kbegin = [&](const Lists *Lp, bool spj, int kind, const int32_t *cnt = nullptr) {
         if (!S.stats) return;
         S.kkind[S.kslot] = kind;
-        hipLaunchKernelGGL(k_list_counts, dim3(nblk(G, 256)), dim3(256), 0, s, gcount, gs, Lp->cnt.get(),
+        hipLaunchKernelGGL(k_list_counts, dim3(nblk(G, 256)), dim3(256), 0, s, gcount, gs, cnt ? cnt : Lp->cnt.get(),
                            spj ? Lp->scnt.get() : nullptr, S.kcount_try.get() + 4 * kind);
         SPH_LAUNCH_CHECK();
         SPH_HIP(hipEventRecord(S.kev[S.kslot][0], s));
@@ -816,7 +818,18 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
     // gather-list bits and construction counter, before the symmetric lists reuse L
     launch_stash(StashArgs{{L.err.get(), nullptr, nullptr, nullptr}, {2, 0, 0, 0}, stash + kStGather}, s2);
     enqueue_lists(S.tree, S.tp, P, 1, L, 1.0, s2);
-    signal(S.xev[3], s2);  // symmetric lists
+    // with dt in a kernel of its own, the force terms read the symmetric lists
+    // compacted to the entries within reach of either support; dt still runs
+    // over the whole lists, after the compaction on this stream, beside the
+    // gravity and hydro kernels
+    const bool hcompact = split && S.prm.hydro_compact && S.tree.max_group <= kWave;
+    int32_t *hidx = nullptr, *hcnt = nullptr;
+    if (hcompact) {
+        hidx = S.hidx.ensure((size_t) G * (size_t) L.cap);
+        hcnt = S.hcnt.ensure((size_t) std::max(G, 1));
+        enqueue_sym_compact(S.tree, P, L, kpos, hidx, hcnt, s2);
+    }
+    signal(S.xev[3], s2);  // symmetric lists (and their compacted form)
     auto hydro_args = [&]() {
         HydroArgs a{G, gs, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal,
                     P.x, P.y, P.z, P.vx, P.vy, P.vz, P.h, P.rho, P.p, P.cs, P.gs, P.bal, P.m,
@@ -859,8 +872,9 @@ void forces_enqueue(sphgpu_ctx *c, SimState &S, bool timed, bool restore_h) {
     {
         await(s, S.xev[3]);
         HydroArgs a = hydro_args();
+        if (hcompact) a.L = ListRef{hidx, nullptr, hcnt, L.cap};
         a.sched = sched_for(4);
-        kbegin(&L, false, 2);
+        kbegin(&L, false, 2, hcompact ? hcnt : nullptr);
         if (mixed) launch_hydro_mixed(a, s);
         else launch_hydro_fp64(a, s);
         kend();
@@ -1350,6 +1364,7 @@ void sphgpu_sim_default_params(sphgpu_sim_params *p) {
     p->persistent = 0;
     p->graph = 0;
     p->vsig_split = 1;
+    p->hydro_compact = 1;
 }
 
 int32_t sphgpu_sim_init(sphgpu_ctx *c, const sphgpu_sim_params *prm, int64_t n, const double *pos, const double *vel,
@@ -1366,6 +1381,8 @@ int32_t sphgpu_sim_init(sphgpu_ctx *c, const sphgpu_sim_params *prm, int64_t n, 
             throw Error(SPHGPU_ERR_ARG, "n_group must be in [1, SPHGPU_MAX_GROUP]");
         if (!(prm->theta >= 0)) throw Error(SPHGPU_ERR_ARG, "theta must be >= 0");
         if (prm->vsig_split < 0 || prm->vsig_split > 3) throw Error(SPHGPU_ERR_ARG, "vsig_split must be in [0, 3]");
+        if (prm->hydro_compact < 0 || prm->hydro_compact > 1)
+            throw Error(SPHGPU_ERR_ARG, "hydro_compact must be 0 or 1");
         const int nsys = prm->nsys > 0 ? prm->nsys : 1;
         if (nsys > 1 && !system) throw Error(SPHGPU_ERR_ARG, "nsys > 1 needs a system array");
         for (int64_t i = 0; i < n; ++i) {
@@ -1639,7 +1656,9 @@ int32_t sphgpu_sim_lists(sphgpu_ctx *c, int32_t kind, int64_t *n_groups, int64_t
     return guard([&] {
         check_ctx(c);
         SimState &S = sim_of(c);
-        if (kind < 0 || kind > 3) throw Error(SPHGPU_ERR_ARG, "kind must be 0, 1, 2 or 3");
+        if (kind < 0 || kind > 4) throw Error(SPHGPU_ERR_ARG, "kind must be in [0, 4]");
+        if (kind == 4 && S.tree.max_group > kWave)
+            throw Error(SPHGPU_ERR_ARG, "kind 4 needs groups of at most 64 particles");
         if (!n_groups || !n_j || !n_s) throw Error(SPHGPU_ERR_ARG, "null size output");
         hipStream_t s = c->stream;
         // The particles are in key order after every force evaluation (init and
@@ -1654,13 +1673,21 @@ int32_t sphgpu_sim_lists(sphgpu_ctx *c, int32_t kind, int64_t *n_groups, int64_t
             build_lists_sph(S.tree, S.tp, S.P, 0, L, s, kGatherMargin);
             enqueue_gather_compact(S.tree, S.P, L, kGatherMargin, s);
         } else {
-            if (kind == 1) tree_update_maxH(S.tree, S.P.h, s);
-            build_lists_sph(S.tree, S.tp, S.P, kind, L, s);
+            if (kind >= 1) tree_update_maxH(S.tree, S.P.h, s);
+            build_lists_sph(S.tree, S.tp, S.P, kind == 4 ? 1 : kind, L, s);
+        }
+        const int32_t *idx_d = L.idx.get(), *cnt_d = L.cnt.get();
+        if (kind == 4) {
+            const size_t G = (size_t) std::max<int64_t>(S.tree.gcap, 1);
+            int32_t *hidx = S.hidx.ensure(G * (size_t) L.cap), *hcnt = S.hcnt.ensure(G);
+            enqueue_sym_compact(S.tree, S.P, L, nullptr, hidx, hcnt, s);
+            idx_d = hidx;
+            cnt_d = hcnt;
         }
         SPH_HIP(hipStreamSynchronize(s));
         const int64_t G = S.tree.ngroups;
         std::vector<int32_t> cnt(G), scnt(G);
-        SPH_HIP(hipMemcpy(cnt.data(), L.cnt.get(), sizeof(int32_t) * G, hipMemcpyDeviceToHost));
+        SPH_HIP(hipMemcpy(cnt.data(), cnt_d, sizeof(int32_t) * G, hipMemcpyDeviceToHost));
         if (kind == 2) SPH_HIP(hipMemcpy(scnt.data(), L.scnt.get(), sizeof(int32_t) * G, hipMemcpyDeviceToHost));
         int64_t tj = 0, ts = 0;
         for (int64_t g = 0; g < G; ++g) {
@@ -1673,7 +1700,7 @@ int32_t sphgpu_sim_lists(sphgpu_ctx *c, int32_t kind, int64_t *n_groups, int64_t
         if (!group_start && !j_off && !j_idx && !s_off && !s_idx) return;
         if (group_start) SPH_HIP(hipMemcpy(group_start, S.tree.gstart.get(), sizeof(int32_t) * (G + 1), hipMemcpyDeviceToHost));
         std::vector<int32_t> all((size_t) G * L.cap);
-        SPH_HIP(hipMemcpy(all.data(), L.idx.get(), sizeof(int32_t) * all.size(), hipMemcpyDeviceToHost));
+        SPH_HIP(hipMemcpy(all.data(), idx_d, sizeof(int32_t) * all.size(), hipMemcpyDeviceToHost));
         int64_t o = 0;
         for (int64_t g = 0; g < G; ++g) {
             if (j_off) j_off[g] = o;

@@ -429,6 +429,12 @@ bool lists_grow_bits(Lists &L, int32_t err);
 // the construction it follows).  pos: optional packed (x, y, z, m) records.
 void enqueue_gather_compact(const Tree &t, const PArrays &P, Lists &L, double hfac, hipStream_t s,
                             const int32_t *need = nullptr, const double4 *pos = nullptr);
+// Writes to (oidx, ocnt), capacity L.cap per group, each group's symmetric
+// list reduced in order to the entries within max(max_i 2.5 h_i, 2.5 h_j) of
+// some particle of the group: every pair inside either support is kept.
+// Groups of at most 64 particles; pos: packed (x, y, z, m) records or null.
+void enqueue_sym_compact(const Tree &t, const PArrays &P, const Lists &L, const double4 *pos, int32_t *oidx,
+                         int32_t *ocnt, hipStream_t s);
 // Synchronous forms: capacities grow until the lists fit.
 bool build_lists_sph(const Tree &t, const TreeParams &tp, const PArrays &P, int kind, Lists &L, hipStream_t s,
                      double hfac = 1.0);

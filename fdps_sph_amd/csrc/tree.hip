@@ -1055,6 +1055,115 @@ __global__ __launch_bounds__(kWave, 8) void k_list_compact(const int32_t *gcount
     if (lane == 0) cnt[g] = count;
 }
 
+// mfma_dot_any_ with a threshold per entry instead of per particle: the last
+// K pair is (w_i, 1) on the particle side against (1, -Q_j) on the entry side,
+// so each accumulator holds |o_j - o_i|^2 - Q_j (w = |o|^2).  Returns the
+// wave-uniform mask of entries < nt with |o_j - o_i|^2 < Q_j for some lane
+// (wi = +inf for a lane without a particle admits nothing).  qs[jj] = -Q_j.
+__device__ __forceinline__ uint64_t mfma_dot_any_q_(const float4 *q, const float *qs, int nt, float tx, float ty,
+                                                    float tz, float wi, float px, float py, float pz, float pw) {
+    const int lane = threadIdx.x, h = lane >> 5, m = lane & 31;
+    uint64_t any = 0;
+#pragma unroll
+    for (int jb = 0; jb < 2; ++jb) {
+        const float4 o = q[32 * jb + m];
+        const float b0 = h ? o.y : o.x, b1 = h ? o.w : o.z, b2 = h ? qs[32 * jb + m] : 1.f;
+        float mn = INFINITY;
+#pragma unroll
+        for (int ib = 0; ib < 2; ++ib) {
+            const bool own = (ib == h);  // particle 32 ib + m is this lane's own, else its partner's
+            const float a0 = h ? (own ? ty : py) : (own ? tx : px);
+            const float a1 = h ? 1.f : (own ? tz : pz);
+            const float a2 = h ? 1.f : (own ? wi : pw);
+            f32x16_ c = f32x16_{};
+            c = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, b2, c, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) mn = fminf(mn, c[r]);
+        }
+        const uint64_t b = __ballot(mn < 0.f);
+        any |= (uint64_t) ((uint32_t) b | (uint32_t) (b >> 32)) << (32 * jb);
+    }
+    if (nt < kWave) any &= (1ULL << nt) - 1;
+    return any;
+}
+
+// Compaction of the symmetric lists for the hydro force terms.  A symmetric
+// list holds every particle whose support or the group's reaches the group's
+// tree cell; at 1M about 40 % of it lies beyond both supports of every pair
+// with the group (CPU count on the impact), staged, masked and skipped by the
+// hydro kernel.  This writes to a second list, in list order, the entries j
+// with |x_j - x_i| < max(max_i H_i, H_j) for some i of the group (H = 2.5 h,
+// with the hydro kernel's 1e-4 room), a superset of the pairs inside either
+// support, so the force sums take the same nonzero terms.  The threshold is
+// the entry's (dot_cut_j_: room for the FP32 offsets' rounding, with the
+// group's largest |o_i|_1).  Out of place: the dt kernel reads the whole
+// list meanwhile, since v_sig_max runs over every entry (src/force.h:122-124).
+// Groups of at most 64 particles; pos: packed records or null (x, y, z read).
+__global__ __launch_bounds__(kWave, 8) void k_list_compact_sym(const int32_t *gcount, const int32_t *gstart,
+                                                            const double *x, const double *y, const double *z,
+                                                            const double *h, const double4 *pos,
+                                                            const int32_t *idx, const int32_t *cnt, int64_t cap,
+                                                            int32_t *oidx, int32_t *ocnt) {
+    const int G = *gcount;
+    if ((int) blockIdx.x >= G) return;
+    const int g = xcd_remap((int) blockIdx.x, G);
+    const int lane = threadIdx.x;
+    const int i0 = gstart[g], ni = gstart[g + 1] - i0;
+    const bool act = lane < ni;
+    const int p = i0 + (act ? lane : 0);
+    const double xi = x[p], yi = y[p], zi = z[p];
+    const float Hmax = wave_maxf(act ? (float) (kSupport * h[p]) * 1.0001f : 0.f);
+    const double cx = 0.5 * (wave_min(act ? xi : INFINITY) + wave_max(act ? xi : -INFINITY));
+    const double cy = 0.5 * (wave_min(act ? yi : INFINITY) + wave_max(act ? yi : -INFINITY));
+    const double cz = 0.5 * (wave_min(act ? zi : INFINITY) + wave_max(act ? zi : -INFINITY));
+    const float oxi = (float) (xi - cx), oyi = (float) (yi - cy), ozi = (float) (zi - cz);
+    const float wi = act ? fmaf(oxi, oxi, fmaf(oyi, oyi, ozi * ozi)) : INFINITY;
+    const float n1max = wave_maxf(act ? norm1_(oxi, oyi, ozi) : 0.f);
+    const float tx = -2.f * oxi, ty = -2.f * oyi, tz = -2.f * ozi;
+    const float ptx = __shfl_xor(tx, 32, kWave), pty = __shfl_xor(ty, 32, kWave), ptz = __shfl_xor(tz, 32, kWave);
+    const float pwi = __shfl_xor(wi, 32, kWave);
+    const int32_t *list = idx + (int64_t) g * cap;
+    int32_t *out = oidx + (int64_t) g * cap;
+    const int nj = cnt[g];
+    const uint64_t lt = (1ULL << lane) - 1;
+    __shared__ float4 so[kWave];
+    __shared__ float sq[kWave];
+    // software-pipelined: the records of chunk c+1 and the list index of chunk
+    // c+2 are in flight while chunk c is tested; a lane past the end loads
+    // particle 0
+    auto load = [&](int q) { return pos ? pos[q] : make_double4(x[q], y[q], z[q], 0.0); };
+    int q = (lane < nj) ? list[lane] : 0;
+    double4 r = load(q);
+    double hq = h[q];
+    int qn = (kWave + lane < nj) ? list[kWave + lane] : 0;
+    int count = 0;
+    for (int base = 0; base < nj; base += kWave) {
+        const int k = base + lane;
+        if (k < nj) {
+            const float ox = (float) (r.x - cx), oy = (float) (r.y - cy), oz = (float) (r.z - cz);
+            so[lane] = make_float4(ox, oy, oz, fmaf(ox, ox, fmaf(oy, oy, oz * oz)));
+            sq[lane] = -dot_cut_j_(fmaxf(Hmax, (float) (kSupport * hq) * 1.0001f), norm1_(ox, oy, oz), n1max);
+        }
+        const int qk = q;
+        if (base + kWave < nj) {
+            q = qn;
+            r = load(q);
+            hq = h[q];
+            qn = (base + 2 * kWave + lane < nj) ? list[base + 2 * kWave + lane] : 0;
+        }
+        __syncthreads();
+        const uint64_t kept = mfma_dot_any_q_(so, sq, min(kWave, nj - base), tx, ty, tz, wi, ptx, pty, ptz, pwi);
+        const bool keep = (k < nj) && ((kept >> lane) & 1);
+        const uint64_t b = __ballot(keep);
+        if (keep) out[count + __popcll(b & lt)] = qk;
+        count += __popcll(b);
+        __syncthreads();
+    }
+    if (lane == 0) ocnt[g] = count;
+}
+
 // The same for groups of more than 64 particles: the group's radius and box
 // come from all its particles, read 64 at a time, and an entry is kept when
 // some particle of any of those waves admits it.
@@ -1450,6 +1559,17 @@ void enqueue_gather_compact(const Tree &t, const PArrays &P, Lists &L, double hf
     hipLaunchKernelGGL(t.max_group > kWave ? k_list_compact_multi : k_list_compact, dim3((unsigned) G), dim3(kWave), 0,
                        s, t.info.get() + kInfoG, t.gstart.get(), P.x, P.y, P.z, P.h, pos, hfac, L.idx.get(),
                        L.cnt.get(), L.cap, L.crad.get(), need);
+    SPH_LAUNCH_CHECK();
+}
+
+void enqueue_sym_compact(const Tree &t, const PArrays &P, const Lists &L, const double4 *pos, int32_t *oidx,
+                         int32_t *ocnt, hipStream_t s) {
+    if (L.kind != 1) throw Error(-1, "enqueue_sym_compact: the lists held are not symmetric lists");
+    if (t.max_group > kWave) throw Error(-1, "enqueue_sym_compact: groups of more than 64 particles");
+    const int64_t G = t.gcap;
+    if (G <= 0) return;
+    hipLaunchKernelGGL(k_list_compact_sym, dim3((unsigned) G), dim3(kWave), 0, s, t.info.get() + kInfoG,
+                       t.gstart.get(), P.x, P.y, P.z, P.h, pos, L.idx.get(), L.cnt.get(), L.cap, oidx, ocnt);
     SPH_LAUNCH_CHECK();
 }
 

@@ -213,6 +213,15 @@ typedef struct {
                                 force terms only; default 1.  The FP64 form's results are
                                 identical either way; the precision form's dt becomes the
                                 FP64 dt of its state. */
+    int32_t hydro_compact;   /* 1 (default): where dt runs in its own kernel (vsig_split) and
+                                groups hold at most 64 particles, the hydro force terms read
+                                each symmetric list reduced, in order, to the entries within
+                                max(max_i 2.5 h_i, 2.5 h_j) of some particle of the group (every
+                                pair inside either support; dt still runs over the whole list,
+                                after the compaction on the second stream, beside the gravity
+                                and hydro kernels); 0: the whole lists.  The force sums take
+                                the same terms in another order (results move in the last
+                                bits). */
 } sphgpu_sim_params;
 
 void sphgpu_sim_default_params(sphgpu_sim_params *p);
@@ -289,7 +298,10 @@ int32_t sphgpu_sim_kernel_stats(sphgpu_ctx *ctx, sphgpu_kernel_stat *out, int32_
  * the tree first).  kind: 0 gather (density/derivative), 1 symmetric
  * (hydro), 2 gravity, 3 the gather lists as a force evaluation uses them:
  * built with the 8 % margin on h and compacted to the entries within R_g of
- * some particle of the group, R_g = 1.08 max_i 2.5 h_i.  First call with NULL
+ * some particle of the group, R_g = 1.08 max_i 2.5 h_i, 4 the symmetric
+ * lists as the hydro force terms read them with hydro_compact: reduced, in
+ * order, to the entries within max(max_i 2.5 h_i, 2.5 h_j) of some particle
+ * of the group (groups of at most 64 particles).  First call with NULL
  * arrays to get the sizes. */
 int32_t sphgpu_sim_lists(sphgpu_ctx *ctx, int32_t kind, int64_t *n_groups, int64_t *n_j, int64_t *n_s,
                          int32_t *group_start, int64_t *j_off, int32_t *j_idx, int64_t *s_off, int32_t *s_idx);

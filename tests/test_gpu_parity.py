@@ -259,6 +259,42 @@ def test_compacted_gather_lists(ctx64, small_impact):
     assert kept_total < box_total
 
 
+def test_compacted_symmetric_lists(ctx64, small_impact):
+    """The symmetric lists as the hydro force terms read them (export kind 4):
+    each group's list is an ordered subsequence of its symmetric list (kind 1)
+    that keeps every particle inside either support of some member
+    (|x_i - x_j| < 2.5 max(h_i, h_j), FP64 distances) and nothing beyond
+    max(max_i 2.5 h_i, 2.5 h_j) plus the FP32 test's room."""
+    _print("[compacted symmetric lists n=2000]")
+    sim = Simulation(ctx64, small_impact)
+    full, comp = sim.lists(1), sim.lists(4)
+    st = sim.state()
+    x, h = st["pos"], st["smth"]
+    gs = comp["group_start"]
+    assert np.array_equal(gs, full["group_start"])
+    kept_total = full_total = 0
+    for g in range(len(gs) - 1):
+        i = np.arange(gs[g], gs[g + 1])
+        lst = full["j_idx"][full["j_off"][g]:full["j_off"][g + 1]]
+        got = comp["j_idx"][comp["j_off"][g]:comp["j_off"][g + 1]]
+        pos = np.searchsorted(lst, got) if np.all(np.diff(lst) > 0) else None
+        if pos is not None:  # an ordered subsequence of an ascending list
+            assert np.all(np.diff(got) > 0) and np.all(lst[np.minimum(pos, len(lst) - 1)] == got), g
+        else:
+            it = iter(lst.tolist())
+            assert all(any(v == w for w in it) for v in got.tolist()), g
+        d = np.sqrt(((x[lst][None, :, :] - x[i][:, None, :]) ** 2).sum(axis=2))  # (n_i, n_list)
+        H = 2.5 * np.maximum(h[i][:, None], h[lst][None, :])
+        need = lst[np.any(d < H * (1 - 1e-9), axis=0)]
+        assert np.all(np.isin(need, got)), g
+        reach = np.maximum(2.5 * h[i].max(), 2.5 * h[lst]) * (1 + 2e-4)
+        assert np.all(np.isin(got, lst[d.min(axis=0) <= reach])), g
+        kept_total += len(got)
+        full_total += len(lst)
+    _print(f"  kept {kept_total} of {full_total} symmetric-list entries")
+    assert kept_total < full_total
+
+
 # ------------------------------------------------------------------ multiwalk boundary
 def test_two_tags_in_flight_and_empty_lists(ctx64, oracle):
     """FDPS double-buffers multiwalk batches (tag_max > 1): batches of

@@ -11,6 +11,13 @@ Checked on the iron/dunite impact (20k particles, 200 x 200 tables):
   (every field within 1e-12, dt included);
 * the precision form with the split (its dt then formed in FP64): per-step
   parity against the FP64 oracle at the precision form's bar.
+
+With the split, the hydro kernel can read the symmetric lists compacted to
+the entries within reach of either support (`hydro_compact`, the default)
+while dt runs over the whole lists: the force sums then take the same
+nonzero terms in another order, so the step is held to the oracle per step
+(both forms), gives the uncompacted step's dt bit for bit at the initial
+force evaluation, and replays from recorded graphs bit for bit.
 """
 import numpy as np
 import pytest
@@ -55,6 +62,7 @@ def test_split_fp64_bit_identical(ctxv, impact):
     ctxv.set_precision("fp64")
     ref, tref = _run(ctxv, impact, 4, vsig_split=0, graph=0)
     for params in ({"vsig_split": 1, "graph": 0}, {"vsig_split": 1, "graph": 1}, {"vsig_split": 3, "graph": 0}):
+        params["hydro_compact"] = 0
         print(f"[fp64 {params}]", flush=True)
         got, tgot = _run(ctxv, impact, 4, **params)
         for f in FIELDS:
@@ -62,20 +70,43 @@ def test_split_fp64_bit_identical(ctxv, impact):
         assert np.array_equal(tgot[0], tref[0]) and np.array_equal(tgot[1], tref[1])
 
 
-def test_split_fp64_oracle(ctxv, oracle, tabsv, impact):
-    print("[iron/dunite 20k fp64 vsig_split=1]", flush=True)
+@pytest.mark.parametrize("compact", [0, 1])
+def test_split_fp64_oracle(ctxv, oracle, tabsv, impact, compact):
+    print(f"[iron/dunite 20k fp64 vsig_split=1 hydro_compact={compact}]", flush=True)
     ctxv.set_precision("fp64")
     prm = oracle.params()
-    sim = Simulation(ctxv, impact, vsig_split=1)
+    sim = Simulation(ctxv, impact, vsig_split=1, hydro_compact=compact)
     run_per_step(sim, oracle, impact, tabsv, prm, 2, fp64_check(tabsv, prm))
 
 
-def test_split_precision_form(ctxv, oracle, tabsv, impact):
-    print("[iron/dunite 20k precision form vsig_split=2]", flush=True)
+def test_compact_fp64_dt_and_replay(ctxv, impact):
+    """Compacted lists: the initial evaluation's dt equals the whole lists' bit
+    for bit (dt reads the whole lists either way), and recorded steps replay
+    the direct steps bit for bit."""
+    ctxv.set_precision("fp64")
+    dts = []
+    for compact in (0, 1):
+        print(f"[fp64 initial evaluation hydro_compact={compact}]", flush=True)
+        sim = Simulation(ctxv, impact, vsig_split=1, hydro_compact=compact)
+        dts.append((sim.get("DT"), sim.get("ID"), sim.time()))
+    assert np.array_equal(dts[0][1], dts[1][1])
+    assert np.array_equal(dts[0][0], dts[1][0])
+    assert np.array_equal(dts[0][2][0], dts[1][2][0])
+    ref, tref = _run(ctxv, impact, 4, vsig_split=1, hydro_compact=1, graph=0)
+    print("[fp64 hydro_compact=1 recorded]", flush=True)
+    got, tgot = _run(ctxv, impact, 4, vsig_split=1, hydro_compact=1, graph=1)
+    for f in FIELDS:
+        assert np.array_equal(got[f], ref[f]), f
+    assert np.array_equal(tgot[0], tref[0]) and np.array_equal(tgot[1], tref[1])
+
+
+@pytest.mark.parametrize("compact", [0, 1])
+def test_split_precision_form(ctxv, oracle, tabsv, impact, compact):
+    print(f"[iron/dunite 20k precision form vsig_split=2 hydro_compact={compact}]", flush=True)
     ctxv.set_precision("mixed")
     try:
         prm = oracle.params()
-        sim = Simulation(ctxv, impact, vsig_split=2)
+        sim = Simulation(ctxv, impact, vsig_split=2, hydro_compact=compact)
         run_per_step(sim, oracle, impact, tabsv, prm, 2, mixed_check(tabsv, prm), dt_tol=1e-5)
     finally:
         ctxv.set_precision("fp64")
@@ -84,3 +115,5 @@ def test_split_precision_form(ctxv, oracle, tabsv, impact):
 def test_split_params_checked(ctxv, impact):
     with pytest.raises(Exception):
         Simulation(ctxv, impact, vsig_split=4)
+    with pytest.raises(Exception):
+        Simulation(ctxv, impact, hydro_compact=2)
